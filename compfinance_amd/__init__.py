@@ -1,0 +1,479 @@
+"""MI355X Monte Carlo pricing and AAD-risk engine - Python binding.
+
+Thin ctypes layer over the C ABI of ``libcfamd.so`` (``include/cfamd.h``).
+Names follow the reference library (asavine/CompFinance): models
+``BlackScholes`` / ``Dupire``, products ``European`` / ``UOC`` /
+``Europeans``, ``NumericalParam``, the simulation templates
+``mcParallelSimul`` / ``mcParallelSimulAAD`` (mcBase.h:314-400, :564-753), the
+store functions ``putBlackScholes`` ... ``putEuropeans`` (store.h:38-205) and
+the entry points ``value``, ``AADriskOne``, ``AADriskAggregate``,
+``dupireAADRisk`` (main.h:43-411).
+
+Every computation runs in the HIP kernels of the library; there is no CPU
+fallback.  Importing works without a GPU (labels, timelines and validation
+are host code); simulations need one.
+"""
+import ctypes
+import os
+from collections import OrderedDict
+
+import numpy as np
+
+__all__ = [
+    "FP64", "FP32", "AAD_CHECKPOINT", "AAD_RECOMPUTE",
+    "BlackScholes", "Dupire", "European", "UOC", "Europeans", "NumericalParam",
+    "mcParallelSimul", "mcParallelSimulAAD", "SimulResults", "AADSimulResults",
+    "putBlackScholes", "putDupire", "putEuropean", "putBarrier", "putEuropeans",
+    "value", "AADriskOne", "AADriskAggregate", "dupireAADRisk", "clearStore",
+    "CfError", "lib", "last_kernel_stats", "LIB_PATH",
+]
+
+FP64, FP32 = 0, 1
+AAD_CHECKPOINT, AAD_RECOMPUTE = 0, 1
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcfamd.so")
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class cf_model(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("spot", ctypes.c_double),
+        ("vol", ctypes.c_double), ("rate", ctypes.c_double), ("div", ctypes.c_double),
+        ("spot_measure", ctypes.c_int),
+        ("n_spots", ctypes.c_int), ("n_times", ctypes.c_int),
+        ("spots", _c_double_p), ("times", _c_double_p), ("vols", _c_double_p),
+        ("max_dt", ctypes.c_double),
+    ]
+
+
+class cf_product(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("strike", ctypes.c_double), ("exercise", ctypes.c_double),
+        ("settlement", ctypes.c_double), ("barrier", ctypes.c_double), ("maturity", ctypes.c_double),
+        ("monitor_freq", ctypes.c_double), ("smooth", ctypes.c_double), ("call_put", ctypes.c_int),
+        ("n_options", ctypes.c_int), ("maturities", _c_double_p), ("strikes", _c_double_p),
+    ]
+
+
+class cf_num(ctypes.Structure):
+    _fields_ = [
+        ("parallel", ctypes.c_int), ("use_sobol", ctypes.c_int), ("num_path", ctypes.c_int),
+        ("seed1", ctypes.c_int), ("seed2", ctypes.c_int), ("precision", ctypes.c_int),
+        ("aad_mode", ctypes.c_int), ("first_path", ctypes.c_int),
+    ]
+
+
+class cf_kernel_stats(ctypes.Structure):
+    _fields_ = [
+        ("main_kernel_ms", ctypes.c_double), ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("grid_blocks", ctypes.c_int), ("block_threads", ctypes.c_int), ("lds_bytes", ctypes.c_int),
+        ("path_steps", ctypes.c_int64),
+    ]
+
+
+class CfError(RuntimeError):
+    """Error raised by the engine (the reference raises std::runtime_error)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded ``libcfamd.so``; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m compfinance_amd.build` "
+                "(the engine has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        sig = {
+            "cf_num_payoffs": (ctypes.c_int, [P(cf_product)]),
+            "cf_num_params": (ctypes.c_int, [P(cf_model)]),
+            "cf_payoff_label": (ctypes.c_int, [P(cf_product), ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+            "cf_param_label": (ctypes.c_int, [P(cf_model), ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
+            "cf_product_timeline": (ctypes.c_int, [P(cf_product), _c_double_p, ctypes.c_int]),
+            "cf_sim_timeline": (ctypes.c_int, [P(cf_model), P(cf_product), _c_double_p, P(ctypes.c_int), ctypes.c_int]),
+            "cf_mc_simul": (ctypes.c_int, [P(cf_model), P(cf_product), P(cf_num), _c_double_p, _c_double_p, _c_double_p]),
+            "cf_mc_simul_aad": (ctypes.c_int, [P(cf_model), P(cf_product), P(cf_num), _c_double_p, _c_double_p,
+                                               _c_double_p, _c_double_p, _c_double_p, _c_double_p]),
+            "cf_aad_footprint": (ctypes.c_int64, [P(cf_model), P(cf_product), P(cf_num)]),
+            "cf_last_error": (ctypes.c_char_p, []),
+            "cf_last_kernel_stats": (ctypes.c_int, [P(cf_kernel_stats)]),
+            "cf_version": (ctypes.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc < 0:
+        raise CfError(rc, lib().cf_last_error().decode())
+    return rc
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_c_double_p) if a is not None else None
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+# ---------------------------------------------------------------- models
+
+
+class BlackScholes:
+    """BlackScholes(spot, vol, spotMeasure=False, rate=0, div=0) (mcMdlBS.h:72-93)."""
+
+    def __init__(self, spot, vol, spotMeasure=False, rate=0.0, div=0.0):
+        self.spot, self.vol, self.spotMeasure, self.rate, self.div = (
+            float(spot), float(vol), bool(spotMeasure), float(rate), float(div))
+
+    def descriptor(self):
+        d = cf_model()
+        d.kind, d.spot, d.vol, d.rate, d.div = 0, self.spot, self.vol, self.rate, self.div
+        d.spot_measure = int(self.spotMeasure)
+        return d
+
+    def parameters(self):
+        return [self.spot, self.vol, self.rate, self.div]
+
+    def numParams(self):
+        return 4
+
+    def parameterLabels(self):
+        return _param_labels(self)
+
+
+class Dupire:
+    """Dupire(spot, spots, times, vols, maxDt=0.25) (mcMdlDupire.h:76-111).
+
+    vols[i][j] = local vol at spots[i], times[j] (spot major)."""
+
+    def __init__(self, spot, spots, times, vols, maxDt=0.25):
+        self.spot = float(spot)
+        self.spots = _f64(spots)
+        self.times = _f64(times)
+        self.vols = _f64(vols).reshape(len(self.spots), len(self.times))
+        self.maxDt = float(maxDt)
+
+    def descriptor(self):
+        d = cf_model()
+        d.kind, d.spot = 1, self.spot
+        d.n_spots, d.n_times = len(self.spots), len(self.times)
+        self._keep = (self.spots, self.times, np.ascontiguousarray(self.vols))
+        d.spots, d.times, d.vols = (_dptr(a) for a in self._keep)
+        d.max_dt = self.maxDt
+        return d
+
+    def parameters(self):
+        return [self.spot] + list(self.vols.ravel())
+
+    def numParams(self):
+        return 1 + self.vols.size
+
+    def parameterLabels(self):
+        return _param_labels(self)
+
+
+# ---------------------------------------------------------------- products
+
+
+class European:
+    """European(strike, exerciseDate, settlementDate=exerciseDate) (mcPrd.h:44-86)."""
+
+    def __init__(self, strike, exerciseDate, settlementDate=None):
+        self.strike, self.exerciseDate = float(strike), float(exerciseDate)
+        self.settlementDate = float(settlementDate) if settlementDate else self.exerciseDate
+
+    def descriptor(self):
+        d = cf_product()
+        d.kind, d.strike, d.exercise, d.settlement = 0, self.strike, self.exerciseDate, self.settlementDate
+        return d
+
+    def payoffLabels(self):
+        return _payoff_labels(self)
+
+    def timeline(self):
+        return _product_timeline(self)
+
+
+class UOC:
+    """UOC(strike, barrier, maturity, monitorFreq, smooth, callPut=False) (mcPrd.h:149-208).
+
+    Non-positive smoothing is replaced by 1e-8 as putBarrier does (store.h:153)."""
+
+    def __init__(self, strike, barrier, maturity, monitorFreq, smooth, callPut=False):
+        self.strike, self.barrier, self.maturity = float(strike), float(barrier), float(maturity)
+        self.monitorFreq, self.smooth, self.callPut = float(monitorFreq), float(smooth), bool(callPut)
+
+    def descriptor(self):
+        d = cf_product()
+        d.kind, d.strike, d.barrier, d.maturity = 1, self.strike, self.barrier, self.maturity
+        d.monitor_freq, d.smooth, d.call_put = self.monitorFreq, self.smooth, int(self.callPut)
+        return d
+
+    def payoffLabels(self):
+        return _payoff_labels(self)
+
+    def timeline(self):
+        return _product_timeline(self)
+
+
+class Europeans:
+    """Europeans over (maturity, strike) pairs, maturity major (store.h:184-205, mcPrd.h:305-336)."""
+
+    def __init__(self, maturities, strikes):
+        self.maturities, self.strikes = _f64(maturities), _f64(strikes)
+        if self.maturities.shape != self.strikes.shape:
+            raise ValueError("maturities and strikes must have the same length")
+
+    def descriptor(self):
+        d = cf_product()
+        d.kind, d.n_options = 2, len(self.maturities)
+        self._keep = (self.maturities, self.strikes)
+        d.maturities, d.strikes = _dptr(self.maturities), _dptr(self.strikes)
+        return d
+
+    def payoffLabels(self):
+        return _payoff_labels(self)
+
+    def timeline(self):
+        return _product_timeline(self)
+
+
+class NumericalParam:
+    """NumericalParam (main.h:33-40) plus precision, AAD variant and first path."""
+
+    def __init__(self, parallel=True, useSobol=True, numPath=65536, seed1=12345, seed2=1234,
+                 precision=FP64, aadMode=AAD_CHECKPOINT, firstPath=0):
+        self.parallel, self.useSobol, self.numPath = parallel, useSobol, int(numPath)
+        self.seed1, self.seed2 = int(seed1), int(seed2)
+        self.precision, self.aadMode, self.firstPath = int(precision), int(aadMode), int(firstPath)
+
+    def descriptor(self):
+        n = cf_num()
+        n.parallel, n.use_sobol, n.num_path = int(bool(self.parallel)), int(bool(self.useSobol)), self.numPath
+        n.seed1, n.seed2, n.precision, n.aad_mode, n.first_path = (
+            self.seed1, self.seed2, self.precision, self.aadMode, self.firstPath)
+        return n
+
+
+def _payoff_labels(prd):
+    d = prd.descriptor()
+    n = _check(lib().cf_num_payoffs(ctypes.byref(d)))
+    out = []
+    buf = ctypes.create_string_buffer(512)
+    for k in range(n):
+        _check(lib().cf_payoff_label(ctypes.byref(d), k, buf, 512))
+        out.append(buf.value.decode())
+    return out
+
+
+def _param_labels(mdl):
+    d = mdl.descriptor()
+    n = _check(lib().cf_num_params(ctypes.byref(d)))
+    out = []
+    buf = ctypes.create_string_buffer(512)
+    for k in range(n):
+        _check(lib().cf_param_label(ctypes.byref(d), k, buf, 512))
+        out.append(buf.value.decode())
+    return out
+
+
+def _product_timeline(prd):
+    d = prd.descriptor()
+    n = _check(lib().cf_product_timeline(ctypes.byref(d), None, 0))
+    out = np.zeros(n)
+    lib().cf_product_timeline(ctypes.byref(d), _dptr(out), n)
+    return out
+
+
+def simulation_timeline(mdl, prd):
+    """Simulation timeline of ``mdl`` for ``prd`` and its product-date flags."""
+    dm, dp = mdl.descriptor(), prd.descriptor()
+    n = _check(lib().cf_sim_timeline(ctypes.byref(dm), ctypes.byref(dp), None, None, 0))
+    t = np.zeros(n)
+    c = np.zeros(n, dtype=np.int32)
+    lib().cf_sim_timeline(ctypes.byref(dm), ctypes.byref(dp), _dptr(t),
+                          c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n)
+    return t, c.astype(bool)
+
+
+def last_kernel_stats():
+    """Device timings of the last simulation call on this thread."""
+    s = cf_kernel_stats()
+    _check(lib().cf_last_kernel_stats(ctypes.byref(s)))
+    return {f: getattr(s, f) for f, _ in cf_kernel_stats._fields_}
+
+
+# ---------------------------------------------------------------- simulation
+
+
+class SimulResults:
+    def __init__(self, means, std_errs, payoffs):
+        self.means, self.std_errs, self.payoffs = means, std_errs, payoffs
+
+
+class AADSimulResults:
+    """payoffs[nPath][nPay], aggregated[nPath], risks[nParam] (mcBase.h:405-422),
+    plus the means the engine reduces on the device."""
+
+    def __init__(self, payoffs, aggregated, risks, payoff_means, agg_mean):
+        self.payoffs, self.aggregated, self.risks = payoffs, aggregated, risks
+        self.payoff_means, self.agg_mean = payoff_means, agg_mean
+
+
+def _num(num, nPath=None, precision=None):
+    if num is None:
+        num = NumericalParam(numPath=nPath if nPath is not None else 65536)
+    n = num.descriptor()
+    if nPath is not None:
+        n.num_path = int(nPath)
+    if precision is not None:
+        n.precision = int(precision)
+    return n
+
+
+def mcParallelSimul(prd, mdl, nPath=None, num=None, per_path=True, precision=None):
+    """mcParallelSimul (mcBase.h:314-400): per-path payoff matrix (nPath x nPay)
+    when ``per_path``, plus device-reduced means and standard errors."""
+    dm, dp = mdl.descriptor(), prd.descriptor()
+    n = _num(num, nPath, precision)
+    npay = _check(lib().cf_num_payoffs(ctypes.byref(dp)))
+    means = np.zeros(npay)
+    ses = np.zeros(npay)
+    pp = np.zeros((n.num_path, npay)) if per_path else None
+    _check(lib().cf_mc_simul(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n), _dptr(means), _dptr(ses),
+                             _dptr(pp)))
+    return SimulResults(means, ses, pp)
+
+
+def mcParallelSimulAAD(prd, mdl, nPath=None, weights=None, num=None, per_path=False, precision=None):
+    """mcParallelSimulAAD (mcBase.h:564-753) with the linear aggregator
+    sum_k weights[k] * payoff_k (default: the first payoff, mcBase.h:425)."""
+    dm, dp = mdl.descriptor(), prd.descriptor()
+    n = _num(num, nPath, precision)
+    npay = _check(lib().cf_num_payoffs(ctypes.byref(dp)))
+    nparam = _check(lib().cf_num_params(ctypes.byref(dm)))
+    w = np.zeros(npay)
+    if weights is None:
+        w[0] = 1.0
+    else:
+        w[:] = _f64(weights)
+    means = np.zeros(npay)
+    agg = np.zeros(1)
+    risks = np.zeros(nparam)
+    pp = np.zeros((n.num_path, npay)) if per_path else None
+    pa = np.zeros(n.num_path) if per_path else None
+    _check(lib().cf_mc_simul_aad(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n), _dptr(w), _dptr(means),
+                                 _dptr(agg), _dptr(risks), _dptr(pp), _dptr(pa)))
+    return AADSimulResults(pp, pa, risks, means, float(agg[0]))
+
+
+def aad_footprint(prd, mdl, num):
+    """Device bytes one AAD call would take (checkpoint buffers dominate)."""
+    dm, dp = mdl.descriptor(), prd.descriptor()
+    n = num.descriptor()
+    return _check(lib().cf_aad_footprint(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n)))
+
+
+# ---------------------------------------------------------------- store + entry points
+# store.h:30-36 keeps one registry of models and one of products, keyed by id.
+
+_model_store = {}
+_product_store = {}
+
+
+def putBlackScholes(spot, vol, qSpot, rate, div, store):
+    _model_store[store] = BlackScholes(spot, vol, qSpot, rate, div)
+
+
+def putDupire(spot, spots, times, vols, maxDt, store):
+    _model_store[store] = Dupire(spot, spots, times, vols, maxDt)
+
+
+def putEuropean(strike, exerciseDate, settlementDate, store):
+    _product_store[store] = European(strike, exerciseDate, settlementDate)
+
+
+def putBarrier(strike, barrier, maturity, monitorFreq, smooth, callPut, store):
+    _product_store[store] = UOC(strike, barrier, maturity, monitorFreq, smooth, callPut)
+
+
+def putEuropeans(maturities, strikes, store):
+    _product_store[store] = Europeans(maturities, strikes)
+
+
+def clearStore():
+    _model_store.clear()
+    _product_store.clear()
+
+
+def _get(modelId, productId, who):
+    mdl, prd = _model_store.get(modelId), _product_store.get(productId)
+    if mdl is None or prd is None:
+        raise CfError(-2, f"{who} : Could not retrieve model and product")
+    return mdl, prd
+
+
+def value(modelId, productId, num):
+    """main.h:80-96: {identifiers, values} of the mean payoffs."""
+    mdl, prd = _get(modelId, productId, "value()")
+    res = mcParallelSimul(prd, mdl, num=num, per_path=False)
+    return OrderedDict(identifiers=prd.payoffLabels(), values=res.means, std_errs=res.std_errs)
+
+
+def AADriskOne(modelId, productId, num, riskPayoff=""):
+    """main.h:99-173."""
+    mdl, prd = _get(modelId, productId, "AADrisk()")
+    labels = prd.payoffLabels()
+    idx = 0
+    if riskPayoff:
+        if riskPayoff not in labels:
+            raise CfError(-2, "AADriskOne() : payoff not found")
+        idx = labels.index(riskPayoff)
+    w = np.zeros(len(labels))
+    w[idx] = 1.0
+    return _risk_report(mdl, prd, labels, w, num)
+
+
+def AADriskAggregate(modelId, productId, notionals, num):
+    """main.h:176-254: notionals {payoff label: notional}."""
+    mdl, prd = _get(modelId, productId, "AADriskAggregate()")
+    labels = prd.payoffLabels()
+    w = np.zeros(len(labels))
+    for lab, nv in notionals.items():
+        if lab not in labels:
+            raise CfError(-2, "AADriskAggregate() : payoff not found")
+        w[labels.index(lab)] = nv
+    return _risk_report(mdl, prd, labels, w, num)
+
+
+def _risk_report(mdl, prd, labels, w, num):
+    res = mcParallelSimulAAD(prd, mdl, weights=w, num=num)
+    return OrderedDict(payoffIds=labels, payoffValues=res.payoff_means, riskPayoffValue=res.agg_mean,
+                       paramIds=mdl.parameterLabels(), risks=res.risks)
+
+
+def dupireAADRisk(modelId, productId, notionals, num):
+    """main.h:364-411: value, delta and the spots x times vega matrix."""
+    mdl = _model_store.get(modelId)
+    if mdl is None:
+        raise CfError(-2, "dupireAADRisk() : Model not found")
+    if not isinstance(mdl, Dupire):
+        raise CfError(-1, "dupireAADRisk() : Model not a Dupire")
+    r = AADriskAggregate(modelId, productId, notionals, num)
+    return OrderedDict(value=r["riskPayoffValue"], delta=r["risks"][0],
+                       vega=np.asarray(r["risks"][1:]).reshape(len(mdl.spots), len(mdl.times)))

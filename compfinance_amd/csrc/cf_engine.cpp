@@ -1,0 +1,902 @@
+// Host engine behind the descriptor-level C ABI (include/cfamd.h):
+// validates the descriptors, builds the timelines and tables in FP64 exactly
+// as the reference does, stages them on the device, launches the path kernel
+// and the cross-block reductions, and maps adjoints back to model parameters.
+//
+// Reference steps restated here:
+//   product timelines     mcPrd.h:44-81, :149-208, :305-336; store.h:153, :190-195
+//   Dupire timeline/init  mcMdlDupire.h:165-217 with fillData utility.h:11-79
+//   BS timeline/init      mcMdlBS.h:149-277
+//   labels                mcPrd.h:66-80, :197-207, :325-335; mcMdlDupire.h:95-107;
+//                         mcMdlBS.h:87-90
+//   risk normalisation    mcBase.h:738-746 (sum of adjoints / nPath)
+//   adjoint of the local-vol pre-interpolation (before the tape mark):
+//                         mcMdlDupire.h:203-215, mcBase.h:717-735
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <iomanip>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cfamd.h"
+#include "cf_internal.h"
+#include "sobol_init.inc"
+
+namespace {
+
+using std::size_t;
+using std::string;
+using std::vector;
+
+constexpr double kHalfDay = 0.00136986301369863;   // mcMdlDupire.h:26
+constexpr double kOneHour = 0.000114469;           // mcPrd.h:26
+constexpr double kEps = 1.0e-08;                   // utility.h:5, store.h:153
+constexpr int kBlockThreads = 1024;
+
+// ------------------------------------------------------------------ errors
+
+thread_local string g_err;
+thread_local cf_kernel_stats g_stats{};
+
+struct CfError : std::runtime_error {
+    int code;
+    CfError(int c, const string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] void fail(int code, const string& msg) { throw CfError(code, msg); }
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        return f();
+    } catch (const CfError& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of host memory";
+        return CF_ERR_CAPACITY;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return CF_ERR_ARG;
+    }
+}
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(CF_ERR_DEVICE, string(what) + ": " + hipGetErrorString(e));
+}
+
+string fmt2(double x)
+{
+    std::ostringstream o;
+    o << std::fixed << std::setprecision(2) << x;
+    return o.str();
+}
+
+// ------------------------------------------------------------------ timelines
+
+// utility.h:11-79: union with the added points (equality within minDx), then
+// supplementary points so that no gap exceeds maxDx.
+vector<double> fill_data(const vector<double>& original, double maxDx, double minDx, const vector<double>& add)
+{
+    vector<double> seq;
+    if (!add.empty()) {
+        std::set_union(original.begin(), original.end(), add.begin(), add.end(), std::back_inserter(seq),
+                       [minDx](double x, double y) { return x < y - minDx; });
+    } else {
+        seq = original;
+    }
+    vector<double> out;
+    out.push_back(seq[0]);
+    for (size_t k = 1; k < seq.size(); ++k) {
+        const double cur = out.back(), nxt = seq[k];
+        if (nxt - cur > maxDx) {
+            const int pts = int((nxt - cur) / maxDx - kEps) + 1;
+            const double spacing = (nxt - cur) / pts;
+            double t = cur + spacing;
+            while (t < nxt - minDx) {
+                out.push_back(t);
+                t += spacing;
+            }
+        }
+        out.push_back(nxt);
+    }
+    return out;
+}
+
+// linear interpolation with flat extrapolation (interp.h:26-63) and its
+// two-node weights
+struct Interp1 {
+    int j0, j1;
+    double w0, w1;
+};
+
+Interp1 interp_weights(const double* xs, int n, double x)
+{
+    const double* it = std::upper_bound(xs, xs + n, x);
+    if (it == xs + n) return {n - 1, n - 1, 1.0, 0.0};
+    if (it == xs) return {0, 0, 1.0, 0.0};
+    const int k = int(it - xs) - 1;
+    const double t = (x - xs[k]) / (xs[k + 1] - xs[k]);
+    return {k, k + 1, 1.0 - t, t};
+}
+
+double interp_value(const double* xs, int n, const double* ys, double x)
+{
+    const double* it = std::upper_bound(xs, xs + n, x);
+    if (it == xs + n) return ys[n - 1];
+    if (it == xs) return ys[0];
+    const size_t k = size_t(it - xs) - 1;
+    const double x1 = xs[k], x2 = xs[k + 1], y1 = ys[k], y2 = ys[k + 1];
+    const double t = (x - x1) / (x2 - x1);
+    return y1 + (y2 - y1) * t;
+}
+
+// ------------------------------------------------------------------ plans
+
+struct SampleDefH {
+    bool numeraire = true;
+    bool has_fwd = false, has_disc = false;
+    double fwd_mat = 0.0, disc_mat = 0.0;
+};
+
+struct ProductPlan {
+    int kind = 0;
+    vector<double> timeline;
+    vector<SampleDefH> defs;
+    double strike = 0.0, barrier = 0.0, smooth = 0.0, freq = 0.0, maturity = 0.0;
+    double exercise = 0.0, settlement = 0.0;
+    bool put = false;
+    vector<double> mats;              // Europeans: distinct maturities, increasing
+    vector<vector<double>> strikes;   // Europeans: strikes per maturity, input order
+    vector<int> pay_first;
+    vector<double> pay_strike;
+    int n_pay = 0;
+};
+
+void check_finite(double x, const char* what)
+{
+    if (!std::isfinite(x)) fail(CF_ERR_ARG, string(what) + " must be finite");
+}
+
+ProductPlan plan_product(const cf_product* p)
+{
+    if (!p) fail(CF_ERR_ARG, "null product");
+    ProductPlan d;
+    d.kind = p->kind;
+    if (p->kind == CF_PRODUCT_EUROPEAN) {
+        // mcPrd.h:44-81
+        check_finite(p->strike, "strike");
+        check_finite(p->exercise, "exercise date");
+        if (p->exercise < 0.0) fail(CF_ERR_ARG, "exercise date must be non-negative");
+        d.strike = p->strike;
+        d.exercise = p->exercise;
+        d.settlement = p->settlement > 0.0 ? p->settlement : p->exercise;
+        d.timeline = {d.exercise};
+        SampleDefH s;
+        s.numeraire = true;
+        s.has_fwd = true;
+        s.fwd_mat = d.settlement;
+        s.has_disc = true;
+        s.disc_mat = d.settlement;
+        d.defs = {s};
+        d.n_pay = 1;
+    } else if (p->kind == CF_PRODUCT_BARRIER) {
+        // mcPrd.h:149-208; store.h:153
+        check_finite(p->strike, "strike");
+        check_finite(p->barrier, "barrier");
+        check_finite(p->maturity, "maturity");
+        check_finite(p->monitor_freq, "monitoring frequency");
+        if (!(p->monitor_freq > 0.0)) fail(CF_ERR_ARG, "monitoring frequency must be positive");
+        if (!(p->maturity > 0.0)) fail(CF_ERR_ARG, "maturity must be positive");
+        if (p->maturity / p->monitor_freq > 20000.0) fail(CF_ERR_CAPACITY, "too many monitoring dates");
+        d.strike = p->strike;
+        d.barrier = p->barrier;
+        d.smooth = p->smooth <= 0 ? kEps : p->smooth;
+        d.put = p->call_put != 0;
+        d.freq = p->monitor_freq;
+        d.maturity = p->maturity;
+        d.timeline.push_back(0.0);
+        double t = 0.0 + p->monitor_freq;
+        while (p->maturity - t > kOneHour) {
+            d.timeline.push_back(t);
+            t += p->monitor_freq;
+        }
+        d.timeline.push_back(p->maturity);
+        for (size_t i = 0; i < d.timeline.size(); ++i) {
+            SampleDefH s;
+            s.numeraire = false;
+            s.has_fwd = true;
+            s.fwd_mat = d.timeline[i];
+            d.defs.push_back(s);
+        }
+        d.defs.back().numeraire = true;
+        d.n_pay = 2;
+    } else if (p->kind == CF_PRODUCT_EUROPEANS) {
+        // store.h:184-205, mcPrd.h:305-336
+        if (p->n_options <= 0 || !p->maturities || !p->strikes) fail(CF_ERR_ARG, "no options");
+        std::map<double, vector<double>> opts;
+        for (int i = 0; i < p->n_options; ++i) {
+            check_finite(p->maturities[i], "maturity");
+            check_finite(p->strikes[i], "strike");
+            if (p->maturities[i] < 0.0) fail(CF_ERR_ARG, "maturities must be non-negative");
+            opts[p->maturities[i]].push_back(p->strikes[i]);
+        }
+        for (auto& kv : opts) {
+            d.mats.push_back(kv.first);
+            d.strikes.push_back(kv.second);
+            d.timeline.push_back(kv.first);
+            SampleDefH s;
+            s.numeraire = true;
+            s.has_fwd = true;
+            s.fwd_mat = kv.first;
+            d.defs.push_back(s);
+        }
+        d.pay_first.push_back(0);
+        for (auto& ks : d.strikes) {
+            for (double k : ks) d.pay_strike.push_back(k);
+            d.pay_first.push_back(int(d.pay_strike.size()));
+        }
+        d.n_pay = int(d.pay_strike.size());
+    } else {
+        fail(CF_ERR_UNSUPPORTED, "product kind not supported by the GPU engine");
+    }
+    return d;
+}
+
+vector<string> payoff_labels(const ProductPlan& d)
+{
+    vector<string> out;
+    if (d.kind == CF_PRODUCT_EUROPEAN) {
+        if (d.settlement == d.exercise) out.push_back("call " + fmt2(d.strike) + " " + fmt2(d.exercise));
+        else out.push_back("call " + fmt2(d.strike) + " " + fmt2(d.exercise) + " " + fmt2(d.settlement));
+    } else if (d.kind == CF_PRODUCT_BARRIER) {
+        const string vanilla = string(d.put ? "put " : "call ") + fmt2(d.maturity) + " " + fmt2(d.strike);
+        out.push_back(vanilla + " up and out " + fmt2(d.barrier) + " monitoring freq " + fmt2(d.freq) + " smooth " +
+                      fmt2(d.smooth));
+        out.push_back(vanilla);
+    } else {
+        for (size_t i = 0; i < d.mats.size(); ++i)
+            for (double k : d.strikes[i]) out.push_back("call " + fmt2(d.mats[i]) + " " + fmt2(k));
+    }
+    return out;
+}
+
+struct ModelPlan {
+    int kind = 0;
+    double spot = 0.0;
+    vector<double> timeline;
+    vector<int> step_sample;   // [n_steps + 1]
+    int n_steps = 0;
+    // Dupire
+    int n_spots = 0, n_times = 0;
+    vector<double> spots, times, log_spots, inv_dx, iv;   // iv [n_steps][n_spots]
+    // Black-Scholes
+    double vol = 0.0, rate = 0.0, div = 0.0;
+    bool spot_measure = false;
+    vector<double> drift, stdv;
+    vector<double> ff, disc, num;   // per product sample
+    vector<double> bs_aux;
+    double s_t0 = 0.0;              // first sample's forward (barrier smoothing width)
+};
+
+void validate_model(const cf_model* m)
+{
+    if (!m) fail(CF_ERR_ARG, "null model");
+    check_finite(m->spot, "spot");
+    if (!(m->spot > 0.0)) fail(CF_ERR_ARG, "spot must be positive");
+    if (m->kind == CF_MODEL_BLACK_SCHOLES) {
+        check_finite(m->vol, "vol");
+        check_finite(m->rate, "rate");
+        check_finite(m->div, "div");
+    } else if (m->kind == CF_MODEL_DUPIRE) {
+        if (m->n_spots < 1 || m->n_times < 1 || !m->spots || !m->times || !m->vols)
+            fail(CF_ERR_ARG, "empty local volatility grid");
+        if (m->n_spots > 1024) fail(CF_ERR_CAPACITY, "more than 1024 local volatility spots");
+        for (int i = 0; i < m->n_spots; ++i) {
+            check_finite(m->spots[i], "local volatility spot");
+            if (!(m->spots[i] > 0.0)) fail(CF_ERR_ARG, "local volatility spots must be positive");
+            if (i > 0 && !(m->spots[i] > m->spots[i - 1])) fail(CF_ERR_ARG, "local volatility spots must increase");
+        }
+        for (int j = 0; j < m->n_times; ++j) {
+            check_finite(m->times[j], "local volatility time");
+            if (j > 0 && !(m->times[j] > m->times[j - 1])) fail(CF_ERR_ARG, "local volatility times must increase");
+        }
+        for (long k = 0; k < long(m->n_spots) * m->n_times; ++k) check_finite(m->vols[k], "local volatility");
+        check_finite(m->max_dt, "maxDt");
+        if (!(m->max_dt > 0.0)) fail(CF_ERR_ARG, "maxDt must be positive");
+    } else {
+        fail(CF_ERR_UNSUPPORTED, "model kind not supported by the GPU engine");
+    }
+}
+
+ModelPlan plan_model(const cf_model* m, const ProductPlan& p)
+{
+    validate_model(m);
+    ModelPlan d;
+    d.kind = m->kind;
+    d.spot = m->spot;
+    const size_t ns = p.timeline.size();
+    if (m->kind == CF_MODEL_DUPIRE) {
+        // mcMdlDupire.h:165-193
+        d.n_spots = m->n_spots;
+        d.n_times = m->n_times;
+        d.spots.assign(m->spots, m->spots + m->n_spots);
+        d.times.assign(m->times, m->times + m->n_times);
+        const double today = 0.0;
+        d.timeline = fill_data(p.timeline, m->max_dt, kHalfDay, {today});
+        d.n_steps = int(d.timeline.size()) - 1;
+        d.step_sample.assign(d.timeline.size(), -1);
+        int s = 0;
+        for (size_t i = 0; i < d.timeline.size(); ++i)
+            if (std::binary_search(p.timeline.begin(), p.timeline.end(), d.timeline[i])) d.step_sample[i] = s++;
+        if (size_t(s) != ns) fail(CF_ERR_ARG, "product dates closer than half a day cannot be simulated");
+        // mcMdlDupire.h:91-92, :195-217
+        d.log_spots.resize(d.n_spots);
+        for (int i = 0; i < d.n_spots; ++i) d.log_spots[i] = std::log(d.spots[i]);
+        d.inv_dx.assign(d.n_spots, 0.0);
+        for (int i = 0; i + 1 < d.n_spots; ++i) d.inv_dx[i] = 1.0 / (d.log_spots[i + 1] - d.log_spots[i]);
+        d.iv.resize(size_t(d.n_steps) * d.n_spots);
+        for (int i = 0; i < d.n_steps; ++i) {
+            const double sqrtdt = std::sqrt(d.timeline[i + 1] - d.timeline[i]);
+            for (int j = 0; j < d.n_spots; ++j)
+                d.iv[size_t(i) * d.n_spots + j] =
+                    sqrtdt * interp_value(m->times, m->n_times, m->vols + size_t(j) * m->n_times, d.timeline[i]);
+        }
+        d.ff.assign(ns, 1.0);
+        d.disc.assign(ns, 1.0);
+        d.num.assign(ns, 1.0);
+        d.s_t0 = std::exp(std::log(m->spot));   // spot written at t = 0 (mcMdlDupire.h:245-253)
+    } else {
+        // mcMdlBS.h:149-195
+        d.vol = m->vol;
+        d.rate = m->rate;
+        d.div = m->div;
+        d.spot_measure = m->spot_measure != 0;
+        const double today = 0.0;
+        d.timeline.assign(1, today);
+        for (double t : p.timeline)
+            if (t > today) d.timeline.push_back(t);
+        const bool todayOn = p.timeline[0] == today;
+        d.n_steps = int(d.timeline.size()) - 1;
+        d.step_sample.assign(d.timeline.size(), -1);
+        int s = 0;
+        if (todayOn) d.step_sample[0] = s++;
+        for (int i = 1; i <= d.n_steps; ++i) d.step_sample[i] = s++;
+        if (size_t(s) != ns) fail(CF_ERR_ARG, "product timeline not simulable in Black-Scholes");
+        // mcMdlBS.h:197-277
+        const double mu = d.rate - d.div;
+        d.drift.resize(d.n_steps);
+        d.stdv.resize(d.n_steps);
+        for (int i = 0; i < d.n_steps; ++i) {
+            const double dt = d.timeline[i + 1] - d.timeline[i];
+            d.stdv[i] = d.vol * std::sqrt(dt);
+            d.drift[i] = d.spot_measure ? (mu + 0.5 * d.vol * d.vol) * dt : (mu - 0.5 * d.vol * d.vol) * dt;
+        }
+        d.ff.assign(ns, 1.0);
+        d.disc.assign(ns, 1.0);
+        d.num.assign(ns, 1.0);
+        d.bs_aux.assign(3 * ns + d.n_steps, 0.0);
+        for (size_t i = 0; i < ns; ++i) {
+            const SampleDefH& def = p.defs[i];
+            const double t = p.timeline[i];
+            if (def.numeraire) d.num[i] = d.spot_measure ? std::exp(d.div * t) / d.spot : std::exp(d.rate * t);
+            if (def.has_disc) d.disc[i] = std::exp(-d.rate * (def.disc_mat - t));
+            if (def.has_fwd) d.ff[i] = std::exp(mu * (def.fwd_mat - t));
+            d.bs_aux[i] = t;
+            d.bs_aux[ns + i] = def.has_fwd ? def.fwd_mat - t : 0.0;
+            d.bs_aux[2 * ns + i] = def.has_disc ? def.disc_mat - t : 0.0;
+        }
+        for (int i = 0; i < d.n_steps; ++i) d.bs_aux[3 * ns + i] = d.timeline[i + 1] - d.timeline[i];
+        d.s_t0 = m->spot * d.ff[0];   // forward of the first sample at t = 0
+    }
+    if (d.n_steps > cfk::kSobolDims)
+        fail(CF_ERR_CAPACITY, "more simulation steps than Sobol dimensions (" + std::to_string(cfk::kSobolDims) + ")");
+    return d;
+}
+
+vector<string> param_labels(const cf_model* m)
+{
+    vector<string> out;
+    out.push_back("spot");
+    if (m->kind == CF_MODEL_BLACK_SCHOLES) {
+        out.push_back("vol");
+        out.push_back("rate");
+        out.push_back("div");
+    } else {
+        for (int i = 0; i < m->n_spots; ++i)
+            for (int j = 0; j < m->n_times; ++j) out.push_back("lvol " + fmt2(m->spots[i]) + " " + fmt2(m->times[j]));
+    }
+    return out;
+}
+
+// ------------------------------------------------------------------ device
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return p;
+        if (p) hip_check(hipFree(p), "hipFree");
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        hip_check(hipMalloc(&p, want), "hipMalloc");
+        cap = want;
+        return p;
+    }
+};
+
+struct Engine {
+    int device = -1;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    uint32_t* d_dirs = nullptr;
+    DevBuf tables, block_out, ivbar_blocks, reduced, ivbar, per_path, per_path_agg, ckpt;
+    std::mutex mu;
+};
+
+std::mutex g_engines_mu;
+std::map<int, std::unique_ptr<Engine>> g_engines;
+
+Engine& engine()
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_engines_mu);
+    auto& slot = g_engines[dev];
+    if (!slot) {
+        auto e = std::make_unique<Engine>();
+        e->device = dev;
+        hip_check(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, dev),
+                  "hipDeviceGetAttribute");
+        hip_check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& v : e->ev) hip_check(hipEventCreate(&v), "hipEventCreate");
+        // direction numbers [32][kSobolDims], Joe-Kuo recurrence
+        vector<uint32_t> dirs(32 * size_t(cfk::kSobolDims));
+        for (int d = 0; d < cfk::kSobolDims; ++d) {
+            const unsigned s = kSobolInit[d][0], a = kSobolInit[d][1];
+            uint32_t v[32];
+            for (unsigned k = 0; k < 32; ++k) {
+            if (s == 0) {
+                v[k] = 1u << (31 - k);   // van der Corput dimension
+            } else if (k < s) {
+                v[k] = kSobolInit[d][2 + k] << (31 - k);
+                } else {
+                    uint32_t x = v[k - s] ^ (v[k - s] >> s);
+                    for (unsigned j = 1; j < s; ++j)
+                        if ((a >> (s - 1 - j)) & 1u) x ^= v[k - j];
+                    v[k] = x;
+                }
+            }
+            for (int k = 0; k < 32; ++k) dirs[size_t(k) * cfk::kSobolDims + d] = v[k];
+        }
+        hip_check(hipMalloc(&e->d_dirs, dirs.size() * 4), "hipMalloc");
+        hip_check(hipMemcpy(e->d_dirs, dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        slot = std::move(e);
+    }
+    return *slot;
+}
+
+// Host staging of the per-call tables, one upload.
+struct Staging {
+    vector<unsigned char> bytes;
+    template <class T>
+    size_t add(const T* data, size_t n)
+    {
+        const size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + n * sizeof(T));
+        if (n) std::memcpy(bytes.data() + off, data, n * sizeof(T));
+        return off;
+    }
+    template <class T>
+    size_t add(const vector<T>& v) { return add(v.data(), v.size()); }
+};
+
+struct Call {
+    bool adjoint = false;
+    const cf_num* num = nullptr;
+    const double* weights = nullptr;
+    // outputs
+    double* means = nullptr;
+    double* std_errs = nullptr;
+    double* per_path = nullptr;
+    double* agg_mean = nullptr;
+    double* risks = nullptr;
+    double* per_path_agg = nullptr;
+};
+
+void validate_num(const cf_num* num)
+{
+    if (!num) fail(CF_ERR_ARG, "null numerical parameters");
+    if (!num->use_sobol) fail(CF_ERR_UNSUPPORTED, "only the Sobol generator is implemented on the GPU engine");
+    if (num->num_path < 1) fail(CF_ERR_ARG, "numPath must be positive");
+    if (num->first_path < 0) fail(CF_ERR_ARG, "first path must be non-negative");
+    if (int64_t(num->first_path) + num->num_path >= (int64_t(1) << 32) - 1)
+        fail(CF_ERR_CAPACITY, "Sobol index beyond 2^32 - 1");
+    if (num->precision != CF_FP64 && num->precision != CF_FP32) fail(CF_ERR_ARG, "unknown precision");
+    if (num->aad_mode != CF_AAD_CHECKPOINT && num->aad_mode != CF_AAD_RECOMPUTE) fail(CF_ERR_ARG, "unknown AAD mode");
+}
+
+struct LaunchShape {
+    int block = kBlockThreads;
+    int grid = 0;
+    size_t smem = 0;
+    int slots = 0;
+};
+
+LaunchShape shape_for(Engine& E, const cfk::KernelConfig& cfg, const ModelPlan& md, const ProductPlan& pd,
+                      int64_t n_path, int64_t first)
+{
+    LaunchShape s;
+    const int waves = s.block / cfk::kWave;
+    const cfk::SmemLayout L =
+        cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves,
+                         cfg.precision == CF_FP64 ? 8 : 4, md.kind == CF_MODEL_DUPIRE,
+                         pd.kind == CF_PRODUCT_EUROPEANS, cfg.adjoint);
+    s.smem = L.total;
+    int max_lds = 0;
+    hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, E.device), "attribute");
+    if (s.smem > size_t(max_lds))
+        fail(CF_ERR_CAPACITY, "tables need " + std::to_string(s.smem) + " bytes of LDS per block, device allows " +
+                                  std::to_string(max_lds));
+    int per_cu = 0;
+    int rc = cfk::path_kernel_occupancy(cfg, s.block, s.smem, &per_cu);
+    if (rc != 0) hip_check(hipError_t(rc), "occupancy");
+    if (per_cu < 1) fail(CF_ERR_CAPACITY, "path kernel does not fit on a compute unit");
+    const int64_t q0 = (first + 1) >> 6, q1 = (first + n_path) >> 6;
+    const int64_t nq = q1 - q0 + 1;
+    const int64_t want = (nq + waves - 1) / waves;
+    s.grid = int(std::min<int64_t>(want, int64_t(per_cu) * E.num_cus));
+    s.slots = s.grid * waves;
+    return s;
+}
+
+int run(const cf_model* mdl, const cf_product* prd, Call& c)
+{
+    validate_num(c.num);
+    const ProductPlan pd = plan_product(prd);
+    const ModelPlan md = plan_model(mdl, pd);
+    const int npay = pd.n_pay;
+    const int64_t n_path = c.num->num_path, first = c.num->first_path;
+    const bool fp64 = c.num->precision == CF_FP64;
+    const bool dup = md.kind == CF_MODEL_DUPIRE;
+    if (c.adjoint && !c.weights) fail(CF_ERR_ARG, "null aggregation weights");
+    if (c.adjoint && c.num->aad_mode == CF_AAD_RECOMPUTE)
+        fail(CF_ERR_UNSUPPORTED, "the recompute AAD variant is not available; use the checkpoint variant");
+
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lk(E.mu);
+    hip_check(hipSetDevice(E.device), "hipSetDevice");
+
+    cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, c.adjoint};
+    const LaunchShape sh = shape_for(E, cfg, md, pd, n_path, first);
+    const int stride = cfk::out_stride(npay);
+    const int ns = int(pd.timeline.size());
+
+    // ---- tables
+    Staging st;
+    const size_t o_ss = st.add(md.step_sample);
+    size_t o_knots = 0, o_invdx = 0, o_iv = 0, o_drift = 0, o_std = 0;
+    if (dup) {
+        if (fp64) {
+            o_knots = st.add(md.log_spots);
+            o_invdx = st.add(md.inv_dx);
+            o_iv = st.add(md.iv);
+        } else {
+            // FP32 state is log(S / S0): knots shifted by log(S0) in FP64 first
+            const double ls0 = std::log(md.spot);
+            vector<float> k(md.n_spots), ix(md.n_spots), v(md.iv.size());
+            for (int j = 0; j < md.n_spots; ++j) {
+                k[j] = float(md.log_spots[j] - ls0);
+                ix[j] = float(md.inv_dx[j]);
+            }
+            for (size_t e = 0; e < v.size(); ++e) v[e] = float(md.iv[e]);
+            o_knots = st.add(k);
+            o_invdx = st.add(ix);
+            o_iv = st.add(v);
+        }
+    } else {
+        if (fp64) {
+            o_drift = st.add(md.drift);
+            o_std = st.add(md.stdv);
+        } else {
+            vector<float> a(md.drift.begin(), md.drift.end()), b(md.stdv.begin(), md.stdv.end());
+            o_drift = st.add(a);
+            o_std = st.add(b);
+        }
+    }
+    const size_t o_ff = st.add(md.ff), o_disc = st.add(md.disc), o_num = st.add(md.num);
+    const size_t o_aux = st.add(md.bs_aux);
+    size_t o_pf = 0, o_pk = 0;
+    if (pd.kind == CF_PRODUCT_EUROPEANS) {
+        o_pf = st.add(pd.pay_first);
+        o_pk = st.add(pd.pay_strike);
+    }
+    vector<double> w(npay, 0.0);
+    if (c.adjoint) w.assign(c.weights, c.weights + npay);
+    const size_t o_w = st.add(w);
+
+    unsigned char* dt = static_cast<unsigned char*>(E.tables.ensure(st.bytes.size()));
+    double* d_block = static_cast<double*>(E.block_out.ensure(size_t(sh.grid) * stride * 8));
+    double* d_red = static_cast<double*>(E.reduced.ensure(size_t(stride) * 8));
+    const size_t ivn = dup ? size_t(md.n_steps) * md.n_spots : 0;
+    double* d_ivb = nullptr;
+    double* d_iv_red = nullptr;
+    void* d_ck = nullptr;
+    if (c.adjoint) {
+        if (dup && ivn) {
+            d_ivb = static_cast<double*>(E.ivbar_blocks.ensure(size_t(sh.grid) * ivn * 8));
+            d_iv_red = static_cast<double*>(E.ivbar.ensure(ivn * 8));
+        }
+        const size_t ck_bytes = size_t(sh.slots) * std::max(md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4);
+        d_ck = E.ckpt.ensure(ck_bytes);
+    }
+    double* d_pp = c.per_path ? static_cast<double*>(E.per_path.ensure(size_t(n_path) * npay * 8)) : nullptr;
+    double* d_ppa = c.per_path_agg ? static_cast<double*>(E.per_path_agg.ensure(size_t(n_path) * 8)) : nullptr;
+
+    hip_check(hipEventRecord(E.ev[0], E.stream), "hipEventRecord");
+    hip_check(hipMemcpyAsync(dt, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, E.stream), "upload");
+
+    cfk::LaunchParams P{};
+    P.first_path = first;
+    P.n_path = n_path;
+    P.n_steps = md.n_steps;
+    P.dirs = E.d_dirs;
+    P.n_spots = md.n_spots;
+    P.knots = dt + o_knots;
+    P.inv_dx = dt + o_invdx;
+    P.iv = dt + o_iv;
+    P.drift = dt + o_drift;
+    P.stdv = dt + o_std;
+    P.spot = md.spot;
+    P.x0 = fp64 ? std::log(md.spot) : 0.0;
+    P.spot_measure = md.spot_measure ? 1 : 0;
+    P.bs_vol = md.vol;
+    P.bs_aux = reinterpret_cast<const double*>(dt + o_aux);
+    P.n_samples = ns;
+    P.step_sample = reinterpret_cast<const int*>(dt + o_ss);
+    P.ff = reinterpret_cast<const double*>(dt + o_ff);
+    P.disc = reinterpret_cast<const double*>(dt + o_disc);
+    P.num = reinterpret_cast<const double*>(dt + o_num);
+    P.n_pay = npay;
+    P.strike = pd.strike;
+    if (pd.kind == CF_PRODUCT_BARRIER) {
+        // mcPrd.h:247-250: width from the spot written at t = 0, no derivative
+        const double smooth = md.s_t0 * pd.smooth;
+        P.two_smooth = 2 * smooth;
+        P.bar_smooth = pd.barrier + smooth;
+        P.minus_smooth = pd.barrier - smooth;
+    }
+    P.put = pd.put ? 1 : 0;
+    P.pay_first = reinterpret_cast<const int*>(dt + o_pf);
+    P.pay_strike = reinterpret_cast<const double*>(dt + o_pk);
+    P.block_out = d_block;
+    P.per_path = d_pp;
+    P.weights = reinterpret_cast<const double*>(dt + o_w);
+    P.ckpt = d_ck;
+    P.ivbar_blocks = d_ivb;
+    P.per_path_agg = d_ppa;
+
+    hip_check(hipEventRecord(E.ev[1], E.stream), "hipEventRecord");
+    int rc = cfk::launch_path_kernel(cfg, P, sh.grid, sh.block, sh.smem, E.stream);
+    if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
+    hip_check(hipEventRecord(E.ev[2], E.stream), "hipEventRecord");
+    rc = cfk::launch_reduce_rows(d_block, sh.grid, stride, d_red, E.stream);
+    if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
+    if (d_ivb) {
+        rc = cfk::launch_reduce_rows(d_ivb, sh.grid, int(ivn), d_iv_red, E.stream);
+        if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
+    }
+    hip_check(hipEventRecord(E.ev[3], E.stream), "hipEventRecord");
+
+    vector<double> red(stride);
+    hip_check(hipMemcpyAsync(red.data(), d_red, size_t(stride) * 8, hipMemcpyDeviceToHost, E.stream), "download");
+    vector<double> ivbar(ivn);
+    if (d_ivb) hip_check(hipMemcpyAsync(ivbar.data(), d_iv_red, ivn * 8, hipMemcpyDeviceToHost, E.stream), "download");
+    if (d_pp)
+        hip_check(hipMemcpyAsync(c.per_path, d_pp, size_t(n_path) * npay * 8, hipMemcpyDeviceToHost, E.stream),
+                  "download");
+    if (d_ppa)
+        hip_check(hipMemcpyAsync(c.per_path_agg, d_ppa, size_t(n_path) * 8, hipMemcpyDeviceToHost, E.stream),
+                  "download");
+    hip_check(hipStreamSynchronize(E.stream), "kernel execution");
+
+    float t_main = 0, t_red = 0, t_all = 0;
+    hip_check(hipEventElapsedTime(&t_main, E.ev[1], E.ev[2]), "event");
+    hip_check(hipEventElapsedTime(&t_red, E.ev[2], E.ev[3]), "event");
+    hip_check(hipEventElapsedTime(&t_all, E.ev[0], E.ev[3]), "event");
+    g_stats.main_kernel_ms = t_main;
+    g_stats.reduce_ms = t_red;
+    g_stats.total_ms = t_all;
+    g_stats.grid_blocks = sh.grid;
+    g_stats.block_threads = sh.block;
+    g_stats.lds_bytes = int(sh.smem);
+    g_stats.path_steps = n_path * int64_t(md.n_steps);
+
+    // ---- results
+    const double np = double(n_path);
+    for (int k = 0; k < npay; ++k) {
+        const double mean = red[k] / np;
+        if (c.means) c.means[k] = mean;
+        if (c.std_errs) {
+            const double var = std::max(0.0, red[npay + k] / np - mean * mean);
+            c.std_errs[k] = std::sqrt(var / np);
+        }
+    }
+    if (c.adjoint) {
+        if (c.agg_mean) c.agg_mean[0] = red[2 * npay] / np;
+        const double* padj = red.data() + 2 * npay + 2;
+        if (dup) {
+            const int nS = md.n_spots, nT = md.n_times;
+            vector<double> r(1 + size_t(nS) * nT, 0.0);
+            r[0] = padj[0] / np;
+            // adjoint of interpVols[i][j] = sqrt(dt_i) * interp_t(vols[j], t_i)
+            for (int i = 0; i < md.n_steps; ++i) {
+                const double sqrtdt = std::sqrt(md.timeline[i + 1] - md.timeline[i]);
+                const Interp1 ip = interp_weights(md.times.data(), nT, md.timeline[i]);
+                for (int j = 0; j < nS; ++j) {
+                    const double a = ivbar[size_t(i) * nS + j] * sqrtdt;
+                    if (a == 0.0) continue;
+                    r[1 + size_t(j) * nT + ip.j0] += a * ip.w0;
+                    if (ip.w1 != 0.0) r[1 + size_t(j) * nT + ip.j1] += a * ip.w1;
+                }
+            }
+            for (size_t k = 1; k < r.size(); ++k) r[k] /= np;
+            std::copy(r.begin(), r.end(), c.risks);
+        } else {
+            for (int k = 0; k < 4; ++k) c.risks[k] = padj[k] / np;
+        }
+    }
+    return CF_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char* cf_last_error(void) { return g_err.c_str(); }
+
+const char* cf_version(void) { return "compfinance-amd 0.1 (gfx950)"; }
+
+int cf_last_kernel_stats(cf_kernel_stats* out)
+{
+    if (!out) {
+        g_err = "null stats";
+        return CF_ERR_ARG;
+    }
+    *out = g_stats;
+    return CF_OK;
+}
+
+int cf_num_payoffs(const cf_product* prd)
+{
+    return guarded([&] { return plan_product(prd).n_pay; });
+}
+
+int cf_num_params(const cf_model* mdl)
+{
+    return guarded([&] {
+        validate_model(mdl);
+        return mdl->kind == CF_MODEL_BLACK_SCHOLES ? 4 : 1 + mdl->n_spots * mdl->n_times;
+    });
+}
+
+static int copy_label(const string& s, char* buf, size_t cap)
+{
+    if (!buf || cap == 0) fail(CF_ERR_ARG, "null label buffer");
+    const size_t n = std::min(s.size(), cap - 1);
+    std::memcpy(buf, s.data(), n);
+    buf[n] = 0;
+    return int(s.size());
+}
+
+int cf_payoff_label(const cf_product* prd, int k, char* buf, size_t cap)
+{
+    return guarded([&] {
+        const auto labels = payoff_labels(plan_product(prd));
+        if (k < 0 || k >= int(labels.size())) fail(CF_ERR_ARG, "payoff index out of range");
+        return copy_label(labels[k], buf, cap);
+    });
+}
+
+int cf_param_label(const cf_model* mdl, int k, char* buf, size_t cap)
+{
+    return guarded([&] {
+        validate_model(mdl);
+        const auto labels = param_labels(mdl);
+        if (k < 0 || k >= int(labels.size())) fail(CF_ERR_ARG, "parameter index out of range");
+        return copy_label(labels[k], buf, cap);
+    });
+}
+
+int cf_product_timeline(const cf_product* prd, double* times, int cap)
+{
+    return guarded([&] {
+        const auto p = plan_product(prd);
+        for (int i = 0; i < int(p.timeline.size()) && i < cap; ++i) times[i] = p.timeline[i];
+        return int(p.timeline.size());
+    });
+}
+
+int cf_sim_timeline(const cf_model* mdl, const cf_product* prd, double* times, int* common, int cap)
+{
+    return guarded([&] {
+        const auto p = plan_product(prd);
+        const auto m = plan_model(mdl, p);
+        for (int i = 0; i < int(m.timeline.size()) && i < cap; ++i) {
+            if (times) times[i] = m.timeline[i];
+            if (common) common[i] = m.step_sample[i] >= 0 ? 1 : 0;
+        }
+        return int(m.timeline.size());
+    });
+}
+
+int cf_mc_simul(const cf_model* mdl, const cf_product* prd, const cf_num* num, double* means, double* std_errs,
+                double* per_path)
+{
+    return guarded([&] {
+        Call c;
+        c.num = num;
+        c.means = means;
+        c.std_errs = std_errs;
+        c.per_path = per_path;
+        if (!means) fail(CF_ERR_ARG, "null means");
+        return run(mdl, prd, c);
+    });
+}
+
+int cf_mc_simul_aad(const cf_model* mdl, const cf_product* prd, const cf_num* num, const double* weights,
+                    double* payoff_means, double* agg_mean, double* risks, double* per_path_payoffs,
+                    double* per_path_agg)
+{
+    return guarded([&] {
+        if (!risks) fail(CF_ERR_ARG, "null risks");
+        Call c;
+        c.adjoint = true;
+        c.num = num;
+        c.weights = weights;
+        c.means = payoff_means;
+        c.agg_mean = agg_mean;
+        c.risks = risks;
+        c.per_path = per_path_payoffs;
+        c.per_path_agg = per_path_agg;
+        return run(mdl, prd, c);
+    });
+}
+
+int64_t cf_aad_footprint(const cf_model* mdl, const cf_product* prd, const cf_num* num)
+{
+    int64_t bytes = 0;
+    const int rc = guarded([&] {
+        validate_num(num);
+        const ProductPlan pd = plan_product(prd);
+        const ModelPlan md = plan_model(mdl, pd);
+        Engine& E = engine();
+        const bool fp64 = num->precision == CF_FP64;
+        cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, true};
+        const LaunchShape sh = shape_for(E, cfg, md, pd, num->num_path, num->first_path);
+        const int64_t ivn = md.kind == CF_MODEL_DUPIRE ? int64_t(md.n_steps) * md.n_spots : 0;
+        bytes = int64_t(sh.slots) * std::max(md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4) +
+                int64_t(sh.grid) * (ivn + cfk::out_stride(pd.n_pay)) * 8;
+        return CF_OK;
+    });
+    return rc == CF_OK ? bytes : int64_t(rc);
+}
+
+}  // extern "C"

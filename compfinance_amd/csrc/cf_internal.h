@@ -1,0 +1,120 @@
+// Launch parameters and shared-memory layout of the path kernels, shared by
+// the host engine and the HIP kernels.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+#ifdef __HIPCC__
+#define CF_HD __host__ __device__
+#else
+#define CF_HD
+#endif
+
+namespace cfk {
+
+enum ModelKind { MODEL_BS = 0, MODEL_DUPIRE = 1 };
+enum ProductKind { PRD_EUROPEAN = 0, PRD_BARRIER = 1, PRD_EUROPEANS = 2 };
+
+constexpr int kWave = 64;
+constexpr int kSobolDims = 1101;     // direction-number table stride
+
+// Per-block accumulator row: payoff sums, payoff sums of squares, aggregate
+// sum, aggregate sum of squares, then up to kMaxParamAdj parameter adjoints
+// accumulated in registers (Dupire: spot; Black-Scholes: spot, vol, rate, div).
+constexpr int kMaxParamAdj = 4;
+CF_HD inline int out_stride(int n_pay) { return 2 * n_pay + 2 + kMaxParamAdj; }
+
+struct LaunchParams {
+    // paths [first_path, first_path + n_path); path p uses Sobol point p + 1
+    int64_t first_path;
+    int64_t n_path;
+    int n_steps;                 // simulation steps = Sobol dimension
+    const uint32_t* dirs;        // [32][kSobolDims] direction numbers
+
+    // model tables (real type of the kernel)
+    int n_spots;                 // Dupire knots
+    const void* knots;           // [n_spots] log spots (FP64) / log(spot/S0) (FP32)
+    const void* inv_dx;          // [n_spots] 1 / (knot[j+1] - knot[j]), last unused
+    const void* iv;              // [n_steps][n_spots] sqrt(dt)-scaled local vols
+    const void* drift;           // Black-Scholes [n_steps]
+    const void* stdv;            // Black-Scholes [n_steps]
+    double spot;                 // S0
+    double x0;                   // initial state: log(S0) (FP64), 0 (FP32)
+    int spot_measure;
+    double bs_vol;               // Black-Scholes volatility (adjoint only)
+    const double* bs_aux;        // Black-Scholes adjoint data: [n_samples] sample
+                                 // time, [n_samples] forward maturity - sample time,
+                                 // [n_samples] discount maturity - sample time,
+                                 // [n_steps] step length
+
+    // product samples on the simulation timeline
+    int n_samples;
+    const int* step_sample;      // [n_steps + 1] sample index at each timeline point, -1 if none
+    const double* ff;            // [n_samples] forward factor (Black-Scholes)
+    const double* disc;          // [n_samples] discount factor
+    const double* num;           // [n_samples] numeraire (spot measure: divided by spot)
+
+    // product
+    int n_pay;
+    double strike;
+    double bar_smooth, minus_smooth, two_smooth;
+    int put;
+    const int* pay_first;        // Europeans [n_samples + 1]: payoffs of sample s
+    const double* pay_strike;    // Europeans [n_pay]
+
+    // outputs
+    double* block_out;           // [gridDim][out_stride(n_pay)]
+    double* per_path;            // [n_path][n_pay] or null
+
+    // adjoint
+    const double* weights;       // [n_pay] aggregation weights
+    void* ckpt;                  // [slots][n_steps][64] path states
+    double* ivbar_blocks;        // [gridDim][n_steps][n_spots] local-vol adjoints
+    double* per_path_agg;        // [n_path] or null
+};
+
+// Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
+struct SmemLayout {
+    size_t lo, hi, step_sample, knots, inv_dx, iv, drift, stdv, samples, pay_first, pay_strike, acc, ivbar, total;
+};
+
+CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int n_pay, int waves,
+                                    int real_bytes, bool dupire, bool europeans, bool adjoint)
+{
+    SmemLayout L{};
+    size_t o = 0;
+    L.lo = o;          o = cf_align16(o + size_t(n_steps) * kWave * 4);
+    L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
+    L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
+    L.knots = o;       o = cf_align16(o + (dupire ? size_t(n_spots) * real_bytes : 0));
+    L.inv_dx = o;      o = cf_align16(o + (dupire ? size_t(n_spots) * real_bytes : 0));
+    L.iv = o;          o = cf_align16(o + (dupire ? size_t(n_steps) * n_spots * real_bytes : 0));
+    L.drift = o;       o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
+    L.stdv = o;        o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
+    L.samples = o;     o = cf_align16(o + (dupire ? 0 : size_t(n_samples) * 3 * 8));
+    L.pay_first = o;   o = cf_align16(o + (europeans ? size_t(n_samples + 1) * 4 : 0));
+    L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
+    L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
+    L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(n_steps) * n_spots * 8 : 0));
+    L.total = o;
+    return L;
+}
+
+// Host-side launch entry points, defined in cf_kernels.hip.
+struct KernelConfig {
+    int precision;   // 0 = FP64, 1 = FP32
+    int model;
+    int product;
+    bool adjoint;
+};
+
+// Returns 0 on success, else a hipError_t value.
+int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block,
+                       size_t smem, void* stream);
+int launch_reduce_rows(const double* in, int rows, int cols, double* out, void* stream);
+int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* blocks_per_cu);
+
+}  // namespace cfk

@@ -1,0 +1,708 @@
+/*
+ * CPU restatement, in FP64, of the reference Monte Carlo hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: the tests, the smoke
+ * entry and the CPU-baseline leg of the benchmark call it to produce expected
+ * values.  No product code may link, load or call it.
+ *
+ * It follows the reference algorithm step by step, in plain C++17:
+ *   Sobol            sobol.h:55-151 (Gray-code next, skipTo), direction numbers
+ *                    regenerated from Joe-Kuo data (sobol.cpp:16-3666)
+ *   inverse normal   gaussians.h:47-87 (Beasley-Springer-Moro)
+ *   interpolation    interp.h:26-63 (upper_bound, linear, flat extrapolation)
+ *   timeline fill    utility.h:11-79 (fillData)
+ *   Dupire           mcMdlDupire.h:165-280 (allocate, init, generatePath)
+ *   Black-Scholes    mcMdlBS.h:149-350
+ *   products         mcPrd.h:29-400 (European, UOC, Europeans), store.h:144-205
+ *   simulation       mcBase.h:267-308 (mcSimul), :427-532 (mcSimulAAD)
+ *   AAD semantics    AADExpr.h:115-253, :364-598 (operator derivatives),
+ *                    AADNode.h:73-82 (propagation), AADTape.h:135-156 (mark)
+ *
+ * Risks are computed with a small operation tape of its own (record the
+ * path, sweep it backward to the mark, sweep the model initialisation once at
+ * the end), i.e. by the same method as the reference and NOT by the
+ * hand-derived adjoint the GPU kernels use, so the two derivations check each
+ * other.  Pinned by the golden vectors under tests/golden (reference build).
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../compfinance_amd/csrc/sobol_init.inc"
+
+namespace orc {
+
+using std::vector;
+
+// ---------------------------------------------------------------- constants
+static const double kOneOver2Pow32 = 2.3283064365387E-10;   // sobol.h:26 (not exactly 2^-32)
+static const double kHalfDay = 0.00136986301369863;         // mcMdlDupire.h:26
+static const double kOneHour = 0.000114469;                 // mcPrd.h:26
+static const double kEps = 1.0e-08;                         // utility.h:5, store.h:153
+
+// ---------------------------------------------------------------- Sobol
+// Direction numbers dir[k * dims + d], k = 0..31, regenerated with the
+// Joe-Kuo recurrence from {degree s, polynomial a, m_1..m_s}.
+static void sobolDirections(int dims, vector<uint32_t>& dir)
+{
+    if (dims < 0 || dims > CF_SOBOL_MAX_DIM) throw std::runtime_error("Sobol dimension out of range");
+    dir.assign(32 * (size_t)dims, 0u);
+    for (int d = 0; d < dims; ++d) {
+        const unsigned s = kSobolInit[d][0], a = kSobolInit[d][1];
+        uint32_t v[32];
+        for (unsigned k = 0; k < 32; ++k) {
+            if (s == 0) {
+                v[k] = 1u << (31 - k);   // van der Corput dimension
+            } else if (k < s) {
+                v[k] = kSobolInit[d][2 + k] << (31 - k);
+            } else {
+                uint32_t x = v[k - s] ^ (v[k - s] >> s);
+                for (unsigned j = 1; j < s; ++j)
+                    if ((a >> (s - 1 - j)) & 1u) x ^= v[k - j];
+                v[k] = x;
+            }
+        }
+        for (int k = 0; k < 32; ++k) dir[(size_t)k * dims + d] = v[k];
+    }
+}
+
+struct Sobol {
+    int dims = 0;
+    vector<uint32_t> dir, state;
+    uint32_t index = 0;
+
+    void init(int d)
+    {
+        dims = d;
+        sobolDirections(d, dir);
+        state.assign(d, 0u);
+        index = 0;
+    }
+    // sobol.h:77-100: XOR the direction number of the rightmost zero bit
+    void next()
+    {
+        uint32_t n = index, j = 0;
+        while (n & 1u) { n >>= 1; ++j; }
+        const uint32_t* v = &dir[(size_t)j * dims];
+        for (int i = 0; i < dims; ++i) state[i] ^= v[i];
+        ++index;
+    }
+    // sobol.h:119-151: state = XOR of direction numbers over the bits of
+    // gray(b); skipTo(0) leaves the generator untouched
+    void skipTo(uint32_t b)
+    {
+        if (!b) return;
+        std::fill(state.begin(), state.end(), 0u);
+        const uint32_t g = b ^ (b >> 1);
+        for (int k = 0; k < 32; ++k)
+            if ((g >> k) & 1u) {
+                const uint32_t* v = &dir[(size_t)k * dims];
+                for (int i = 0; i < dims; ++i) state[i] ^= v[i];
+            }
+        index = b;
+    }
+};
+
+// gaussians.h:47-87, Beasley-Springer-Moro
+static double invNormalCdf(double p)
+{
+    const bool sup = p > 0.5;
+    const double up = sup ? 1.0 - p : p;
+    const double x = up - 0.5;
+    if (std::fabs(x) < 0.42) {
+        const double r = x * x;
+        const double num = x * (((-25.44106049637 * r + 41.39119773534) * r + -18.61500062529) * r + 2.50662823884);
+        const double den = ((((3.13082909833 * r + -21.06224101826) * r + 23.08336743743) * r + -8.47351093090) * r + 1.0);
+        const double z = num / den;
+        return sup ? -z : z;
+    }
+    double r = std::log(-std::log(up));
+    r = 0.3374754822726147 +
+        r * (0.9761690190917186 +
+        r * (0.1607979714918209 +
+        r * (0.0276438810333863 +
+        r * (0.0038405729373609 +
+        r * (0.0003951896511919 +
+        r * (0.0000321767881768 +
+        r * (0.0000002888167364 +
+        r * 0.0000003960315187)))))));
+    return sup ? r : -r;
+}
+
+// utility.h:11-79
+static vector<double> fillData(const vector<double>& original, double maxDx, double minDx,
+                               const double* add, size_t nAdd)
+{
+    vector<double> seq;
+    if (nAdd > 0) {
+        std::set_union(original.begin(), original.end(), add, add + nAdd, std::back_inserter(seq),
+                       [minDx](double x, double y) { return x < y - minDx; });
+    } else {
+        seq = original;
+    }
+    vector<double> out;
+    out.push_back(seq[0]);
+    for (size_t k = 1; k < seq.size(); ++k) {
+        const double cur = out.back(), nxt = seq[k];
+        if (nxt - cur > maxDx) {
+            const int n = int((nxt - cur) / maxDx - kEps) + 1;
+            const double spacing = (nxt - cur) / n;
+            double t = cur + spacing;
+            while (t < nxt - minDx) { out.push_back(t); t += spacing; }
+        }
+        out.push_back(nxt);
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------- tape AD
+// One node per recorded operation with at most two arguments.
+struct Tape {
+    vector<double> der;    // 2 per node
+    vector<int> arg;       // 2 per node, -1 = none
+    vector<double> adj;
+    size_t markPos = 0;
+
+    void clear() { der.clear(); arg.clear(); adj.clear(); markPos = 0; }
+    int record(double d0, int a0, double d1, int a1)
+    {
+        der.push_back(d0); der.push_back(d1);
+        arg.push_back(a0); arg.push_back(a1);
+        adj.push_back(0.0);
+        return int(adj.size()) - 1;
+    }
+    void mark() { markPos = adj.size(); }
+    void rewindToMark()
+    {
+        der.resize(2 * markPos); arg.resize(2 * markPos); adj.resize(markPos);
+    }
+    // propagate node adjoints from `from` down to `to`, both inclusive
+    void propagate(long from, long to)
+    {
+        for (long i = from; i >= to; --i) {
+            const double a = adj[i];
+            if (a == 0.0) continue;
+            for (int k = 0; k < 2; ++k) {
+                const int j = arg[2 * i + k];
+                if (j >= 0) adj[j] += der[2 * i + k] * a;
+            }
+        }
+    }
+};
+static Tape g_tape;
+
+struct Var {
+    double v = 0.0;
+    int i = -1;   // node index, -1 = constant
+    Var() = default;
+    Var(double x) : v(x), i(-1) {}
+    Var(double x, int idx) : v(x), i(idx) {}
+};
+
+static inline Var mk(double v, double d0, int a0, double d1 = 0.0, int a1 = -1)
+{
+    if (a0 < 0 && a1 < 0) return Var(v);
+    return Var(v, g_tape.record(d0, a0, d1, a1));
+}
+static inline Var leaf(double v) { return Var(v, g_tape.record(0.0, -1, 0.0, -1)); }
+
+// AADExpr.h:115-193 (binary), :451-540 (with a double)
+static inline Var operator+(const Var& a, const Var& b) { return mk(a.v + b.v, 1.0, a.i, 1.0, b.i); }
+static inline Var operator-(const Var& a, const Var& b) { return mk(a.v - b.v, 1.0, a.i, -1.0, b.i); }
+static inline Var operator*(const Var& a, const Var& b) { return mk(a.v * b.v, b.v, a.i, a.v, b.i); }
+static inline Var operator/(const Var& a, const Var& b) { return mk(a.v / b.v, 1.0 / b.v, a.i, -a.v / b.v / b.v, b.i); }
+static inline Var operator+(const Var& a, double d) { return mk(a.v + d, 1.0, a.i); }
+static inline Var operator+(double d, const Var& a) { return mk(d + a.v, 1.0, a.i); }
+static inline Var operator-(const Var& a, double d) { return mk(a.v - d, 1.0, a.i); }
+static inline Var operator-(double d, const Var& a) { return mk(d - a.v, -1.0, a.i); }
+static inline Var operator*(const Var& a, double d) { return mk(a.v * d, d, a.i); }
+static inline Var operator*(double d, const Var& a) { return mk(d * a.v, d, a.i); }
+static inline Var operator/(const Var& a, double d) { return mk(a.v / d, 1.0 / d, a.i); }
+static inline Var operator/(double d, const Var& a) { return mk(d / a.v, -d / a.v / a.v, a.i); }
+static inline Var operator-(const Var& a) { return mk(-a.v, -1.0, a.i); }
+// AADExpr.h:364-392
+static inline Var exp(const Var& a) { const double e = std::exp(a.v); return mk(e, e, a.i); }
+static inline Var log(const Var& a) { return mk(std::log(a.v), 1.0 / a.v, a.i); }
+static inline Var sqrt(const Var& a) { const double s = std::sqrt(a.v); return mk(s, 0.5 / s, a.i); }
+// AADExpr.h:571-583: derivative 1 only when strictly greater
+static inline Var max(const Var& a, double d) { return mk(std::max(a.v, d), a.v > d ? 1.0 : 0.0, a.i); }
+static inline Var max(double d, const Var& a) { return max(a, d); }
+
+static inline double val(double x) { return x; }
+static inline double val(const Var& x) { return x.v; }
+using std::exp;
+using std::log;
+using std::sqrt;
+static inline double max(double a, double b) { return std::max(a, b); }
+
+// interp.h:26-63 (linear, flat extrapolation), ys may be active
+template <class Y, class X>
+static Y interp(const double* xs, size_t n, const Y* ys, const X& x0)
+{
+    const double* it = std::upper_bound(xs, xs + n, val(x0));
+    if (it == xs + n) return ys[n - 1];
+    if (it == xs) return ys[0];
+    const size_t k = size_t(it - xs) - 1;
+    const double x1 = xs[k], x2 = xs[k + 1];
+    const Y y1 = ys[k], y2 = ys[k + 1];
+    const auto t = (x0 - x1) / (x2 - x1);
+    return y1 + (y2 - y1) * t;
+}
+
+// ---------------------------------------------------------------- scenarios
+struct SampleDef {
+    bool numeraire = true;
+    vector<double> discountMats;
+    vector<double> forwardMats;   // one asset
+};
+
+template <class T>
+struct Sample {
+    T numeraire;
+    vector<T> discounts;
+    vector<T> forwards;
+};
+
+template <class T>
+static void allocateAndInit(const vector<SampleDef>& defs, vector<Sample<T>>& path)
+{
+    // mcBase.h:80-97: numeraire 1, discounts 1, forwards 100
+    path.resize(defs.size());
+    for (size_t i = 0; i < defs.size(); ++i) {
+        path[i].numeraire = T(1.0);
+        path[i].discounts.assign(defs[i].discountMats.size(), T(1.0));
+        path[i].forwards.assign(defs[i].forwardMats.size(), T(100.0));
+    }
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------- C descriptors
+extern "C" {
+
+typedef struct {
+    int kind;                 // 0 = Black-Scholes, 1 = Dupire
+    double spot;
+    double vol, rate, div;    // Black-Scholes
+    int spot_measure;
+    int n_spots, n_times;     // Dupire local-vol grid
+    const double* spots;
+    const double* times;
+    const double* vols;       // n_spots x n_times, spot major
+    double max_dt;
+} or_model;
+
+typedef struct {
+    int kind;                 // 0 = European, 1 = up-and-out barrier, 2 = Europeans
+    double strike, exercise, settlement;
+    double barrier, maturity, monitor_freq, smooth;
+    int call_put;             // 0 = call, 1 = put (barrier only)
+    int n_options;
+    const double* maturities;
+    const double* strikes;
+} or_product;
+
+}  // extern "C"
+
+namespace orc {
+
+// ---------------------------------------------------------------- products
+struct ProductDef {
+    int kind = 0;
+    vector<double> timeline;
+    vector<SampleDef> defline;
+    // European
+    double strike = 0.0;
+    // UOC
+    double barrier = 0.0, smooth = 0.0;
+    bool put = false;
+    // Europeans (maturity major)
+    vector<vector<double>> strikes;
+    int nPay = 0;
+};
+
+static ProductDef makeProduct(const or_product& p)
+{
+    ProductDef d;
+    d.kind = p.kind;
+    if (p.kind == 0) {
+        // mcPrd.h:44-81
+        d.strike = p.strike;
+        d.timeline = {p.exercise};
+        SampleDef s;
+        s.numeraire = true;
+        s.forwardMats = {p.settlement};
+        s.discountMats = {p.settlement};
+        d.defline = {s};
+        d.nPay = 1;
+    } else if (p.kind == 1) {
+        // mcPrd.h:149-193; store.h:153 (non-positive smoothing -> EPS)
+        d.strike = p.strike;
+        d.barrier = p.barrier;
+        d.smooth = p.smooth <= 0 ? kEps : p.smooth;
+        d.put = p.call_put != 0;
+        d.timeline.push_back(0.0);
+        double t = 0.0 + p.monitor_freq;
+        while (p.maturity - t > kOneHour) { d.timeline.push_back(t); t += p.monitor_freq; }
+        d.timeline.push_back(p.maturity);
+        for (size_t i = 0; i < d.timeline.size(); ++i) {
+            SampleDef s;
+            s.numeraire = false;
+            s.forwardMats = {d.timeline[i]};
+            d.defline.push_back(s);
+        }
+        d.defline.back().numeraire = true;
+        d.nPay = 2;
+    } else if (p.kind == 2) {
+        // store.h:184-205 then mcPrd.h:305-336
+        std::map<double, vector<double>> opts;
+        for (int i = 0; i < p.n_options; ++i) opts[p.maturities[i]].push_back(p.strikes[i]);
+        for (auto& kv : opts) {
+            d.timeline.push_back(kv.first);
+            d.strikes.push_back(kv.second);
+            SampleDef s;
+            s.numeraire = true;
+            s.forwardMats = {kv.first};
+            d.defline.push_back(s);
+            d.nPay += int(kv.second.size());
+        }
+    } else {
+        throw std::runtime_error("unknown product kind");
+    }
+    return d;
+}
+
+template <class T>
+static void payoffs(const ProductDef& d, const vector<Sample<T>>& path, vector<T>& out)
+{
+    if (d.kind == 0) {
+        // mcPrd.h:113-125
+        const auto& s = path.front();
+        out[0] = max(s.forwards[0] - d.strike, 0.0) * s.discounts[0] / s.numeraire;
+    } else if (d.kind == 1) {
+        // mcPrd.h:235-288; smoothing width is a plain double, no derivative
+        const double smooth = val(path.front().forwards[0] * d.smooth);
+        const double twoSmooth = 2 * smooth, barSmooth = d.barrier + smooth, minusSmooth = d.barrier - smooth;
+        T alive(1.0);
+        for (const auto& s : path) {
+            const T& spot = s.forwards[0];
+            if (val(spot) > barSmooth) { alive = T(0.0); break; }
+            else if (val(spot) > minusSmooth) alive = alive * ((barSmooth - spot) / twoSmooth);
+        }
+        const T& fin = path.back().forwards[0];
+        T euro = d.put ? max(d.strike - fin, 0.0) / path.back().numeraire
+                       : max(fin - d.strike, 0.0) / path.back().numeraire;
+        out[0] = alive * euro;
+        out[1] = euro;
+    } else {
+        // mcPrd.h:374-399
+        size_t k = 0;
+        for (size_t i = 0; i < d.timeline.size(); ++i)
+            for (double K : d.strikes[i]) out[k++] = max(path[i].forwards[0] - K, 0.0) / path[i].numeraire;
+    }
+}
+
+// ---------------------------------------------------------------- models
+template <class T>
+struct Model {
+    int kind = 0;
+    // parameters (model order of the reference: BS [spot, vol, rate, div],
+    // Dupire [spot, vols spot-major])
+    vector<T> params;
+    // Black-Scholes
+    bool spotMeasure = false;
+    // Dupire
+    vector<double> spots, logSpots, times;
+    int nS = 0, nT = 0;
+    double maxDt = 0.25;
+
+    // simulation state
+    vector<double> timeline;
+    vector<char> common;
+    bool todayOn = false;
+    vector<T> iv;              // Dupire (step x spot), sqrt(dt) pre-multiplied
+    vector<T> stds, drifts;    // Black-Scholes
+    vector<T> nums, discs, ffs;   // per product sample (one entry per sample)
+
+    const ProductDef* prd = nullptr;
+
+    int simDim() const { return int(timeline.size()) - 1; }
+
+    void allocate(const ProductDef& p)
+    {
+        prd = &p;
+        const double today = 0.0;
+        if (kind == 1) {
+            // mcMdlDupire.h:165-193
+            timeline = fillData(p.timeline, maxDt, kHalfDay, &today, 1);
+            common.resize(timeline.size());
+            for (size_t i = 0; i < timeline.size(); ++i)
+                common[i] = std::binary_search(p.timeline.begin(), p.timeline.end(), timeline[i]);
+        } else {
+            // mcMdlBS.h:149-195
+            timeline.assign(1, today);
+            for (double t : p.timeline) if (t > today) timeline.push_back(t);
+            todayOn = p.timeline[0] == today;
+        }
+    }
+
+    void init()
+    {
+        const ProductDef& p = *prd;
+        const size_t n = timeline.size() - 1;
+        if (kind == 1) {
+            // mcMdlDupire.h:195-217
+            iv.resize(n * nS);
+            for (size_t i = 0; i < n; ++i) {
+                const double sqrtdt = std::sqrt(timeline[i + 1] - timeline[i]);
+                for (int j = 0; j < nS; ++j)
+                    iv[i * nS + j] = sqrtdt * interp(times.data(), size_t(nT), &params[1 + (size_t)j * nT], timeline[i]);
+            }
+        } else {
+            // mcMdlBS.h:197-277
+            const T& spot = params[0];
+            const T& vol = params[1];
+            const T& rate = params[2];
+            const T& div = params[3];
+            const T mu = rate - div;
+            stds.resize(n);
+            drifts.resize(n);
+            for (size_t i = 0; i < n; ++i) {
+                const double dt = timeline[i + 1] - timeline[i];
+                stds[i] = vol * std::sqrt(dt);
+                drifts[i] = spotMeasure ? (mu + 0.5 * vol * vol) * dt : (mu - 0.5 * vol * vol) * dt;
+            }
+            const size_t m = p.timeline.size();
+            nums.assign(m, T(0.0));
+            discs.assign(m, T(1.0));
+            ffs.assign(m, T(1.0));
+            for (size_t i = 0; i < m; ++i) {
+                const SampleDef& def = p.defline[i];
+                if (def.numeraire)
+                    nums[i] = spotMeasure ? exp(div * p.timeline[i]) / spot : exp(rate * p.timeline[i]);
+                if (!def.discountMats.empty()) discs[i] = exp(-rate * (def.discountMats[0] - p.timeline[i]));
+                if (!def.forwardMats.empty()) ffs[i] = exp(mu * (def.forwardMats[0] - p.timeline[i]));
+            }
+        }
+    }
+
+    void generatePath(const vector<double>& g, vector<Sample<T>>& path) const
+    {
+        const ProductDef& p = *prd;
+        if (kind == 1) {
+            // mcMdlDupire.h:238-280; spot written into every forward, the
+            // numeraire and discounts keep their defaults
+            T logspot = log(params[0]);
+            size_t idx = 0;
+            if (common[0]) { const T s = exp(logspot); for (auto& f : path[idx].forwards) f = s; ++idx; }
+            const size_t n = timeline.size() - 1;
+            for (size_t i = 0; i < n; ++i) {
+                const T vol = interp(logSpots.data(), size_t(nS), &iv[i * nS], logspot);
+                logspot = logspot + vol * (-0.5 * vol + g[i]);
+                if (common[i + 1]) { const T s = exp(logspot); for (auto& f : path[idx].forwards) f = s; ++idx; }
+            }
+        } else {
+            // mcMdlBS.h:288-350
+            T spot = params[0];
+            size_t idx = 0;
+            auto fill = [&](size_t k, const T& s) {
+                const SampleDef& def = p.defline[k];
+                if (def.numeraire) {
+                    path[k].numeraire = nums[k];
+                    if (spotMeasure) path[k].numeraire = path[k].numeraire * s;
+                }
+                for (auto& f : path[k].forwards) f = s * ffs[k];
+                for (auto& df : path[k].discounts) df = discs[k];
+            };
+            if (todayOn) { fill(idx, spot); ++idx; }
+            const size_t n = timeline.size() - 1;
+            for (size_t i = 0; i < n; ++i) {
+                spot = spot * exp(drifts[i] + stds[i] * g[i]);
+                fill(idx, spot);
+                ++idx;
+            }
+        }
+    }
+};
+
+template <class T>
+static Model<T> makeModel(const or_model& m)
+{
+    Model<T> md;
+    md.kind = m.kind;
+    if (m.kind == 0) {
+        md.params = {T(m.spot), T(m.vol), T(m.rate), T(m.div)};
+        md.spotMeasure = m.spot_measure != 0;
+    } else if (m.kind == 1) {
+        md.nS = m.n_spots;
+        md.nT = m.n_times;
+        md.spots.assign(m.spots, m.spots + m.n_spots);
+        md.times.assign(m.times, m.times + m.n_times);
+        md.logSpots.resize(md.nS);
+        for (int i = 0; i < md.nS; ++i) md.logSpots[i] = std::log(md.spots[i]);
+        md.params.resize(1 + (size_t)md.nS * md.nT);
+        md.params[0] = T(m.spot);
+        for (size_t k = 0; k < (size_t)md.nS * md.nT; ++k) md.params[1 + k] = T(m.vols[k]);
+        md.maxDt = m.max_dt;
+    } else {
+        throw std::runtime_error("unknown model kind");
+    }
+    return md;
+}
+
+}  // namespace orc
+
+// ---------------------------------------------------------------- C entry points
+extern "C" {
+
+static thread_local std::string g_err;
+const char* or_last_error(void) { return g_err.c_str(); }
+
+int or_num_payoffs(const or_product* p)
+{
+    try { return orc::makeProduct(*p).nPay; } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int or_num_params(const or_model* m)
+{
+    return m->kind == 0 ? 4 : 1 + m->n_spots * m->n_times;
+}
+
+// Simulation timeline and product-date flags; returns its length.
+int or_timeline(const or_model* m, const or_product* p, double* times, int* common, int cap)
+{
+    try {
+        orc::ProductDef d = orc::makeProduct(*p);
+        auto md = orc::makeModel<double>(*m);
+        md.allocate(d);
+        const int n = int(md.timeline.size());
+        for (int i = 0; i < n && i < cap; ++i) {
+            times[i] = md.timeline[i];
+            if (common) common[i] = md.kind == 1 ? md.common[i] : 1;
+        }
+        return n;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// fillData on an arbitrary sorted sequence (utility.h:11-79)
+int or_fill_data(const double* orig, int n, double maxDx, double minDx, const double* add, int nAdd,
+                 double* out, int cap)
+{
+    std::vector<double> o(orig, orig + n);
+    auto f = orc::fillData(o, maxDx, minDx, add, size_t(nAdd));
+    for (size_t i = 0; i < f.size() && int(i) < cap; ++i) out[i] = f[i];
+    return int(f.size());
+}
+
+void or_sobol_dirs(int dims, unsigned* out)
+{
+    std::vector<uint32_t> d;
+    orc::sobolDirections(dims, d);
+    std::memcpy(out, d.data(), d.size() * sizeof(unsigned));
+}
+
+// State after skipTo(b), then after each of n_next calls to next():
+// out[(k) * dims + d], k = 0..n_next.
+int or_sobol_states(int dims, unsigned b, int n_next, unsigned* out)
+{
+    try {
+        orc::Sobol s;
+        s.init(dims);
+        s.skipTo(b);
+        std::memcpy(out, s.state.data(), dims * sizeof(unsigned));
+        for (int k = 1; k <= n_next; ++k) {
+            s.next();
+            std::memcpy(out + (size_t)k * dims, s.state.data(), dims * sizeof(unsigned));
+        }
+        return 0;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+void or_inv_normal(long n, const double* u, double* out)
+{
+    for (long i = 0; i < n; ++i) out[i] = orc::invNormalCdf(u[i]);
+}
+
+// mcSimul (mcBase.h:267-308) over paths [first, first + n): per-path payoffs
+// out[p * nPay + k].
+int or_value(const or_model* m, const or_product* p, long first, long n, double* out)
+{
+    try {
+        orc::ProductDef d = orc::makeProduct(*p);
+        auto md = orc::makeModel<double>(*m);
+        md.allocate(d);
+        md.init();
+        orc::Sobol rng;
+        rng.init(md.simDim());
+        rng.skipTo(uint32_t(first));
+        std::vector<double> g(md.simDim()), pay(d.nPay);
+        std::vector<orc::Sample<double>> path;
+        orc::allocateAndInit(d.defline, path);
+        for (long i = 0; i < n; ++i) {
+            rng.next();
+            for (int k = 0; k < md.simDim(); ++k) g[k] = orc::invNormalCdf(orc::kOneOver2Pow32 * rng.state[k]);
+            md.generatePath(g, path);
+            orc::payoffs(d, path, pay);
+            std::memcpy(out + (size_t)i * d.nPay, pay.data(), d.nPay * sizeof(double));
+        }
+        return 0;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// mcSimulAAD (mcBase.h:427-532): the model initialisation is recorded once
+// before the mark, every path is recorded after it, swept back to the mark
+// and rewound; the initialisation is swept once at the end.  The aggregate is
+// payoff[risk_index] (main.h:135) when weights is NULL, otherwise the inner
+// product with weights (main.h:210-213).  risks are averaged over paths.
+int or_aad(const or_model* m, const or_product* p, long n, int risk_index, const double* weights,
+           double* payoffs_out, double* agg_out, double* risks_out)
+{
+    using orc::Var;
+    try {
+        orc::ProductDef d = orc::makeProduct(*p);
+        if (!weights && (risk_index < 0 || risk_index >= d.nPay)) throw std::runtime_error("payoff not found");
+        orc::g_tape.clear();
+        auto md = orc::makeModel<Var>(*m);
+        for (auto& q : md.params) q = orc::leaf(q.v);
+        md.allocate(d);
+        md.init();
+        std::vector<orc::Sample<Var>> path;
+        orc::allocateAndInit(d.defline, path);
+        orc::g_tape.mark();
+        orc::Sobol rng;
+        rng.init(md.simDim());
+        std::vector<double> g(md.simDim());
+        std::vector<Var> pay(d.nPay);
+        for (long i = 0; i < n; ++i) {
+            orc::g_tape.rewindToMark();
+            rng.next();
+            for (int k = 0; k < md.simDim(); ++k) g[k] = orc::invNormalCdf(orc::kOneOver2Pow32 * rng.state[k]);
+            md.generatePath(g, path);
+            orc::payoffs(d, path, pay);
+            Var res;
+            if (weights) {
+                res = Var(0.0);
+                for (int k = 0; k < d.nPay; ++k) res = res + pay[k] * weights[k];
+            } else {
+                res = pay[risk_index];
+            }
+            if (res.i >= long(orc::g_tape.markPos)) {
+                orc::g_tape.adj[res.i] = 1.0;
+                orc::g_tape.propagate(res.i, long(orc::g_tape.markPos));
+            }
+            if (agg_out) agg_out[i] = res.v;
+            if (payoffs_out) for (int k = 0; k < d.nPay; ++k) payoffs_out[(size_t)i * d.nPay + k] = pay[k].v;
+        }
+        orc::g_tape.propagate(long(orc::g_tape.markPos) - 1, 0);
+        for (size_t j = 0; j < md.params.size(); ++j) risks_out[j] = orc::g_tape.adj[md.params[j].i] / double(n);
+        orc::g_tape.clear();
+        return 0;
+    } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+}  // extern "C"
