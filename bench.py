@@ -1,0 +1,176 @@
+"""Benchmark of the Dupire barrier pricing + AAD risk engine on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (one "step" = one AADriskOne-equivalent call over all paths):
+Dupire local vol on the 30 x 36 grid (1,080 vols), up-and-out call K=120,
+B=150, 5% smoothing, 3y weekly monitoring (156 log-Euler steps), 2^20 Sobol
+paths, price + delta + 1,080 local-vol sensitivities, FP32 (FP64 sums).  The
+same workload in FP64 and the 2^19-path FP64/FP32 pricing run are measured
+beside it.  Inputs are synthetic (the local-vol formula of the
+configuration); all tables are resident on the device before timing.
+
+Prints one JSON line.  `value` is paths*steps per second of the FP32 AAD
+call; `roofline` is for the AAD path kernel (vector ALU bound); the CPU
+baseline times the reference engine (oracle/_ref) on the host cores, or the
+FP64 CPU restatement when the reference build is absent.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import compfinance_amd as cf  # noqa: E402
+from compfinance_amd import workloads as W  # noqa: E402
+
+METRIC = "Dupire price+1080 risks wall-ms & paths·steps/s FP32 vs FP64; serve p99 @QPS"
+
+# Algorithmic work per path-step (FMA = 2 FLOP, FP32 transcendentals weighted
+# 4, FP64 divide 8 and exp/log 20): forward 41/69, adjoint 26/46 FLOP.
+FLOP_PER_PATH_STEP = {("aad", cf.FP32): 67.0, ("aad", cf.FP64): 115.0,
+                      ("price", cf.FP32): 41.0, ("price", cf.FP64): 69.0}
+PEAK_TFLOPS = {cf.FP32: 157.3, cf.FP64: 78.6}   # MI355X vector peaks
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_aad_dupire_barrier_fp32.json")
+
+
+def _time_calls(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    cf.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+        kernel_ms.append(cf.last_kernel_stats()["main_kernel_ms"])
+    cf.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, kernel_ms
+
+
+def _cpu_baseline(n_steps):
+    """Reference engine (oracle/_ref) on the host cores, bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    mdl = po.Model(**W.dupire_args(30))
+    prd = po.Product(**W.barrier_args())
+    cores = os.cpu_count() or 1
+    try:
+        cores = min(cores, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    if po.ref_available():
+        n = 1 << 20
+        try:
+            out = po.ref_aad(mdl, prd, n, risk_index=0, threads=cores, repeats=5)
+            ms = out["wall_ms_median"]
+            return {"value": n * n_steps / (ms / 1e3), "unit": "paths*steps/s", "cores": cores,
+                    "kind": "reference",
+                    "sample": "reference AADriskOne (mcParallelSimulAAD, ThreadPool %d threads), 2^20 paths x %d "
+                              "steps, 30x36 grid, median of 5: %.1f ms" % (cores, n_steps, ms)}
+        except (subprocess.CalledProcessError, OSError, ValueError):
+            pass
+    n = 1 << 14
+    t0 = time.perf_counter()
+    po.aad(mdl, prd, n, 0)
+    dt = time.perf_counter() - t0
+    return {"value": n * n_steps / dt, "unit": "paths*steps/s", "cores": 1, "kind": "port",
+            "sample": "FP64 CPU restatement with tape AAD, 2^14 paths x %d steps, one thread: %.1f s" % (n_steps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the engine shards paths across GPUs only by running one "
+              "process per GPU on disjoint path ranges (first_path), which this benchmark does not launch.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    mdl = cf.Dupire(**W.dupire_args(30))
+    prd = cf.UOC(**W.barrier_args())
+    n3, n2 = W.C3_PATHS, W.C2_PATHS
+    n_steps = int(len(cf.simulation_timeline(mdl, prd)[0]) - 1)
+    w = np.array([1.0, 0.0])
+
+    def aad(prec):
+        return lambda: cf.mcParallelSimulAAD(prd, mdl, weights=w, num=cf.NumericalParam(numPath=n3, precision=prec))
+
+    def price(prec):
+        return lambda: cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=n2, precision=prec), per_path=False)
+
+    # headline: FP32 AAD
+    wall32, k32 = _time_calls(aad(cf.FP32), args.steps, args.warmup)
+    ms32 = 1e3 * wall32 / args.steps
+    r32 = aad(cf.FP32)()
+    # FP64 AAD form (fewer timed calls)
+    s64 = max(3, args.steps // 4)
+    wall64, k64 = _time_calls(aad(cf.FP64), s64, 1)
+    ms64 = 1e3 * wall64 / s64
+    r64 = aad(cf.FP64)()
+    # pricing, C2
+    wp32, kp32 = _time_calls(price(cf.FP32), s64, 1)
+    wp64, kp64 = _time_calls(price(cf.FP64), s64, 1)
+    p32 = price(cf.FP32)()
+    p64 = price(cf.FP64)()
+
+    path_steps = n3 * n_steps
+    kavg = float(np.mean(k32))
+    flop = FLOP_PER_PATH_STEP[("aad", cf.FP32)] * path_steps
+    achieved = flop / (kavg * 1e-3) / 1e12
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        with open(PROFILE_TRAFFIC) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    # FP32 vs FP64 in standard errors of the FP64 price (risk errors reported by the tests)
+    se64 = float(p64.std_errs[0])
+    out = {
+        "metric": METRIC,
+        "value": path_steps / (ms32 * 1e-3),
+        "unit": "paths*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (local-vol formula of the configuration)",
+        "config": {"workload": "Dupire up-and-out call K=120 B=150 3y weekly, 30x36 local-vol grid, price + delta "
+                               "+ 1080 local-vol risks (AADriskOne), FP32 with FP64 sums, checkpoint adjoint",
+                   "paths": n3, "time_steps": n_steps, "risks": int(r32.risks.size)},
+        "fp32_aad_wall_ms": ms32,
+        "fp64_aad_wall_ms": ms64,
+        "fp64_aad_paths_steps_per_s": path_steps / (ms64 * 1e-3),
+        "fp32_over_fp64_aad": ms64 / ms32,
+        "aad_kernel_ms": {"fp32": kavg, "fp64": float(np.mean(k64))},
+        "price_c2": {"paths": n2, "fp32_wall_ms": 1e3 * wp32 / s64, "fp64_wall_ms": 1e3 * wp64 / s64,
+                     "fp32_kernel_ms": float(np.mean(kp32)), "fp64_kernel_ms": float(np.mean(kp64)),
+                     "fp64_value": float(p64.means[0]), "fp64_std_err": se64,
+                     "fp32_minus_fp64_in_se": float((p32.means[0] - p64.means[0]) / se64)},
+        "aad_values": {"fp32": float(r32.agg_mean), "fp64": float(r64.agg_mean),
+                       "delta_fp32": float(r32.risks[0]), "delta_fp64": float(r64.risks[0])},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[cf.FP32], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS[cf.FP32], "traffic": traffic,
+                     "kernel": "path_kernel<float, Dupire, barrier, adjoint>", "kernel_ms": kavg,
+                     "flop_per_path_step": FLOP_PER_PATH_STEP[("aad", cf.FP32)], "path_steps": path_steps},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = _cpu_baseline(n_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -107,6 +107,25 @@ def lib():
             "cf_last_error": (ctypes.c_char_p, []),
             "cf_last_kernel_stats": (ctypes.c_int, [P(cf_kernel_stats)]),
             "cf_version": (ctypes.c_char_p, []),
+            "cf_device_synchronize": (ctypes.c_int, []),
+            "cf_put_black_scholes": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                                    ctypes.c_double, ctypes.c_char_p]),
+            "cf_put_dupire": (ctypes.c_int, [ctypes.c_double, _c_double_p, ctypes.c_int, _c_double_p, ctypes.c_int,
+                                             _c_double_p, ctypes.c_double, ctypes.c_char_p]),
+            "cf_put_european": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p]),
+            "cf_put_barrier": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_int, ctypes.c_char_p]),
+            "cf_put_europeans": (ctypes.c_int, [_c_double_p, _c_double_p, ctypes.c_int, ctypes.c_char_p]),
+            "cf_value": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, P(cf_num), _c_double_p, ctypes.c_int]),
+            "cf_aad_risk_one": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, P(cf_num), ctypes.c_char_p,
+                                               _c_double_p, ctypes.c_int, _c_double_p, _c_double_p, ctypes.c_int]),
+            "cf_aad_risk_aggregate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_char_p),
+                                                     _c_double_p, ctypes.c_int, P(cf_num), _c_double_p, ctypes.c_int,
+                                                     _c_double_p, _c_double_p, ctypes.c_int]),
+            "cf_dupire_aad_risk": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, P(ctypes.c_char_p), _c_double_p,
+                                                  ctypes.c_int, P(cf_num), _c_double_p, _c_double_p, _c_double_p,
+                                                  ctypes.c_int]),
+            "cf_store_clear": (None, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -311,6 +330,11 @@ def simulation_timeline(mdl, prd):
     return t, c.astype(bool)
 
 
+def synchronize():
+    """Wait for the device (bench timing brackets)."""
+    _check(lib().cf_device_synchronize())
+
+
 def last_kernel_stats():
     """Device timings of the last simulation call on this thread."""
     s = cf_kernel_stats()
@@ -390,90 +414,129 @@ def aad_footprint(prd, mdl, num):
 
 
 # ---------------------------------------------------------------- store + entry points
-# store.h:30-36 keeps one registry of models and one of products, keyed by id.
+# The store and the entry points run in the library (C++ mirror of store.h /
+# main.h behind the store-level C ABI); the Python side keeps the objects only
+# to label results.
 
 _model_store = {}
 _product_store = {}
 
 
+def _id(x):
+    return x.encode() if isinstance(x, str) else x
+
+
 def putBlackScholes(spot, vol, qSpot, rate, div, store):
+    """store.h:38-54"""
+    _check(lib().cf_put_black_scholes(float(spot), float(vol), int(bool(qSpot)), float(rate), float(div), _id(store)))
     _model_store[store] = BlackScholes(spot, vol, qSpot, rate, div)
 
 
 def putDupire(spot, spots, times, vols, maxDt, store):
-    _model_store[store] = Dupire(spot, spots, times, vols, maxDt)
+    """store.h:56-73; vols spot major"""
+    m = Dupire(spot, spots, times, vols, maxDt)
+    v = np.ascontiguousarray(m.vols)
+    _check(lib().cf_put_dupire(m.spot, _dptr(m.spots), len(m.spots), _dptr(m.times), len(m.times), _dptr(v),
+                               m.maxDt, _id(store)))
+    _model_store[store] = m
 
 
 def putEuropean(strike, exerciseDate, settlementDate, store):
+    """store.h:128-142"""
+    _check(lib().cf_put_european(float(strike), float(exerciseDate), float(settlementDate or 0.0), _id(store)))
     _product_store[store] = European(strike, exerciseDate, settlementDate)
 
 
 def putBarrier(strike, barrier, maturity, monitorFreq, smooth, callPut, store):
-    _product_store[store] = UOC(strike, barrier, maturity, monitorFreq, smooth, callPut)
+    """store.h:144-163"""
+    _check(lib().cf_put_barrier(float(strike), float(barrier), float(maturity), float(monitorFreq), float(smooth),
+                                int(bool(callPut)), _id(store)))
+    _product_store[store] = UOC(strike, barrier, maturity, monitorFreq, smooth if smooth > 0 else 1e-8, callPut)
 
 
 def putEuropeans(maturities, strikes, store):
-    _product_store[store] = Europeans(maturities, strikes)
+    """store.h:184-205"""
+    p = Europeans(maturities, strikes)
+    _check(lib().cf_put_europeans(_dptr(p.maturities), _dptr(p.strikes), len(p.maturities), _id(store)))
+    _product_store[store] = p
 
 
 def clearStore():
+    lib().cf_store_clear()
     _model_store.clear()
     _product_store.clear()
 
 
-def _get(modelId, productId, who):
-    mdl, prd = _model_store.get(modelId), _product_store.get(productId)
-    if mdl is None or prd is None:
-        raise CfError(-2, f"{who} : Could not retrieve model and product")
-    return mdl, prd
+def _num_desc(num):
+    return ctypes.byref(num.descriptor())
+
+
+def _labels(productId):
+    p = _product_store.get(productId)
+    return p.payoffLabels() if p is not None else []
 
 
 def value(modelId, productId, num):
     """main.h:80-96: {identifiers, values} of the mean payoffs."""
-    mdl, prd = _get(modelId, productId, "value()")
-    res = mcParallelSimul(prd, mdl, num=num, per_path=False)
-    return OrderedDict(identifiers=prd.payoffLabels(), values=res.means, std_errs=res.std_errs)
+    n = num.descriptor()
+    cap = 4096
+    out = np.zeros(cap)
+    npay = _check(lib().cf_value(_id(modelId), _id(productId), ctypes.byref(n), _dptr(out), cap))
+    return OrderedDict(identifiers=_labels(productId), values=out[:npay].copy())
+
+
+def _risk_result(modelId, productId, npar, pay, rv, risks):
+    m = _model_store.get(modelId)
+    return OrderedDict(payoffIds=_labels(productId), payoffValues=pay, riskPayoffValue=float(rv[0]),
+                       paramIds=m.parameterLabels() if m is not None else [], risks=risks[:npar].copy())
+
+
+def _caps(productId, modelId):
+    p, m = _product_store.get(productId), _model_store.get(modelId)
+    npay = len(p.payoffLabels()) if p is not None else 1
+    npar = m.numParams() if m is not None else 1
+    return npay, npar
 
 
 def AADriskOne(modelId, productId, num, riskPayoff=""):
-    """main.h:99-173."""
-    mdl, prd = _get(modelId, productId, "AADrisk()")
-    labels = prd.payoffLabels()
-    idx = 0
-    if riskPayoff:
-        if riskPayoff not in labels:
-            raise CfError(-2, "AADriskOne() : payoff not found")
-        idx = labels.index(riskPayoff)
-    w = np.zeros(len(labels))
-    w[idx] = 1.0
-    return _risk_report(mdl, prd, labels, w, num)
+    """main.h:99-173"""
+    npay, npar = _caps(productId, modelId)
+    n = num.descriptor()
+    pay, rv, risks = np.zeros(npay), np.zeros(1), np.zeros(npar)
+    got = _check(lib().cf_aad_risk_one(_id(modelId), _id(productId), ctypes.byref(n),
+                                       _id(riskPayoff) if riskPayoff else None, _dptr(pay), npay, _dptr(rv),
+                                       _dptr(risks), npar))
+    return _risk_result(modelId, productId, got, pay, rv, risks)
+
+
+def _label_array(labels):
+    arr = (ctypes.c_char_p * max(1, len(labels)))()
+    for i, l in enumerate(labels):
+        arr[i] = _id(l)
+    return arr
 
 
 def AADriskAggregate(modelId, productId, notionals, num):
     """main.h:176-254: notionals {payoff label: notional}."""
-    mdl, prd = _get(modelId, productId, "AADriskAggregate()")
-    labels = prd.payoffLabels()
-    w = np.zeros(len(labels))
-    for lab, nv in notionals.items():
-        if lab not in labels:
-            raise CfError(-2, "AADriskAggregate() : payoff not found")
-        w[labels.index(lab)] = nv
-    return _risk_report(mdl, prd, labels, w, num)
-
-
-def _risk_report(mdl, prd, labels, w, num):
-    res = mcParallelSimulAAD(prd, mdl, weights=w, num=num)
-    return OrderedDict(payoffIds=labels, payoffValues=res.payoff_means, riskPayoffValue=res.agg_mean,
-                       paramIds=mdl.parameterLabels(), risks=res.risks)
+    npay, npar = _caps(productId, modelId)
+    labs = list(notionals.keys())
+    nots = _f64([notionals[k] for k in labs]) if labs else np.zeros(1)
+    n = num.descriptor()
+    pay, rv, risks = np.zeros(npay), np.zeros(1), np.zeros(npar)
+    got = _check(lib().cf_aad_risk_aggregate(_id(modelId), _id(productId), _label_array(labs), _dptr(nots), len(labs),
+                                             ctypes.byref(n), _dptr(pay), npay, _dptr(rv), _dptr(risks), npar))
+    return _risk_result(modelId, productId, got, pay, rv, risks)
 
 
 def dupireAADRisk(modelId, productId, notionals, num):
     """main.h:364-411: value, delta and the spots x times vega matrix."""
-    mdl = _model_store.get(modelId)
-    if mdl is None:
-        raise CfError(-2, "dupireAADRisk() : Model not found")
-    if not isinstance(mdl, Dupire):
-        raise CfError(-1, "dupireAADRisk() : Model not a Dupire")
-    r = AADriskAggregate(modelId, productId, notionals, num)
-    return OrderedDict(value=r["riskPayoffValue"], delta=r["risks"][0],
-                       vega=np.asarray(r["risks"][1:]).reshape(len(mdl.spots), len(mdl.times)))
+    m = _model_store.get(modelId)
+    npar = m.numParams() if m is not None else 1
+    labs = list(notionals.keys())
+    nots = _f64([notionals[k] for k in labs]) if labs else np.zeros(1)
+    n = num.descriptor()
+    val, delta, vega = np.zeros(1), np.zeros(1), np.zeros(max(1, npar - 1))
+    got = _check(lib().cf_dupire_aad_risk(_id(modelId), _id(productId), _label_array(labs), _dptr(nots), len(labs),
+                                          ctypes.byref(n), _dptr(val), _dptr(delta), _dptr(vega), len(vega)))
+    shape = (len(m.spots), len(m.times)) if isinstance(m, Dupire) else (got,)
+    return OrderedDict(value=float(val[0]), delta=float(delta[0]), vega=vega[:got].reshape(shape))

@@ -45,9 +45,6 @@ def build(verbose=False):
             continue
         obj = os.path.join(tmpdir, os.path.splitext(src)[0] + ".o")
         cmd = [hipcc()] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", path, "-o", obj]
-        if src.endswith(".cpp"):
-            cmd = [hipcc()] + [f for f in HIP_FLAGS if f != "--offload-arch=gfx950"] + [
-                "-I", os.path.join(ROOT, "include"), "-c", path, "-o", obj]
         _run(cmd, verbose)
         objs.append(obj)
     tmp_lib = LIB + ".tmp"

@@ -766,6 +766,8 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
 
 }  // namespace
 
+void cfk::set_last_error(const char* msg) { g_err = msg; }
+
 // ------------------------------------------------------------------ C ABI
 
 extern "C" {
@@ -773,6 +775,14 @@ extern "C" {
 const char* cf_last_error(void) { return g_err.c_str(); }
 
 const char* cf_version(void) { return "compfinance-amd 0.1 (gfx950)"; }
+
+int cf_device_synchronize(void)
+{
+    return guarded([&] {
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return CF_OK;
+    });
+}
 
 int cf_last_kernel_stats(cf_kernel_stats* out)
 {

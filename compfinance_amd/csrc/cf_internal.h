@@ -117,4 +117,7 @@ int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid,
 int launch_reduce_rows(const double* in, int rows, int cols, double* out, void* stream);
 int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* blocks_per_cu);
 
+// Sets the thread-local message returned by cf_last_error() (cf_engine.cpp).
+void set_last_error(const char* msg);
+
 }  // namespace cfk

@@ -199,6 +199,10 @@ typedef struct cf_kernel_stats {
 } cf_kernel_stats;
 int cf_last_kernel_stats(cf_kernel_stats* out);
 
+/* Wait for all work the engine queued on the current device (the simulation
+ * calls already return only when their results are on the host). */
+int cf_device_synchronize(void);
+
 /* Library version string. */
 const char* cf_version(void);
 

@@ -1,0 +1,196 @@
+// C++17 mirror of the reference entry points (main.h:33-411) on the MI355X
+// engine: NumericalParam, value, AADriskOne, AADriskAggregate, bumpRisk and
+// dupireAADRisk, with the reference's signatures, result layouts and error
+// messages.  NumericalParam carries two more fields, the precision and the
+// AAD variant, defaulting to the reference's FP64 results.
+//
+// A reference caller switches by including this header instead of main.h;
+// `using namespace cfamd;` brings the names into scope as the reference's
+// global definitions were.
+#pragma once
+
+#include <map>
+
+#include "store.h"
+
+namespace cfamd {
+
+// main.h:33-40 plus engine options
+struct NumericalParam {
+    bool parallel = true;
+    bool useSobol = true;
+    int numPath = 0;
+    int seed1 = 12345;
+    int seed2 = 1234;
+    int precision = CF_FP64;
+    int aadMode = CF_AAD_CHECKPOINT;
+};
+
+inline unique_ptr<RNG> makeRng(const NumericalParam& num)
+{
+    if (!num.useSobol) throw std::runtime_error("only the Sobol generator is implemented on the GPU engine");
+    return std::make_unique<Sobol>();
+}
+
+struct OptionsScope {
+    EngineOptions saved;
+    explicit OptionsScope(const NumericalParam& num) : saved(engineOptions())
+    {
+        engineOptions().precision = num.precision;
+        engineOptions().aadMode = num.aadMode;
+    }
+    ~OptionsScope() { engineOptions() = saved; }
+};
+
+struct ValueResults {
+    vector<string> identifiers;
+    vector<double> values;
+};
+
+// main.h:43-77: mean payoffs, reduced on the device (the per-path matrix is
+// not transferred)
+inline ValueResults value(const Model<double>& model, const Product<double>& product, const NumericalParam& num)
+{
+    if (!checkCompatiblity(product, model)) throw std::runtime_error("Model and product are not compatible");
+    OptionsScope scope(num);
+    auto rng = makeRng(num);
+    const cf_model m = model.descriptor();
+    const cf_product p = product.descriptor();
+    const cf_num n = makeNum(*rng, size_t(num.numPath));
+    ValueResults results;
+    results.identifiers = product.payoffLabels();
+    results.values.resize(results.identifiers.size());
+    cfCheck(cf_mc_simul(&m, &p, &n, results.values.data(), nullptr, nullptr));
+    return results;
+}
+
+// main.h:80-96
+inline ValueResults value(const string& modelId, const string& productId, const NumericalParam& num)
+{
+    const Model<double>* model = getModel<double>(modelId);
+    const Product<double>* product = getProduct<double>(productId);
+    if (!model || !product) throw std::runtime_error("value() : Could not retrieve model and product");
+    return value(*model, *product, num);
+}
+
+struct RiskResults {
+    vector<string> payoffIds;
+    vector<double> payoffValues;
+    double riskPayoffValue = 0.0;
+    vector<string> paramIds;
+    vector<double> risks;
+};
+
+inline RiskResults aadRisk(const Model<Number>& model, const Product<Number>& product, const NumericalParam& num,
+                           const vector<double>& weights)
+{
+    if (!checkCompatiblity(product, model)) throw std::runtime_error("Model and product are not compatible");
+    OptionsScope scope(num);
+    auto rng = makeRng(num);
+    const cf_model m = model.descriptor();
+    const cf_product p = product.descriptor();
+    const cf_num n = makeNum(*rng, size_t(num.numPath));
+    RiskResults results;
+    results.payoffIds = product.payoffLabels();
+    results.payoffValues.resize(results.payoffIds.size());
+    results.paramIds = model.parameterLabels();
+    results.risks.resize(results.paramIds.size());
+    cfCheck(cf_mc_simul_aad(&m, &p, &n, weights.data(), results.payoffValues.data(), &results.riskPayoffValue,
+                            results.risks.data(), nullptr, nullptr));
+    return results;
+}
+
+// main.h:99-173
+inline RiskResults AADriskOne(const string& modelId, const string& productId, const NumericalParam& num,
+                              const string& riskPayoff = "")
+{
+    const Model<Number>* model = getModel<Number>(modelId);
+    const Product<Number>* product = getProduct<Number>(productId);
+    if (!model || !product) throw std::runtime_error("AADrisk() : Could not retrieve model and product");
+    const vector<string>& allPayoffs = product->payoffLabels();
+    size_t idx = 0;
+    if (!riskPayoff.empty()) {
+        auto it = std::find(allPayoffs.begin(), allPayoffs.end(), riskPayoff);
+        if (it == allPayoffs.end()) throw std::runtime_error("AADriskOne() : payoff not found");
+        idx = size_t(std::distance(allPayoffs.begin(), it));
+    }
+    vector<double> w(allPayoffs.size(), 0.0);
+    w[idx] = 1.0;
+    return aadRisk(*model, *product, num, w);
+}
+
+// main.h:176-254
+inline RiskResults AADriskAggregate(const string& modelId, const string& productId,
+                                    const std::map<string, double>& notionals, const NumericalParam& num)
+{
+    const Model<Number>* model = getModel<Number>(modelId);
+    const Product<Number>* product = getProduct<Number>(productId);
+    if (!model || !product) throw std::runtime_error("AADriskAggregate() : Could not retrieve model and product");
+    const vector<string>& allPayoffs = product->payoffLabels();
+    vector<double> vnots(allPayoffs.size(), 0.0);
+    for (const auto& notional : notionals) {
+        auto it = std::find(allPayoffs.begin(), allPayoffs.end(), notional.first);
+        if (it == allPayoffs.end()) throw std::runtime_error("AADriskAggregate() : payoff not found");
+        vnots[size_t(std::distance(allPayoffs.begin(), it))] = notional.second;
+    }
+    return aadRisk(*model, *product, num, vnots);
+}
+
+// main.h:260-266
+struct RiskReports {
+    vector<string> payoffs;
+    vector<string> params;
+    vector<double> values;
+    matrix<double> risks;
+};
+
+// main.h:316-359: one-sided finite differences of every payoff to every
+// parameter, bumping by 1e-8, on the same Sobol paths
+inline RiskReports bumpRisk(const string& modelId, const string& productId, const NumericalParam& num,
+                            const double bump = 1.0e-08)
+{
+    auto* orig = getModel<double>(modelId);
+    const Product<double>* product = getProduct<double>(productId);
+    if (!orig || !product) throw std::runtime_error("bumpRisk() : Could not retrieve model and product");
+    RiskReports results;
+    auto baseRes = value(*orig, *product, num);
+    results.payoffs = baseRes.identifiers;
+    results.values = baseRes.values;
+    auto model = orig->clone();
+    results.params = model->parameterLabels();
+    const vector<double*> parameters = model->parameters();
+    const size_t n = parameters.size(), m = results.payoffs.size();
+    results.risks.resize(n, m);
+    for (size_t i = 0; i < n; ++i) {
+        *parameters[i] += bump;
+        auto bumpRes = value(*model, *product, num);
+        *parameters[i] -= bump;
+        for (size_t j = 0; j < m; ++j) results.risks[i][j] = (bumpRes.values[j] - baseRes.values[j]) / bump;
+    }
+    return results;
+}
+
+struct DupireRisk {
+    double value = 0.0;
+    double delta = 0.0;
+    matrix<double> vega;
+};
+
+// main.h:364-411
+inline DupireRisk dupireAADRisk(const string& modelId, const string& productId,
+                                const std::map<string, double>& notionals, const NumericalParam& num)
+{
+    const Model<Number>* model = getModel<Number>(modelId);
+    if (!model) throw std::runtime_error("dupireAADRisk() : Model not found");
+    const Dupire<Number>* dupire = dynamic_cast<const Dupire<Number>*>(model);
+    if (!dupire) throw std::runtime_error("dupireAADRisk() : Model not a Dupire");
+    DupireRisk results;
+    auto simulResults = AADriskAggregate(modelId, productId, notionals, num);
+    results.value = simulResults.riskPayoffValue;
+    results.delta = simulResults.risks[0];
+    results.vega.resize(dupire->spots().size(), dupire->times().size());
+    std::copy(std::next(simulResults.risks.begin()), simulResults.risks.end(), results.vega.begin());
+    return results;
+}
+
+}  // namespace cfamd

@@ -583,7 +583,9 @@ int or_timeline(const or_model* m, const or_product* p, double* times, int* comm
         const int n = int(md.timeline.size());
         for (int i = 0; i < n && i < cap; ++i) {
             times[i] = md.timeline[i];
-            if (common) common[i] = md.kind == 1 ? md.common[i] : 1;
+            // Black-Scholes: every point after today is a product date, today
+            // only when the product timeline starts there (mcMdlBS.h:163)
+            if (common) common[i] = md.kind == 1 ? md.common[i] : (i > 0 || md.todayOn ? 1 : 0);
         }
         return n;
     } catch (const std::exception& e) { g_err = e.what(); return -1; }

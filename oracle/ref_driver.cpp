@@ -244,8 +244,8 @@ int main(int argc, char** argv)
             printVec("payoff_values", payoffValues);
             std::printf("\"risk_payoff_value\": %.17g,\n", riskValue);
             printVec("risks", risks);
-            std::printf("\"param_ids_head\": [");
-            for (size_t i = 0; i < paramIds.size() && i < 4; ++i)
+            std::printf("\"param_ids\": [");
+            for (size_t i = 0; i < paramIds.size(); ++i)
                 std::printf("%s\"%s\"", i ? ", " : "", paramIds[i].c_str());
             std::printf("],\n");
             (void)nOut;

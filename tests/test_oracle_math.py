@@ -1,0 +1,144 @@
+"""Pins of the CPU oracle that do not rely on the reference: closed forms,
+invariants that a dropped term, a wrong sign or a transposed operand would
+break, and finite differences.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+from scipy.special import ndtri
+from scipy.stats import norm
+
+import pyoracle as po
+from compfinance_amd import workloads as W
+
+
+def _bs_closed(S, K, T, vol, r=0.0, q=0.0):
+    d1 = (math.log(S / K) + (r - q + 0.5 * vol * vol) * T) / (vol * math.sqrt(T))
+    d2 = d1 - vol * math.sqrt(T)
+    price = S * math.exp(-q * T) * norm.cdf(d1) - K * math.exp(-r * T) * norm.cdf(d2)
+    delta = math.exp(-q * T) * norm.cdf(d1)
+    vega = S * math.exp(-q * T) * norm.pdf(d1) * math.sqrt(T)
+    rho = K * T * math.exp(-r * T) * norm.cdf(d2)
+    div = -S * T * math.exp(-q * T) * norm.cdf(d1)
+    return price, np.array([delta, vega, rho, div])
+
+
+def test_black_scholes_price_and_greeks_closed_form():
+    # C1: price 7.965567455405804, delta N(0.1), vega 39.695, rho 46.017, div -53.98
+    m, p = po.Model(spot=100.0, vol=0.2), po.Product(strike=100.0, exerciseDate=1.0)
+    n = 1 << 16
+    pay, agg, risks = po.aad(m, p, n, 0)
+    price, greeks = _bs_closed(100.0, 100.0, 1.0, 0.2)
+    assert abs(price - W.C1_CLOSED_FORM) < 1e-12
+    se = pay[:, 0].std() / math.sqrt(n)
+    assert abs(pay[:, 0].mean() - price) < 3 * se
+    np.testing.assert_allclose(risks, greeks, rtol=2e-3)
+
+
+def test_black_scholes_with_rates_closed_form():
+    m = po.Model(spot=100.0, vol=0.25, rate=0.03, div=0.01)
+    p = po.Product(strike=105.0, exerciseDate=2.0)
+    pay, agg, risks = po.aad(m, p, 1 << 16, 0)
+    price, greeks = _bs_closed(100.0, 105.0, 2.0, 0.25, 0.03, 0.01)
+    assert abs(pay[:, 0].mean() - price) < 5e-3 * price
+    np.testing.assert_allclose(risks, greeks, rtol=5e-3)
+
+
+def _flat_dupire(vol, max_dt=1.0 / 52):
+    spots = np.array([50.0, 100.0, 200.0])
+    times = np.array([0.5, 1.0, 3.0])
+    return po.Model(spot=100.0, spots=spots, times=times, vols=np.full((3, 3), vol), maxDt=max_dt)
+
+
+def test_flat_local_vol_is_black_scholes_path_by_path():
+    # with sigma(S, t) = sigma the log-Euler step equals the exact lognormal
+    # step, so both models give the same path on the same Sobol point
+    prd = po.Product(**W.barrier_args(strike=110.0, barrier=140.0, maturity=1.0))
+    d = po.value_paths(_flat_dupire(0.2), prd, 2048)
+    b = po.value_paths(po.Model(spot=100.0, vol=0.2), prd, 2048)
+    np.testing.assert_allclose(d, b, rtol=1e-11, atol=1e-11)
+
+
+def test_flat_local_vol_vegas_sum_to_black_scholes_vega():
+    prd = po.Product(**W.barrier_args(strike=110.0, barrier=140.0, maturity=1.0))
+    _, _, rd = po.aad(_flat_dupire(0.2), prd, 2048, 0)
+    _, _, rb = po.aad(po.Model(spot=100.0, vol=0.2), prd, 2048, 0)
+    assert abs(rd[0] - rb[0]) < 1e-10 * abs(rb[0])
+    assert abs(rd[1:].sum() - rb[1]) < 1e-9 * abs(rb[1])
+
+
+def test_dupire_risks_against_central_differences():
+    # same Sobol paths, bumps of +-1e-6 on seeded grid nodes and on the spot
+    margs = W.dupire_args(30)
+    prd = po.Product(**W.barrier_args())
+    n = 4096
+    _, _, risks = po.aad(po.Model(**margs), prd, n, 0)
+    scale = np.abs(risks[1:]).max()
+    h = 1e-6
+    nodes = np.random.default_rng(7).choice(margs["vols"].size, size=12, replace=False)
+    nodes = np.concatenate([nodes, np.argsort(-np.abs(risks[1:]))[:4]])
+    for k in nodes:
+        up, dn = (margs["vols"].copy().ravel() for _ in range(2))
+        up[k] += h
+        dn[k] -= h
+        fu = po.value_paths(po.Model(**dict(margs, vols=up.reshape(30, 36))), prd, n)[:, 0].mean()
+        fd = po.value_paths(po.Model(**dict(margs, vols=dn.reshape(30, 36))), prd, n)[:, 0].mean()
+        assert abs((fu - fd) / (2 * h) - risks[1 + k]) < 1e-5 * scale, k
+    # delta: the smoothing width S(0) * smooth is a constant for the reference
+    # tape (mcPrd.h:247), so the bumped runs keep the absolute width fixed
+    hs = 1e-4
+    vals = []
+    for s in (100.0 + hs, 100.0 - hs):
+        bump_prd = po.Product(**W.barrier_args(smooth=0.05 * 100.0 / s))
+        vals.append(po.value_paths(po.Model(**dict(margs, spot=s)), bump_prd, n)[:, 0].mean())
+    assert abs((vals[0] - vals[1]) / (2 * hs) - risks[0]) < 1e-4 * abs(risks[0])
+    # ... while a plain spot bump also moves the width and differs
+    plain = [po.value_paths(po.Model(**dict(margs, spot=s)), prd, n)[:, 0].mean() for s in (100.0 + hs, 100.0 - hs)]
+    assert abs((plain[0] - plain[1]) / (2 * hs) - risks[0]) > 1e-3 * abs(risks[0])
+
+
+def test_aggregate_risk_is_linear_in_notionals():
+    margs = W.dupire_args(30)
+    prd = po.Product(maturities=[1.0, 2.0, 3.0, 3.0], strikes=[100.0, 105.0, 90.0, 120.0])
+    n = 2048
+    w = np.array([0.5, 1.5, -0.7, 2.0])
+    _, _, agg = po.aad(po.Model(**margs), prd, n, weights=w)
+    single = sum(w[k] * po.aad(po.Model(**margs), prd, n, k)[2] for k in range(4))
+    assert np.abs(agg - single).max() < 1e-12 * np.abs(single).max()
+
+
+def test_inverse_normal_accuracy_and_symmetry():
+    u = np.concatenate([np.linspace(1e-9, 1 - 1e-9, 20001), [0.08, 0.92, 0.5, 1e-6, 1 - 1e-6]])
+    z = po.inv_normal(u)
+    # Beasley-Springer-Moro: absolute error about 3e-9 up to 6 standard deviations
+    assert np.abs(z - ndtri(u)).max() < 4e-9
+    k = np.arange(1, 2 ** 20, 997, dtype=np.float64) / 2 ** 20   # 1 - u exact
+    np.testing.assert_array_equal(po.inv_normal(1 - k), -po.inv_normal(k))
+    assert np.all(np.diff(po.inv_normal(np.linspace(1e-6, 1 - 1e-6, 5001))) > 0)
+
+
+@pytest.mark.parametrize("k", [6, 10])
+def test_sobol_is_a_net_in_every_dimension(k):
+    # indices 0 .. 2^k - 1: the top k bits of every coordinate are a permutation
+    pts = po.sobol_points(1101, 0, 2 ** k - 1)
+    top = np.concatenate([np.zeros((1, 1101), dtype=np.uint32), pts]) >> (32 - k)
+    for d in range(1101):
+        assert np.array_equal(np.sort(top[:, d]), np.arange(2 ** k)), d
+
+
+def test_sobol_skip_equals_sequential():
+    seq = po.sobol_points(300, 0, 1000)
+    for b in (1, 63, 64, 65, 511, 777):
+        np.testing.assert_array_equal(po.sobol_points(300, b, 200), seq[b: b + 200])
+
+
+def test_timeline_weekly_barrier():
+    m = po.Model(**W.dupire_args(30))
+    p = po.Product(**W.barrier_args())
+    t, c = po.timeline(m, p)
+    assert len(t) == 157 and c.all()
+    assert t[0] == 0.0 and t[-1] == 3.0
+    assert np.all(np.diff(t) > 0) and np.abs(np.diff(t) - 1 / 52).max() < 1e-12
+    # a European at 3y is filled to maxDt steps
+    t2, c2 = po.timeline(m, po.Product(strike=100.0, exerciseDate=3.0))
+    assert len(t2) == 157 and c2.sum() == 1 and c2[-1]
