@@ -108,6 +108,7 @@ def lib():
             "cf_last_kernel_stats": (ctypes.c_int, [P(cf_kernel_stats)]),
             "cf_version": (ctypes.c_char_p, []),
             "cf_device_synchronize": (ctypes.c_int, []),
+            "cf_diag_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int]),
             "cf_put_black_scholes": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_char_p]),
             "cf_put_dupire": (ctypes.c_int, [ctypes.c_double, _c_double_p, ctypes.c_int, _c_double_p, ctypes.c_int,

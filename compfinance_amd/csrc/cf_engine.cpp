@@ -420,6 +420,46 @@ vector<string> param_labels(const cf_model* m)
     return out;
 }
 
+// ------------------------------------------------------------------ knot search
+
+// Bucket table of the kernels' upper_bound (see KnotSearch in cf_kernels.hip),
+// built with the same real type and the same two rounded operations the
+// kernel uses to map a state to its bucket.
+struct BucketTable {
+    vector<uint16_t> table;
+    int nb = 1, fix = 1;
+    double x0 = 0.0, invh = 0.0;
+};
+
+template <class R>
+BucketTable build_buckets(const vector<R>& knots)
+{
+    BucketTable t;
+    const int m = int(knots.size());
+    const R x0 = knots[0];
+    const double range = double(knots[m - 1]) - double(x0);
+    double dmin = range;
+    for (int j = 0; j + 1 < m; ++j) dmin = std::min(dmin, double(knots[j + 1]) - double(knots[j]));
+    if (m >= 2 && range > 0.0 && dmin > 0.0) t.nb = int(std::min(4096.0, std::ceil(2.0 * range / dmin) + 1.0));
+    const R invh = t.nb > 1 ? R(double(t.nb) / range) : R(0);
+    vector<int> cnt(t.nb, 0);
+    for (int k = 0; k < m; ++k) {
+        R bf = (knots[k] - x0) * invh;
+        bf = bf < R(t.nb - 1) ? bf : R(t.nb - 1);
+        cnt[int(bf)]++;
+    }
+    t.table.assign(t.nb, 0);
+    int acc = 0;
+    for (int b = 0; b < t.nb; ++b) {
+        t.table[b] = uint16_t(acc);
+        acc += cnt[b];
+        t.fix = std::max(t.fix, cnt[b]);
+    }
+    t.x0 = double(x0);
+    t.invh = double(invh);
+    return t;
+}
+
 // ------------------------------------------------------------------ device
 
 // Grow-only device buffer.
@@ -538,18 +578,23 @@ struct LaunchShape {
     int slots = 0;
 };
 
-LaunchShape shape_for(Engine& E, const cfk::KernelConfig& cfg, const ModelPlan& md, const ProductPlan& pd,
-                      int64_t n_path, int64_t first)
+LaunchShape shape_for(Engine& E, cfk::KernelConfig& cfg, const ModelPlan& md, const ProductPlan& pd,
+                      int64_t n_path, int64_t first, int n_buckets)
 {
     LaunchShape s;
     const int waves = s.block / cfk::kWave;
-    const cfk::SmemLayout L =
-        cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves,
-                         cfg.precision == CF_FP64 ? 8 : 4, md.kind == CF_MODEL_DUPIRE,
-                         pd.kind == CF_PRODUCT_EUROPEANS, cfg.adjoint);
-    s.smem = L.total;
     int max_lds = 0;
     hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, E.device), "attribute");
+    // FULL layout when it fits, LEAN otherwise
+    for (int lean = 0; lean < 2; ++lean) {
+        cfg.lean = lean != 0;
+        const cfk::SmemLayout L =
+            cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves,
+                             cfg.precision == CF_FP64 ? 8 : 4, md.kind == CF_MODEL_DUPIRE,
+                             pd.kind == CF_PRODUCT_EUROPEANS, cfg.adjoint, cfg.lean, n_buckets);
+        s.smem = L.total;
+        if (s.smem <= size_t(max_lds)) break;
+    }
     if (s.smem > size_t(max_lds))
         fail(CF_ERR_CAPACITY, "tables need " + std::to_string(s.smem) + " bytes of LDS per block, device allows " +
                                   std::to_string(max_lds));
@@ -582,8 +627,21 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     std::lock_guard<std::mutex> lk(E.mu);
     hip_check(hipSetDevice(E.device), "hipSetDevice");
 
-    cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, c.adjoint};
-    const LaunchShape sh = shape_for(E, cfg, md, pd, n_path, first);
+    // knot search tables in the kernel's precision
+    BucketTable bt;
+    vector<float> knots_f;
+    if (dup) {
+        if (fp64) {
+            bt = build_buckets(md.log_spots);
+        } else {
+            const double ls0 = std::log(md.spot);
+            knots_f.resize(md.n_spots);
+            for (int j = 0; j < md.n_spots; ++j) knots_f[j] = float(md.log_spots[j] - ls0);
+            bt = build_buckets(knots_f);
+        }
+    }
+    cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, c.adjoint, false};
+    const LaunchShape sh = shape_for(E, cfg, md, pd, n_path, first, dup ? bt.nb : 0);
     const int stride = cfk::out_stride(npay);
     const int ns = int(pd.timeline.size());
 
@@ -598,14 +656,10 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
             o_iv = st.add(md.iv);
         } else {
             // FP32 state is log(S / S0): knots shifted by log(S0) in FP64 first
-            const double ls0 = std::log(md.spot);
-            vector<float> k(md.n_spots), ix(md.n_spots), v(md.iv.size());
-            for (int j = 0; j < md.n_spots; ++j) {
-                k[j] = float(md.log_spots[j] - ls0);
-                ix[j] = float(md.inv_dx[j]);
-            }
+            vector<float> ix(md.n_spots), v(md.iv.size());
+            for (int j = 0; j < md.n_spots; ++j) ix[j] = float(md.inv_dx[j]);
             for (size_t e = 0; e < v.size(); ++e) v[e] = float(md.iv[e]);
-            o_knots = st.add(k);
+            o_knots = st.add(knots_f);
             o_invdx = st.add(ix);
             o_iv = st.add(v);
         }
@@ -619,6 +673,7 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
             o_std = st.add(b);
         }
     }
+    const size_t o_bucket = st.add(bt.table);
     const size_t o_ff = st.add(md.ff), o_disc = st.add(md.disc), o_num = st.add(md.num);
     const size_t o_aux = st.add(md.bs_aux);
     size_t o_pf = 0, o_pk = 0;
@@ -660,6 +715,11 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     P.knots = dt + o_knots;
     P.inv_dx = dt + o_invdx;
     P.iv = dt + o_iv;
+    P.bucket = reinterpret_cast<const uint16_t*>(dt + o_bucket);
+    P.n_buckets = dup ? bt.nb : 0;
+    P.bucket_fix = bt.fix;
+    P.bucket_x0 = bt.x0;
+    P.bucket_invh = bt.invh;
     P.drift = dt + o_drift;
     P.stdv = dt + o_std;
     P.spot = md.spot;
@@ -775,6 +835,27 @@ extern "C" {
 const char* cf_last_error(void) { return g_err.c_str(); }
 
 const char* cf_version(void) { return "compfinance-amd 0.1 (gfx950)"; }
+
+int cf_diag_copy(size_t bytes, int reps)
+{
+    return guarded([&] {
+        if (bytes < 16 || reps < 1) fail(CF_ERR_ARG, "bad copy size");
+        Engine& E = engine();
+        std::lock_guard<std::mutex> lk(E.mu);
+        void *a = nullptr, *b = nullptr;
+        hip_check(hipMalloc(&a, bytes), "hipMalloc");
+        hip_check(hipMalloc(&b, bytes), "hipMalloc");
+        hip_check(hipMemsetAsync(a, 1, bytes, E.stream), "hipMemset");
+        for (int r = 0; r < reps; ++r) {
+            const int rc = cfk::launch_copy(a, b, bytes, E.stream);
+            if (rc != 0) hip_check(hipError_t(rc), "copy launch");
+        }
+        hip_check(hipStreamSynchronize(E.stream), "copy");
+        hip_check(hipFree(a), "hipFree");
+        hip_check(hipFree(b), "hipFree");
+        return CF_OK;
+    });
+}
 
 int cf_device_synchronize(void)
 {
@@ -899,8 +980,8 @@ int64_t cf_aad_footprint(const cf_model* mdl, const cf_product* prd, const cf_nu
         const ModelPlan md = plan_model(mdl, pd);
         Engine& E = engine();
         const bool fp64 = num->precision == CF_FP64;
-        cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, true};
-        const LaunchShape sh = shape_for(E, cfg, md, pd, num->num_path, num->first_path);
+        cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, true, false};
+        const LaunchShape sh = shape_for(E, cfg, md, pd, num->num_path, num->first_path, 4096);
         const int64_t ivn = md.kind == CF_MODEL_DUPIRE ? int64_t(md.n_steps) * md.n_spots : 0;
         bytes = int64_t(sh.slots) * std::max(md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4) +
                 int64_t(sh.grid) * (ivn + cfk::out_stride(pd.n_pay)) * 8;

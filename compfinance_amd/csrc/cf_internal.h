@@ -37,6 +37,9 @@ struct LaunchParams {
     const void* knots;           // [n_spots] log spots (FP64) / log(spot/S0) (FP32)
     const void* inv_dx;          // [n_spots] 1 / (knot[j+1] - knot[j]), last unused
     const void* iv;              // [n_steps][n_spots] sqrt(dt)-scaled local vols
+    const uint16_t* bucket;      // [n_buckets] knot search table
+    int n_buckets, bucket_fix;   // table size, knots per bucket at most
+    double bucket_x0, bucket_invh;   // exactly representable in the kernel's real type
     const void* drift;           // Black-Scholes [n_steps]
     const void* stdv;            // Black-Scholes [n_steps]
     double spot;                 // S0
@@ -76,22 +79,29 @@ struct LaunchParams {
 
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
-    size_t lo, hi, step_sample, knots, inv_dx, iv, drift, stdv, samples, pay_first, pay_strike, acc, ivbar, total;
+    size_t lo, hi, step_sample, knots, inv_dx, bucket, iv, drift, stdv, samples, pay_first, pay_strike, acc, ivbar, total;
 };
 
 CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+// Two layouts.  FULL keeps the per-lane low Sobol words (one 32-bit word per
+// dimension and lane) and the local-vol table in LDS.  LEAN, for long
+// timelines or large grids, keeps only direction numbers 0..5 of each
+// dimension (the low word is formed in registers) and reads the local-vol
+// table from global memory through the caches.
 CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int n_pay, int waves,
-                                    int real_bytes, bool dupire, bool europeans, bool adjoint)
+                                    int real_bytes, bool dupire, bool europeans, bool adjoint, bool lean,
+                                    int n_buckets)
 {
     SmemLayout L{};
     size_t o = 0;
-    L.lo = o;          o = cf_align16(o + size_t(n_steps) * kWave * 4);
+    L.lo = o;          o = cf_align16(o + size_t(n_steps) * (lean ? 8 : kWave) * 4);
     L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
     L.knots = o;       o = cf_align16(o + (dupire ? size_t(n_spots) * real_bytes : 0));
     L.inv_dx = o;      o = cf_align16(o + (dupire ? size_t(n_spots) * real_bytes : 0));
-    L.iv = o;          o = cf_align16(o + (dupire ? size_t(n_steps) * n_spots * real_bytes : 0));
+    L.bucket = o;      o = cf_align16(o + (dupire ? size_t(n_buckets) * 2 : 0));
+    L.iv = o;          o = cf_align16(o + (dupire && !lean ? size_t(n_steps) * n_spots * real_bytes : 0));
     L.drift = o;       o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
     L.stdv = o;        o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
     L.samples = o;     o = cf_align16(o + (dupire ? 0 : size_t(n_samples) * 3 * 8));
@@ -109,6 +119,7 @@ struct KernelConfig {
     int model;
     int product;
     bool adjoint;
+    bool lean;       // LEAN shared-memory layout
 };
 
 // Returns 0 on success, else a hipError_t value.
@@ -116,6 +127,8 @@ int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid,
                        size_t smem, void* stream);
 int launch_reduce_rows(const double* in, int rows, int cols, double* out, void* stream);
 int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* blocks_per_cu);
+
+int launch_copy(const void* in, void* out, size_t bytes, void* stream);
 
 // Sets the thread-local message returned by cf_last_error() (cf_engine.cpp).
 void set_last_error(const char* msg);

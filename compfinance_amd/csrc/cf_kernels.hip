@@ -117,15 +117,38 @@ struct Bracket {
     R wa, wb;     // adjoint weights of nodes j and j + 1
 };
 
+// upper_bound on the knots (interp.h:40) through a uniform bucket table:
+// bucket b = min(floor((x - x0) * invh), nb - 1) holds the number of knots
+// in earlier buckets, a lower bound of the answer; at most `fix` knots fall in
+// one bucket and are resolved by comparisons.  The host builds the table with
+// the kernel's own arithmetic, so the answer is exactly upper_bound's.
 template <typename R>
-__device__ __forceinline__ Bracket<R> local_vol(const R* knots, const R* inv_dx, const R* row, int m, int top, R x)
+struct KnotSearch {
+    const R* knots;
+    const uint16_t* table;
+    int m, nb, fix;
+    R x0, invh;
+};
+
+template <typename R>
+__device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
 {
-    int lo = 0;   // number of knots <= x
-    for (int step = top; step > 0; step >>= 1) {
-        const int idx = lo + step;
-        const int c = (idx <= m ? idx : m) - 1;
-        if (idx <= m && knots[c] <= x) lo = idx;
+    if (!(x >= s.x0)) return 0;
+    R bf = (x - s.x0) * s.invh;
+    bf = bf < R(s.nb - 1) ? bf : R(s.nb - 1);
+    int lo = s.table[int(bf)];
+    for (int k = 0; k < s.fix; ++k) {
+        const int c = lo < s.m ? lo : s.m - 1;
+        if (lo < s.m && s.knots[c] <= x) ++lo;
     }
+    return lo;
+}
+
+template <typename R>
+__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x)
+{
+    const int m = ks.m;
+    const int lo = count_le(ks, x);   // number of knots <= x
     int j = lo - 1;
     j = j < 0 ? 0 : j;
     j = j > m - 2 ? m - 2 : j;
@@ -133,7 +156,7 @@ __device__ __forceinline__ Bracket<R> local_vol(const R* knots, const R* inv_dx,
     const int j2 = j + 1 < m ? j + 1 : j;
     const R y1 = row[j], y2 = row[j2];
     const bool inner = lo > 0 && lo < m;
-    const R t = (x - knots[j]) * inv_dx[j];
+    const R t = (x - ks.knots[j]) * inv_dx[j];
     Bracket<R> b;
     b.j = j;
     if (inner) {
@@ -150,6 +173,9 @@ __device__ __forceinline__ Bracket<R> local_vol(const R* knots, const R* inv_dx,
     }
     return b;
 }
+
+__device__ __forceinline__ double recip_mul(double a, double b) { return a / b; }
+__device__ __forceinline__ float recip_mul(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // ------------------------------------------------------------------ products
 
@@ -168,7 +194,7 @@ struct ProductState {
 
 // ------------------------------------------------------------------ kernel
 
-template <typename R, int MODEL, int PRD, bool ADJ>
+template <typename R, int MODEL, int PRD, bool ADJ, bool LEAN>
 __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -177,14 +203,15 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
     const int waves = blockDim.x >> 6;
     const int n = P.n_steps, m = P.n_spots, ns = P.n_samples, npay = P.n_pay;
     const int stride = out_stride(npay);
-    const SmemLayout L = smem_layout(n, m, ns, npay, waves, int(sizeof(R)), DUP, EUS, ADJ);
+    const SmemLayout L = smem_layout(n, m, ns, npay, waves, int(sizeof(R)), DUP, EUS, ADJ, LEAN, P.n_buckets);
 
     uint32_t* lo_tab = reinterpret_cast<uint32_t*>(smem + L.lo);
     uint32_t* hi_all = reinterpret_cast<uint32_t*>(smem + L.hi);
     int* step_sample = reinterpret_cast<int*>(smem + L.step_sample);
     R* knots = reinterpret_cast<R*>(smem + L.knots);
     R* inv_dx = reinterpret_cast<R*>(smem + L.inv_dx);
-    R* iv = reinterpret_cast<R*>(smem + L.iv);
+    uint16_t* bucket = reinterpret_cast<uint16_t*>(smem + L.bucket);
+    R* iv_lds = reinterpret_cast<R*>(smem + L.iv);
     R* drift = reinterpret_cast<R*>(smem + L.drift);
     R* stdv = reinterpret_cast<R*>(smem + L.stdv);
     double* s_ff = reinterpret_cast<double*>(smem + L.samples);
@@ -198,14 +225,21 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
     const int tid = threadIdx.x, nthr = blockDim.x;
 
     // ---- stage the block's tables in LDS
-    for (int e = tid; e < n * kWave; e += nthr) {
-        const int d = e >> 6, r = e & 63;
-        const uint32_t g = uint32_t(r ^ (r >> 1));
-        uint32_t x = 0;
+    if constexpr (LEAN) {
+        for (int e = tid; e < n * 8; e += nthr) {
+            const int d = e >> 3, k = e & 7;
+            lo_tab[e] = k < 6 ? P.dirs[k * kSobolDims + d] : 0u;
+        }
+    } else {
+        for (int e = tid; e < n * kWave; e += nthr) {
+            const int d = e >> 6, r = e & 63;
+            const uint32_t g = uint32_t(r ^ (r >> 1));
+            uint32_t x = 0;
 #pragma unroll
-        for (int k = 0; k < 6; ++k)
-            if ((g >> k) & 1u) x ^= P.dirs[k * kSobolDims + d];
-        lo_tab[e] = x;
+            for (int k = 0; k < 6; ++k)
+                if ((g >> k) & 1u) x ^= P.dirs[k * kSobolDims + d];
+            lo_tab[e] = x;
+        }
     }
     for (int e = tid; e <= n; e += nthr) step_sample[e] = P.step_sample[e];
     if constexpr (DUP) {
@@ -213,7 +247,9 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
         const R* gi = static_cast<const R*>(P.inv_dx);
         const R* gv = static_cast<const R*>(P.iv);
         for (int e = tid; e < m; e += nthr) { knots[e] = gk[e]; inv_dx[e] = gi[e]; }
-        for (int e = tid; e < n * m; e += nthr) iv[e] = gv[e];
+        for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = P.bucket[e];
+        if constexpr (!LEAN)
+            for (int e = tid; e < n * m; e += nthr) iv_lds[e] = gv[e];
     } else {
         const R* gd = static_cast<const R*>(P.drift);
         const R* gs = static_cast<const R*>(P.stdv);
@@ -230,14 +266,40 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
     __syncthreads();
 
     const int wave = tid >> 6, lane = tid & 63;
+    // local-vol table: LDS (FULL) or global memory (LEAN)
+    const R* iv;
+    if constexpr (LEAN) iv = static_cast<const R*>(P.iv); else iv = iv_lds;
+    // LEAN: masks of the bits of gray6(lane), selecting direction numbers 0..5
+    uint32_t lmask[6];
+    {
+        const uint32_t g6 = uint32_t(lane ^ (lane >> 1));
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lmask[k] = ((g6 >> k) & 1u) ? 0xffffffffu : 0u;
+    }
+    // low Sobol word of dimension i for this lane
+    auto low_word = [&](int i) -> uint32_t {
+        if constexpr (LEAN) {
+            const uint32_t* v = lo_tab + i * 8;
+            return (lmask[0] & v[0]) ^ (lmask[1] & v[1]) ^ (lmask[2] & v[2]) ^ (lmask[3] & v[3]) ^
+                   (lmask[4] & v[4]) ^ (lmask[5] & v[5]);
+        } else {
+            return lo_tab[i * kWave + lane];
+        }
+    };
     uint32_t* hi = hi_all + wave * n;
     double* accw = acc + wave * stride;
     const int64_t nlo = P.first_path + 1, nhi = P.first_path + P.n_path;   // Sobol indices
     const int64_t q0 = nlo >> 6, q1 = nhi >> 6;
     const int slot = blockIdx.x * waves + wave;
     R* ck = ADJ ? static_cast<R*>(P.ckpt) + size_t(slot) * n * kWave : nullptr;
-    int top = 1;
-    while (top * 2 <= m) top *= 2;
+    KnotSearch<R> ks;
+    ks.knots = knots;
+    ks.table = bucket;
+    ks.m = m;
+    ks.nb = P.n_buckets;
+    ks.fix = P.bucket_fix;
+    ks.x0 = R(P.bucket_x0);
+    ks.invh = R(P.bucket_invh);
     const double s0 = P.spot;
     const int last = ns - 1;
 
@@ -326,11 +388,11 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
             if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
         }
         for (int i = 0; i < n; ++i) {
-            const uint32_t u = hi[i] ^ lo_tab[i * kWave + lane];
+            const uint32_t u = hi[i] ^ low_word(i);
             const R g = inv_normal(u, R());
             if constexpr (DUP) {
                 if constexpr (ADJ) ck[i * kWave + lane] = x;
-                const Bracket<R> b = local_vol(knots, inv_dx, iv + i * m, m, top, x);
+                const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, x);
                 x = x + b.vol * (R(-0.5) * b.vol + g);     // mcMdlDupire.h:271
             } else {
                 if constexpr (ADJ) ck[i * kWave + lane] = g;
@@ -410,12 +472,20 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
                         abar = S * Fb;
                     }
                 }
+                R xnext = x;     // state at point i + 1
                 for (int i = n - 1; i >= 0; --i) {
                     const R xi = ck[i * kWave + lane];
-                    const uint32_t u = hi[i] ^ lo_tab[i * kWave + lane];
-                    const R g = inv_normal(u, R());
-                    const Bracket<R> b = local_vol(knots, inv_dx, iv + i * m, m, top, xi);
-                    const R w = g - b.vol;                  // d x_{i+1} / d vol
+                    const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
+                    // d x_{i+1} / d vol = g - vol, recovered from the two states:
+                    // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
+                    R w;
+                    if (b.vol != R(0)) {
+                        w = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
+                    } else {
+                        const uint32_t u = hi[i] ^ low_word(i);
+                        w = inv_normal(u, R());
+                    }
+                    xnext = xi;
                     const R vbar = abar * w;
                     abar = abar + vbar * b.slope;
                     if (active && vbar != R(0)) {
@@ -515,35 +585,55 @@ __global__ void reduce_rows_kernel(const double* in, int rows, int cols, double*
     out[c] = s;
 }
 
+// Plain streaming copy, 16 bytes per lane per iteration (diagnostics).
+__global__ void copy_kernel(const uint4* __restrict__ in, uint4* __restrict__ out, size_t n)
+{
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        out[i] = in[i];
+}
+
+int launch_copy(const void* in, void* out, size_t bytes, void* stream)
+{
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const uint4*>(in), static_cast<uint4*>(out), n);
+    return int(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ dispatch
 
 template <typename R, int MODEL, int PRD, bool ADJ>
-static const void* kernel_ptr()
+static const void* kernel_ptr(bool lean)
 {
-    return reinterpret_cast<const void*>(&path_kernel<R, MODEL, PRD, ADJ>);
+    return lean ? reinterpret_cast<const void*>(&path_kernel<R, MODEL, PRD, ADJ, true>)
+                : reinterpret_cast<const void*>(&path_kernel<R, MODEL, PRD, ADJ, false>);
 }
 
 template <typename R, int MODEL, bool ADJ>
-static const void* pick_product(int prd)
+static const void* pick_product(int prd, bool lean)
 {
     switch (prd) {
-    case PRD_EUROPEAN: return kernel_ptr<R, MODEL, PRD_EUROPEAN, ADJ>();
-    case PRD_BARRIER: return kernel_ptr<R, MODEL, PRD_BARRIER, ADJ>();
-    case PRD_EUROPEANS: return kernel_ptr<R, MODEL, PRD_EUROPEANS, ADJ>();
+    case PRD_EUROPEAN: return kernel_ptr<R, MODEL, PRD_EUROPEAN, ADJ>(lean);
+    case PRD_BARRIER: return kernel_ptr<R, MODEL, PRD_BARRIER, ADJ>(lean);
+    case PRD_EUROPEANS: return kernel_ptr<R, MODEL, PRD_EUROPEANS, ADJ>(lean);
     }
     return nullptr;
 }
 
 template <typename R, bool ADJ>
-static const void* pick_model(int model, int prd)
+static const void* pick_model(int model, int prd, bool lean)
 {
-    return model == MODEL_DUPIRE ? pick_product<R, MODEL_DUPIRE, ADJ>(prd) : pick_product<R, MODEL_BS, ADJ>(prd);
+    return model == MODEL_DUPIRE ? pick_product<R, MODEL_DUPIRE, ADJ>(prd, lean)
+                                 : pick_product<R, MODEL_BS, ADJ>(prd, lean);
 }
 
 static const void* pick(const KernelConfig& c)
 {
-    if (c.precision == 0) return c.adjoint ? pick_model<double, true>(c.model, c.product) : pick_model<double, false>(c.model, c.product);
-    return c.adjoint ? pick_model<float, true>(c.model, c.product) : pick_model<float, false>(c.model, c.product);
+    if (c.precision == 0)
+        return c.adjoint ? pick_model<double, true>(c.model, c.product, c.lean)
+                         : pick_model<double, false>(c.model, c.product, c.lean);
+    return c.adjoint ? pick_model<float, true>(c.model, c.product, c.lean)
+                     : pick_model<float, false>(c.model, c.product, c.lean);
 }
 
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block, size_t smem, void* stream)

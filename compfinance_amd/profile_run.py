@@ -1,0 +1,52 @@
+"""Runs benchmark workloads a fixed number of times, for profilers.
+
+    python -m compfinance_amd.profile_run --workload c3_fp32_aad --calls 3
+    rocprofv3 --kernel-trace --stats -f csv -d OUT -- python3 compfinance_amd/profile_run.py --workload all
+
+Workloads: c3_fp32_aad, c3_fp64_aad (Dupire barrier, 30x36 grid, 2^20 paths,
+156 steps, price + 1,081 risks), c2_fp32, c2_fp64 (pricing, 2^19 paths),
+copy (a 1 GiB device-to-device streaming copy, the byte-counter calibration),
+all (every one of them).
+"""
+import argparse
+import os
+import sys
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+import compfinance_amd as cf  # noqa: E402
+from compfinance_amd import workloads as W  # noqa: E402
+
+COPY_BYTES = 1 << 30
+
+
+def run(name, calls):
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    for _ in range(calls):
+        if name == "copy":
+            cf._check(cf.lib().cf_diag_copy(COPY_BYTES, 1))
+        elif name.startswith("c3"):
+            prec = cf.FP32 if "fp32" in name else cf.FP64
+            cf.mcParallelSimulAAD(prd, mdl, weights=[1.0, 0.0], num=cf.NumericalParam(numPath=W.C3_PATHS,
+                                                                                       precision=prec))
+        elif name.startswith("c2"):
+            prec = cf.FP32 if "fp32" in name else cf.FP64
+            cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=W.C2_PATHS, precision=prec), per_path=False)
+        else:
+            raise SystemExit("unknown workload " + name)
+        stats = cf.last_kernel_stats() if name != "copy" else {}
+        print(name, stats.get("main_kernel_ms", ""), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="all")
+    ap.add_argument("--calls", type=int, default=3)
+    a = ap.parse_args()
+    names = ["c3_fp32_aad", "c3_fp64_aad", "c2_fp32", "c2_fp64", "copy"] if a.workload == "all" else [a.workload]
+    for n in names:
+        run(n, a.calls)
+
+
+if __name__ == "__main__":
+    main()

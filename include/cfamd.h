@@ -203,6 +203,11 @@ int cf_last_kernel_stats(cf_kernel_stats* out);
  * calls already return only when their results are on the host). */
 int cf_device_synchronize(void);
 
+/* Diagnostics: copies `bytes` bytes between two device buffers with a plain
+ * streaming kernel, `reps` times.  Used to calibrate hardware byte counters
+ * (a known traffic of 2 x bytes per launch).  No counterpart in the reference. */
+int cf_diag_copy(size_t bytes, int reps);
+
 /* Library version string. */
 const char* cf_version(void);
 
