@@ -113,6 +113,7 @@ def main():
     wall32, k32 = _time_calls(aad(cf.FP32), args.steps, args.warmup)
     ms32 = 1e3 * wall32 / args.steps
     r32 = aad(cf.FP32)()
+    st32 = cf.last_kernel_stats()
     # FP64 AAD form (fewer timed calls)
     s64 = max(3, args.steps // 4)
     wall64, k64 = _time_calls(aad(cf.FP64), s64, 1)
@@ -156,6 +157,9 @@ def main():
         "fp64_aad_paths_steps_per_s": path_steps / (ms64 * 1e-3),
         "fp32_over_fp64_aad": ms64 / ms32,
         "aad_kernel_ms": {"fp32": kavg, "fp64": float(np.mean(k64))},
+        "fp32_aad_call_breakdown_ms": {"host_setup": st32["setup_ms"], "path_kernel": st32["main_kernel_ms"],
+                                       "reductions": st32["reduce_ms"], "device_total": st32["total_ms"],
+                                       "host_total": st32["host_ms"]},
         "price_c2": {"paths": n2, "fp32_wall_ms": 1e3 * wp32 / s64, "fp64_wall_ms": 1e3 * wp64 / s64,
                      "fp32_kernel_ms": float(np.mean(kp32)), "fp64_kernel_ms": float(np.mean(kp64)),
                      "fp64_value": float(p64.means[0]), "fp64_std_err": se64,

@@ -68,7 +68,8 @@ class cf_kernel_stats(ctypes.Structure):
     _fields_ = [
         ("main_kernel_ms", ctypes.c_double), ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
         ("grid_blocks", ctypes.c_int), ("block_threads", ctypes.c_int), ("lds_bytes", ctypes.c_int),
-        ("path_steps", ctypes.c_int64),
+        ("path_steps", ctypes.c_int64), ("setup_ms", ctypes.c_double), ("host_ms", ctypes.c_double),
+        ("lean_layout", ctypes.c_int),
     ]
 
 

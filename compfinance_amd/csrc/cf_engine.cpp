@@ -15,8 +15,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <list>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iomanip>
 #include <map>
@@ -40,7 +43,31 @@ using std::vector;
 constexpr double kHalfDay = 0.00136986301369863;   // mcMdlDupire.h:26
 constexpr double kOneHour = 0.000114469;           // mcPrd.h:26
 constexpr double kEps = 1.0e-08;                   // utility.h:5, store.h:153
-constexpr int kBlockThreads = 1024;
+constexpr int kDefaultBlockThreads = 1024;
+
+// Launch-shape overrides for tuning sweeps: CF_BLOCK_THREADS (64..1024, a
+// multiple of 64) and CF_LAYOUT (full, lean).  Unset in normal use.
+int block_threads()
+{
+    static const int v = [] {
+        const char* e = std::getenv("CF_BLOCK_THREADS");
+        const int b = e ? std::atoi(e) : kDefaultBlockThreads;
+        return (b >= 64 && b <= 1024 && b % 64 == 0) ? b : kDefaultBlockThreads;
+    }();
+    return v;
+}
+
+int forced_layout()   // -1 automatic, 0 FULL, 1 LEAN
+{
+    static const int v = [] {
+        const char* e = std::getenv("CF_LAYOUT");
+        if (!e) return -1;
+        if (std::string(e) == "full") return 0;
+        if (std::string(e) == "lean") return 1;
+        return -1;
+    }();
+    return v;
+}
 
 // ------------------------------------------------------------------ errors
 
@@ -440,7 +467,7 @@ BucketTable build_buckets(const vector<R>& knots)
     const double range = double(knots[m - 1]) - double(x0);
     double dmin = range;
     for (int j = 0; j + 1 < m; ++j) dmin = std::min(dmin, double(knots[j + 1]) - double(knots[j]));
-    if (m >= 2 && range > 0.0 && dmin > 0.0) t.nb = int(std::min(4096.0, std::ceil(2.0 * range / dmin) + 1.0));
+    if (m >= 2 && range > 0.0 && dmin > 0.0) t.nb = int(std::min(16384.0, std::ceil(2.0 * range / dmin) + 1.0));
     const R invh = t.nb > 1 ? R(double(t.nb) / range) : R(0);
     vector<int> cnt(t.nb, 0);
     for (int k = 0; k < m; ++k) {
@@ -479,15 +506,77 @@ struct DevBuf {
     }
 };
 
+// Grow-only pinned host buffer (uploads and downloads without staging copies).
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    void* ensure(size_t bytes)
+    {
+        if (bytes <= cap && p) return p;
+        if (p) hip_check(hipHostFree(p), "hipHostFree");
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes, 4096);
+        hip_check(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc");
+        cap = want;
+        return p;
+    }
+};
+
+// Host staging of a prepared plan's tables, one upload.
+struct Staging {
+    vector<unsigned char> bytes;
+    template <class T>
+    size_t add(const T* data, size_t n)
+    {
+        const size_t off = (bytes.size() + 255) & ~size_t(255);
+        bytes.resize(off + n * sizeof(T));
+        if (n) std::memcpy(bytes.data() + off, data, n * sizeof(T));
+        return off;
+    }
+    template <class T>
+    size_t add(const vector<T>& v) { return add(v.data(), v.size()); }
+};
+
+// A model and product prepared for one precision and one kernel: the plans,
+// the knot search table, the shared-memory layout and the tables resident in
+// device memory.  Prepared once per distinct (model, product, precision,
+// adjoint) and reused by later calls.
+struct Prepared {
+    ProductPlan pd;
+    ModelPlan md;
+    BucketTable bt;
+    cfk::KernelConfig cfg{};
+    size_t smem = 0;
+    int per_cu = 0;
+    int block = 0;
+    DevBuf tables;
+    size_t o_ss = 0, o_knots = 0, o_invdx = 0, o_iv = 0, o_drift = 0, o_std = 0, o_bucket = 0;
+    size_t o_ff = 0, o_disc = 0, o_num = 0, o_aux = 0, o_pf = 0, o_pk = 0;
+    // adjoint of the local-vol pre-interpolation: sqrt(dt_i) and the time
+    // interpolation weights of each step
+    vector<double> sqrtdt;
+    vector<Interp1> tw;
+    ~Prepared()
+    {
+        if (tables.p) (void)hipFree(tables.p);
+    }
+};
+
 struct Engine {
     int device = -1;
     int num_cus = 0;
+    int max_lds = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     uint32_t* d_dirs = nullptr;
-    DevBuf tables, block_out, ivbar_blocks, reduced, ivbar, per_path, per_path_agg, ckpt;
+    DevBuf block_out, ivbar_blocks, reduced, ivbar, per_path, per_path_agg, ckpt, weights;
+    PinnedBuf pin_in, pin_out;
+    std::list<std::pair<string, std::shared_ptr<Prepared>>> plans;   // most recent first
     std::mutex mu;
 };
+
+constexpr size_t kMaxPlans = 64;
 
 std::mutex g_engines_mu;
 std::map<int, std::unique_ptr<Engine>> g_engines;
@@ -503,6 +592,8 @@ Engine& engine()
         e->device = dev;
         hip_check(hipDeviceGetAttribute(&e->num_cus, hipDeviceAttributeMultiprocessorCount, dev),
                   "hipDeviceGetAttribute");
+        hip_check(hipDeviceGetAttribute(&e->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+                  "hipDeviceGetAttribute");
         hip_check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& v : e->ev) hip_check(hipEventCreate(&v), "hipEventCreate");
         // direction numbers [32][kSobolDims], Joe-Kuo recurrence
@@ -511,10 +602,10 @@ Engine& engine()
             const unsigned s = kSobolInit[d][0], a = kSobolInit[d][1];
             uint32_t v[32];
             for (unsigned k = 0; k < 32; ++k) {
-            if (s == 0) {
-                v[k] = 1u << (31 - k);   // van der Corput dimension
-            } else if (k < s) {
-                v[k] = kSobolInit[d][2 + k] << (31 - k);
+                if (s == 0) {
+                    v[k] = 1u << (31 - k);   // van der Corput dimension
+                } else if (k < s) {
+                    v[k] = kSobolInit[d][2 + k] << (31 - k);
                 } else {
                     uint32_t x = v[k - s] ^ (v[k - s] >> s);
                     for (unsigned j = 1; j < s; ++j)
@@ -530,21 +621,6 @@ Engine& engine()
     }
     return *slot;
 }
-
-// Host staging of the per-call tables, one upload.
-struct Staging {
-    vector<unsigned char> bytes;
-    template <class T>
-    size_t add(const T* data, size_t n)
-    {
-        const size_t off = (bytes.size() + 255) & ~size_t(255);
-        bytes.resize(off + n * sizeof(T));
-        if (n) std::memcpy(bytes.data() + off, data, n * sizeof(T));
-        return off;
-    }
-    template <class T>
-    size_t add(const vector<T>& v) { return add(v.data(), v.size()); }
-};
 
 struct Call {
     bool adjoint = false;
@@ -571,202 +647,277 @@ void validate_num(const cf_num* num)
     if (num->aad_mode != CF_AAD_CHECKPOINT && num->aad_mode != CF_AAD_RECOMPUTE) fail(CF_ERR_ARG, "unknown AAD mode");
 }
 
-struct LaunchShape {
-    int block = kBlockThreads;
-    int grid = 0;
-    size_t smem = 0;
-    int slots = 0;
-};
-
-LaunchShape shape_for(Engine& E, cfk::KernelConfig& cfg, const ModelPlan& md, const ProductPlan& pd,
-                      int64_t n_path, int64_t first, int n_buckets)
+// Byte image of everything a prepared plan depends on.
+string plan_key(const cf_model* m, const cf_product* p, int precision, bool adjoint)
 {
-    LaunchShape s;
-    const int waves = s.block / cfk::kWave;
-    int max_lds = 0;
-    hip_check(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, E.device), "attribute");
-    // FULL layout when it fits, LEAN otherwise
-    for (int lean = 0; lean < 2; ++lean) {
-        cfg.lean = lean != 0;
-        const cfk::SmemLayout L =
-            cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves,
-                             cfg.precision == CF_FP64 ? 8 : 4, md.kind == CF_MODEL_DUPIRE,
-                             pd.kind == CF_PRODUCT_EUROPEANS, cfg.adjoint, cfg.lean, n_buckets);
-        s.smem = L.total;
-        if (s.smem <= size_t(max_lds)) break;
+    string k;
+    auto put = [&k](const void* d, size_t n) { k.append(static_cast<const char*>(d), n); };
+    auto putd = [&](double x) { put(&x, sizeof x); };
+    auto puti = [&](int x) { put(&x, sizeof x); };
+    puti(precision);
+    puti(adjoint ? 1 : 0);
+    puti(m->kind);
+    putd(m->spot);
+    if (m->kind == CF_MODEL_BLACK_SCHOLES) {
+        putd(m->vol);
+        putd(m->rate);
+        putd(m->div);
+        puti(m->spot_measure);
+    } else {
+        puti(m->n_spots);
+        puti(m->n_times);
+        putd(m->max_dt);
+        put(m->spots, sizeof(double) * size_t(std::max(m->n_spots, 0)));
+        put(m->times, sizeof(double) * size_t(std::max(m->n_times, 0)));
+        put(m->vols, sizeof(double) * size_t(std::max(m->n_spots, 0)) * size_t(std::max(m->n_times, 0)));
     }
-    if (s.smem > size_t(max_lds))
-        fail(CF_ERR_CAPACITY, "tables need " + std::to_string(s.smem) + " bytes of LDS per block, device allows " +
-                                  std::to_string(max_lds));
-    int per_cu = 0;
-    int rc = cfk::path_kernel_occupancy(cfg, s.block, s.smem, &per_cu);
+    puti(p->kind);
+    if (p->kind == CF_PRODUCT_EUROPEANS) {
+        puti(p->n_options);
+        put(p->maturities, sizeof(double) * size_t(std::max(p->n_options, 0)));
+        put(p->strikes, sizeof(double) * size_t(std::max(p->n_options, 0)));
+    } else {
+        putd(p->strike);
+        putd(p->exercise);
+        putd(p->settlement);
+        putd(p->barrier);
+        putd(p->maturity);
+        putd(p->monitor_freq);
+        putd(p->smooth);
+        puti(p->call_put);
+    }
+    return k;
+}
+
+// Plans, tables and layout of a model and product; FULL layout when it fits,
+// LEAN otherwise.
+std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, bool adjoint)
+{
+    auto P = std::make_shared<Prepared>();
+    P->pd = plan_product(prd);
+    P->md = plan_model(mdl, P->pd);
+    const ProductPlan& pd = P->pd;
+    const ModelPlan& md = P->md;
+    const bool dup = md.kind == CF_MODEL_DUPIRE;
+
+    vector<float> knots_f;
+    if (dup) {
+        if (fp64) {
+            P->bt = build_buckets(md.log_spots);
+        } else {
+            // FP32 state is log(S / S0): knots shifted by log(S0) in FP64 first
+            const double ls0 = std::log(md.spot);
+            knots_f.resize(md.n_spots);
+            for (int j = 0; j < md.n_spots; ++j) knots_f[j] = float(md.log_spots[j] - ls0);
+            P->bt = build_buckets(knots_f);
+        }
+    }
+    P->cfg = cfk::KernelConfig{fp64 ? 0 : 1, md.kind, pd.kind, adjoint, false};
+    P->block = block_threads();
+    const int waves = P->block / cfk::kWave;
+    const int forced = forced_layout();
+    for (int lean = forced == 1 ? 1 : 0; lean < 2; ++lean) {
+        P->cfg.lean = lean != 0;
+        const cfk::SmemLayout L =
+            cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves, fp64 ? 8 : 4, dup,
+                             pd.kind == CF_PRODUCT_EUROPEANS, adjoint, P->cfg.lean, dup ? P->bt.nb : 0);
+        P->smem = L.total;
+        if (P->smem <= size_t(E.max_lds)) break;
+    }
+    if (P->smem > size_t(E.max_lds))
+        fail(CF_ERR_CAPACITY, "tables need " + std::to_string(P->smem) + " bytes of LDS per block, device allows " +
+                                  std::to_string(E.max_lds));
+    const int rc = cfk::path_kernel_occupancy(P->cfg, P->block, P->smem, &P->per_cu);
     if (rc != 0) hip_check(hipError_t(rc), "occupancy");
-    if (per_cu < 1) fail(CF_ERR_CAPACITY, "path kernel does not fit on a compute unit");
-    const int64_t q0 = (first + 1) >> 6, q1 = (first + n_path) >> 6;
-    const int64_t nq = q1 - q0 + 1;
-    const int64_t want = (nq + waves - 1) / waves;
-    s.grid = int(std::min<int64_t>(want, int64_t(per_cu) * E.num_cus));
-    s.slots = s.grid * waves;
-    return s;
+    if (P->per_cu < 1) fail(CF_ERR_CAPACITY, "path kernel does not fit on a compute unit");
+
+    Staging st;
+    P->o_ss = st.add(md.step_sample);
+    if (dup) {
+        if (fp64) {
+            P->o_knots = st.add(md.log_spots);
+            P->o_invdx = st.add(md.inv_dx);
+            P->o_iv = st.add(md.iv);
+        } else {
+            vector<float> ix(md.n_spots), v(md.iv.size());
+            for (int j = 0; j < md.n_spots; ++j) ix[j] = float(md.inv_dx[j]);
+            for (size_t e = 0; e < v.size(); ++e) v[e] = float(md.iv[e]);
+            P->o_knots = st.add(knots_f);
+            P->o_invdx = st.add(ix);
+            P->o_iv = st.add(v);
+        }
+    } else if (fp64) {
+        P->o_drift = st.add(md.drift);
+        P->o_std = st.add(md.stdv);
+    } else {
+        vector<float> a(md.drift.begin(), md.drift.end()), b(md.stdv.begin(), md.stdv.end());
+        P->o_drift = st.add(a);
+        P->o_std = st.add(b);
+    }
+    P->o_bucket = st.add(P->bt.table);
+    P->o_ff = st.add(md.ff);
+    P->o_disc = st.add(md.disc);
+    P->o_num = st.add(md.num);
+    P->o_aux = st.add(md.bs_aux);
+    if (pd.kind == CF_PRODUCT_EUROPEANS) {
+        P->o_pf = st.add(pd.pay_first);
+        P->o_pk = st.add(pd.pay_strike);
+    }
+    P->tables.ensure(st.bytes.size());
+    hip_check(hipMemcpy(P->tables.p, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice), "table upload");
+
+    if (dup && adjoint) {
+        P->sqrtdt.resize(md.n_steps);
+        P->tw.resize(md.n_steps);
+        for (int i = 0; i < md.n_steps; ++i) {
+            P->sqrtdt[i] = std::sqrt(md.timeline[i + 1] - md.timeline[i]);
+            P->tw[i] = interp_weights(md.times.data(), md.n_times, md.timeline[i]);
+        }
+    }
+    return P;
+}
+
+std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, bool adjoint)
+{
+    if (!mdl) fail(CF_ERR_ARG, "null model");
+    if (!prd) fail(CF_ERR_ARG, "null product");
+    const string key = plan_key(mdl, prd, fp64 ? CF_FP64 : CF_FP32, adjoint);
+    for (auto it = E.plans.begin(); it != E.plans.end(); ++it) {
+        if (it->first == key) {
+            E.plans.splice(E.plans.begin(), E.plans, it);
+            return E.plans.front().second;
+        }
+    }
+    auto P = prepare(E, mdl, prd, fp64, adjoint);
+    E.plans.emplace_front(key, P);
+    if (E.plans.size() > kMaxPlans) E.plans.pop_back();
+    return P;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 int run(const cf_model* mdl, const cf_product* prd, Call& c)
 {
+    const auto t_start = std::chrono::steady_clock::now();
     validate_num(c.num);
-    const ProductPlan pd = plan_product(prd);
-    const ModelPlan md = plan_model(mdl, pd);
-    const int npay = pd.n_pay;
-    const int64_t n_path = c.num->num_path, first = c.num->first_path;
-    const bool fp64 = c.num->precision == CF_FP64;
-    const bool dup = md.kind == CF_MODEL_DUPIRE;
     if (c.adjoint && !c.weights) fail(CF_ERR_ARG, "null aggregation weights");
     if (c.adjoint && c.num->aad_mode == CF_AAD_RECOMPUTE)
         fail(CF_ERR_UNSUPPORTED, "the recompute AAD variant is not available; use the checkpoint variant");
+    const int64_t n_path = c.num->num_path, first = c.num->first_path;
+    const bool fp64 = c.num->precision == CF_FP64;
 
     Engine& E = engine();
     std::lock_guard<std::mutex> lk(E.mu);
     hip_check(hipSetDevice(E.device), "hipSetDevice");
-
-    // knot search tables in the kernel's precision
-    BucketTable bt;
-    vector<float> knots_f;
-    if (dup) {
-        if (fp64) {
-            bt = build_buckets(md.log_spots);
-        } else {
-            const double ls0 = std::log(md.spot);
-            knots_f.resize(md.n_spots);
-            for (int j = 0; j < md.n_spots; ++j) knots_f[j] = float(md.log_spots[j] - ls0);
-            bt = build_buckets(knots_f);
-        }
-    }
-    cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, c.adjoint, false};
-    const LaunchShape sh = shape_for(E, cfg, md, pd, n_path, first, dup ? bt.nb : 0);
+    const std::shared_ptr<Prepared> PP = lookup(E, mdl, prd, fp64, c.adjoint);
+    const Prepared& PR = *PP;
+    const ProductPlan& pd = PR.pd;
+    const ModelPlan& md = PR.md;
+    const int npay = pd.n_pay;
+    const bool dup = md.kind == CF_MODEL_DUPIRE;
     const int stride = cfk::out_stride(npay);
     const int ns = int(pd.timeline.size());
 
-    // ---- tables
-    Staging st;
-    const size_t o_ss = st.add(md.step_sample);
-    size_t o_knots = 0, o_invdx = 0, o_iv = 0, o_drift = 0, o_std = 0;
-    if (dup) {
-        if (fp64) {
-            o_knots = st.add(md.log_spots);
-            o_invdx = st.add(md.inv_dx);
-            o_iv = st.add(md.iv);
-        } else {
-            // FP32 state is log(S / S0): knots shifted by log(S0) in FP64 first
-            vector<float> ix(md.n_spots), v(md.iv.size());
-            for (int j = 0; j < md.n_spots; ++j) ix[j] = float(md.inv_dx[j]);
-            for (size_t e = 0; e < v.size(); ++e) v[e] = float(md.iv[e]);
-            o_knots = st.add(knots_f);
-            o_invdx = st.add(ix);
-            o_iv = st.add(v);
-        }
-    } else {
-        if (fp64) {
-            o_drift = st.add(md.drift);
-            o_std = st.add(md.stdv);
-        } else {
-            vector<float> a(md.drift.begin(), md.drift.end()), b(md.stdv.begin(), md.stdv.end());
-            o_drift = st.add(a);
-            o_std = st.add(b);
-        }
-    }
-    const size_t o_bucket = st.add(bt.table);
-    const size_t o_ff = st.add(md.ff), o_disc = st.add(md.disc), o_num = st.add(md.num);
-    const size_t o_aux = st.add(md.bs_aux);
-    size_t o_pf = 0, o_pk = 0;
-    if (pd.kind == CF_PRODUCT_EUROPEANS) {
-        o_pf = st.add(pd.pay_first);
-        o_pk = st.add(pd.pay_strike);
-    }
-    vector<double> w(npay, 0.0);
-    if (c.adjoint) w.assign(c.weights, c.weights + npay);
-    const size_t o_w = st.add(w);
+    // persistent grid: enough waves for the 64-path Sobol blocks, at most the
+    // resident blocks of the device
+    const int waves = PR.block / cfk::kWave;
+    const int64_t q0 = (first + 1) >> 6, q1 = (first + n_path) >> 6;
+    const int64_t want = (q1 - q0 + 1 + waves - 1) / waves;
+    const int grid = int(std::min<int64_t>(want, int64_t(PR.per_cu) * E.num_cus));
+    const int slots = grid * waves;
 
-    unsigned char* dt = static_cast<unsigned char*>(E.tables.ensure(st.bytes.size()));
-    double* d_block = static_cast<double*>(E.block_out.ensure(size_t(sh.grid) * stride * 8));
+    double* d_block = static_cast<double*>(E.block_out.ensure(size_t(grid) * stride * 8));
     double* d_red = static_cast<double*>(E.reduced.ensure(size_t(stride) * 8));
     const size_t ivn = dup ? size_t(md.n_steps) * md.n_spots : 0;
     double* d_ivb = nullptr;
     double* d_iv_red = nullptr;
     void* d_ck = nullptr;
+    double* d_w = static_cast<double*>(E.weights.ensure(size_t(npay) * 8));
     if (c.adjoint) {
         if (dup && ivn) {
-            d_ivb = static_cast<double*>(E.ivbar_blocks.ensure(size_t(sh.grid) * ivn * 8));
+            d_ivb = static_cast<double*>(E.ivbar_blocks.ensure(size_t(grid) * ivn * 8));
             d_iv_red = static_cast<double*>(E.ivbar.ensure(ivn * 8));
         }
-        const size_t ck_bytes = size_t(sh.slots) * std::max(md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4);
+        const size_t ck_bytes = size_t(slots) * std::max(md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4);
         d_ck = E.ckpt.ensure(ck_bytes);
     }
     double* d_pp = c.per_path ? static_cast<double*>(E.per_path.ensure(size_t(n_path) * npay * 8)) : nullptr;
     double* d_ppa = c.per_path_agg ? static_cast<double*>(E.per_path_agg.ensure(size_t(n_path) * 8)) : nullptr;
+    double* h_in = static_cast<double*>(E.pin_in.ensure(size_t(npay) * 8));
+    double* h_out = static_cast<double*>(E.pin_out.ensure((size_t(stride) + ivn) * 8));
 
     hip_check(hipEventRecord(E.ev[0], E.stream), "hipEventRecord");
-    hip_check(hipMemcpyAsync(dt, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice, E.stream), "upload");
+    if (c.adjoint) {
+        std::memcpy(h_in, c.weights, size_t(npay) * 8);
+        hip_check(hipMemcpyAsync(d_w, h_in, size_t(npay) * 8, hipMemcpyHostToDevice, E.stream), "upload");
+    }
 
+    const unsigned char* dt = static_cast<const unsigned char*>(PR.tables.p);
     cfk::LaunchParams P{};
     P.first_path = first;
     P.n_path = n_path;
     P.n_steps = md.n_steps;
     P.dirs = E.d_dirs;
     P.n_spots = md.n_spots;
-    P.knots = dt + o_knots;
-    P.inv_dx = dt + o_invdx;
-    P.iv = dt + o_iv;
-    P.bucket = reinterpret_cast<const uint16_t*>(dt + o_bucket);
-    P.n_buckets = dup ? bt.nb : 0;
-    P.bucket_fix = bt.fix;
-    P.bucket_x0 = bt.x0;
-    P.bucket_invh = bt.invh;
-    P.drift = dt + o_drift;
-    P.stdv = dt + o_std;
+    P.knots = dt + PR.o_knots;
+    P.inv_dx = dt + PR.o_invdx;
+    P.iv = dt + PR.o_iv;
+    P.bucket = reinterpret_cast<const uint16_t*>(dt + PR.o_bucket);
+    P.n_buckets = dup ? PR.bt.nb : 0;
+    P.bucket_fix = PR.bt.fix;
+    P.bucket_x0 = PR.bt.x0;
+    P.bucket_invh = PR.bt.invh;
+    P.drift = dt + PR.o_drift;
+    P.stdv = dt + PR.o_std;
     P.spot = md.spot;
     P.x0 = fp64 ? std::log(md.spot) : 0.0;
     P.spot_measure = md.spot_measure ? 1 : 0;
     P.bs_vol = md.vol;
-    P.bs_aux = reinterpret_cast<const double*>(dt + o_aux);
+    P.bs_aux = reinterpret_cast<const double*>(dt + PR.o_aux);
     P.n_samples = ns;
-    P.step_sample = reinterpret_cast<const int*>(dt + o_ss);
-    P.ff = reinterpret_cast<const double*>(dt + o_ff);
-    P.disc = reinterpret_cast<const double*>(dt + o_disc);
-    P.num = reinterpret_cast<const double*>(dt + o_num);
+    P.step_sample = reinterpret_cast<const int*>(dt + PR.o_ss);
+    P.ff = reinterpret_cast<const double*>(dt + PR.o_ff);
+    P.disc = reinterpret_cast<const double*>(dt + PR.o_disc);
+    P.num = reinterpret_cast<const double*>(dt + PR.o_num);
     P.n_pay = npay;
     P.strike = pd.strike;
     if (pd.kind == CF_PRODUCT_BARRIER) {
         // mcPrd.h:247-250: width from the spot written at t = 0, no derivative
         const double smooth = md.s_t0 * pd.smooth;
         P.two_smooth = 2 * smooth;
+        P.inv_two_smooth = 1.0 / P.two_smooth;
         P.bar_smooth = pd.barrier + smooth;
         P.minus_smooth = pd.barrier - smooth;
     }
     P.put = pd.put ? 1 : 0;
-    P.pay_first = reinterpret_cast<const int*>(dt + o_pf);
-    P.pay_strike = reinterpret_cast<const double*>(dt + o_pk);
+    P.pay_first = reinterpret_cast<const int*>(dt + PR.o_pf);
+    P.pay_strike = reinterpret_cast<const double*>(dt + PR.o_pk);
     P.block_out = d_block;
     P.per_path = d_pp;
-    P.weights = reinterpret_cast<const double*>(dt + o_w);
+    P.weights = d_w;
     P.ckpt = d_ck;
     P.ivbar_blocks = d_ivb;
     P.per_path_agg = d_ppa;
 
     hip_check(hipEventRecord(E.ev[1], E.stream), "hipEventRecord");
-    int rc = cfk::launch_path_kernel(cfg, P, sh.grid, sh.block, sh.smem, E.stream);
+    const double setup_ms = ms_since(t_start);
+    int rc = cfk::launch_path_kernel(PR.cfg, P, grid, PR.block, PR.smem, E.stream);
     if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
     hip_check(hipEventRecord(E.ev[2], E.stream), "hipEventRecord");
-    rc = cfk::launch_reduce_rows(d_block, sh.grid, stride, d_red, E.stream);
+    rc = cfk::launch_reduce_rows(d_block, grid, stride, d_red, E.stream);
     if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
     if (d_ivb) {
-        rc = cfk::launch_reduce_rows(d_ivb, sh.grid, int(ivn), d_iv_red, E.stream);
+        rc = cfk::launch_reduce_rows(d_ivb, grid, int(ivn), d_iv_red, E.stream);
         if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
     }
     hip_check(hipEventRecord(E.ev[3], E.stream), "hipEventRecord");
 
-    vector<double> red(stride);
-    hip_check(hipMemcpyAsync(red.data(), d_red, size_t(stride) * 8, hipMemcpyDeviceToHost, E.stream), "download");
-    vector<double> ivbar(ivn);
-    if (d_ivb) hip_check(hipMemcpyAsync(ivbar.data(), d_iv_red, ivn * 8, hipMemcpyDeviceToHost, E.stream), "download");
+    hip_check(hipMemcpyAsync(h_out, d_red, size_t(stride) * 8, hipMemcpyDeviceToHost, E.stream), "download");
+    if (d_ivb)
+        hip_check(hipMemcpyAsync(h_out + stride, d_iv_red, ivn * 8, hipMemcpyDeviceToHost, E.stream), "download");
     if (d_pp)
         hip_check(hipMemcpyAsync(c.per_path, d_pp, size_t(n_path) * npay * 8, hipMemcpyDeviceToHost, E.stream),
                   "download");
@@ -779,15 +930,9 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     hip_check(hipEventElapsedTime(&t_main, E.ev[1], E.ev[2]), "event");
     hip_check(hipEventElapsedTime(&t_red, E.ev[2], E.ev[3]), "event");
     hip_check(hipEventElapsedTime(&t_all, E.ev[0], E.ev[3]), "event");
-    g_stats.main_kernel_ms = t_main;
-    g_stats.reduce_ms = t_red;
-    g_stats.total_ms = t_all;
-    g_stats.grid_blocks = sh.grid;
-    g_stats.block_threads = sh.block;
-    g_stats.lds_bytes = int(sh.smem);
-    g_stats.path_steps = n_path * int64_t(md.n_steps);
 
     // ---- results
+    const double* red = h_out;
     const double np = double(n_path);
     for (int k = 0; k < npay; ++k) {
         const double mean = red[k] / np;
@@ -799,28 +944,38 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     }
     if (c.adjoint) {
         if (c.agg_mean) c.agg_mean[0] = red[2 * npay] / np;
-        const double* padj = red.data() + 2 * npay + 2;
+        const double* padj = red + 2 * npay + 2;
         if (dup) {
             const int nS = md.n_spots, nT = md.n_times;
-            vector<double> r(1 + size_t(nS) * nT, 0.0);
+            const double* ivbar = h_out + stride;
+            double* r = c.risks;
+            std::fill(r, r + 1 + size_t(nS) * nT, 0.0);
             r[0] = padj[0] / np;
             // adjoint of interpVols[i][j] = sqrt(dt_i) * interp_t(vols[j], t_i)
             for (int i = 0; i < md.n_steps; ++i) {
-                const double sqrtdt = std::sqrt(md.timeline[i + 1] - md.timeline[i]);
-                const Interp1 ip = interp_weights(md.times.data(), nT, md.timeline[i]);
+                const Interp1& ip = PR.tw[i];
                 for (int j = 0; j < nS; ++j) {
-                    const double a = ivbar[size_t(i) * nS + j] * sqrtdt;
+                    const double a = ivbar[size_t(i) * nS + j] * PR.sqrtdt[i];
                     if (a == 0.0) continue;
                     r[1 + size_t(j) * nT + ip.j0] += a * ip.w0;
                     if (ip.w1 != 0.0) r[1 + size_t(j) * nT + ip.j1] += a * ip.w1;
                 }
             }
-            for (size_t k = 1; k < r.size(); ++k) r[k] /= np;
-            std::copy(r.begin(), r.end(), c.risks);
+            for (size_t k = 1; k < 1 + size_t(nS) * nT; ++k) r[k] /= np;
         } else {
             for (int k = 0; k < 4; ++k) c.risks[k] = padj[k] / np;
         }
     }
+    g_stats.main_kernel_ms = t_main;
+    g_stats.reduce_ms = t_red;
+    g_stats.total_ms = t_all;
+    g_stats.grid_blocks = grid;
+    g_stats.block_threads = PR.block;
+    g_stats.lds_bytes = int(PR.smem);
+    g_stats.path_steps = n_path * int64_t(md.n_steps);
+    g_stats.setup_ms = setup_ms;
+    g_stats.host_ms = ms_since(t_start);
+    g_stats.lean_layout = PR.cfg.lean ? 1 : 0;
     return CF_OK;
 }
 
@@ -976,15 +1131,16 @@ int64_t cf_aad_footprint(const cf_model* mdl, const cf_product* prd, const cf_nu
     int64_t bytes = 0;
     const int rc = guarded([&] {
         validate_num(num);
-        const ProductPlan pd = plan_product(prd);
-        const ModelPlan md = plan_model(mdl, pd);
         Engine& E = engine();
+        std::lock_guard<std::mutex> lk(E.mu);
         const bool fp64 = num->precision == CF_FP64;
-        cfk::KernelConfig cfg{fp64 ? 0 : 1, md.kind, pd.kind, true, false};
-        const LaunchShape sh = shape_for(E, cfg, md, pd, num->num_path, num->first_path, 4096);
-        const int64_t ivn = md.kind == CF_MODEL_DUPIRE ? int64_t(md.n_steps) * md.n_spots : 0;
-        bytes = int64_t(sh.slots) * std::max(md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4) +
-                int64_t(sh.grid) * (ivn + cfk::out_stride(pd.n_pay)) * 8;
+        const std::shared_ptr<Prepared> P = lookup(E, mdl, prd, fp64, true);
+        const int waves = P->block / cfk::kWave;
+        const int64_t q0 = (int64_t(num->first_path) + 1) >> 6, q1 = (int64_t(num->first_path) + num->num_path) >> 6;
+        const int64_t grid = std::min<int64_t>((q1 - q0 + 1 + waves - 1) / waves, int64_t(P->per_cu) * E.num_cus);
+        const int64_t ivn = P->md.kind == CF_MODEL_DUPIRE ? int64_t(P->md.n_steps) * P->md.n_spots : 0;
+        bytes = grid * waves * std::max(P->md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4) +
+                grid * (ivn + cfk::out_stride(P->pd.n_pay)) * 8;
         return CF_OK;
     });
     return rc == CF_OK ? bytes : int64_t(rc);

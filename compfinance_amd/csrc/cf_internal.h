@@ -61,7 +61,7 @@ struct LaunchParams {
     // product
     int n_pay;
     double strike;
-    double bar_smooth, minus_smooth, two_smooth;
+    double bar_smooth, minus_smooth, two_smooth, inv_two_smooth;
     int put;
     const int* pay_first;        // Europeans [n_samples + 1]: payoffs of sample s
     const double* pay_strike;    // Europeans [n_pay]

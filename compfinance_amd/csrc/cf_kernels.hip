@@ -24,6 +24,10 @@
 //     (step x spot) grid.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "cf_internal.h"
 
 namespace cfk {
@@ -44,6 +48,48 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// a / b in FP64 from the hardware reciprocal refined by two Newton steps and a
+// residual correction (within an ulp of the IEEE quotient)
+__device__ __forceinline__ double fast_div(double a, double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    r = fma(fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+
+// Natural logarithm of a positive normal double: x = 2^e m with m in
+// [1/sqrt(2), sqrt(2)), ln(m) = 2 atanh(s) with s = (m - 1) / (m + 1),
+// |s| <= 0.1716, summed to s^21 (truncation below 1e-17 relative), and
+// e ln(2) added in two parts.  About two ulps; used where the argument is
+// known to be positive and finite (the inverse-normal tail).
+__device__ __forceinline__ double log_pos(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);   // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);       // x = m 2^e
+    const bool lo = m < 0.70710678118654752440;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double s = fast_div(m - 1.0, m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 21.0;
+    p = fma(p, s2, 1.0 / 19.0);
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    const double ed = double(e);
+    const double ln2_hi = 6.93147180369123816490e-01;   // 0x3FE62E42FEE00000, exact in ed * ln2_hi
+    const double ln2_lo = 1.90821492927058770002e-10;
+    // ed * ln2_hi is exact (32-bit constant times an 11-bit integer)
+    return fma(ed, ln2_hi, fma(2.0 * s * s2, p, fma(ed, ln2_lo, 2.0 * s)));
+}
+
 // Beasley-Springer-Moro inverse normal of u = ONEOVER2POW32 * xi
 // (gaussians.h:47-87, sobol.h:26, :110-116), FP64, same branches and
 // constants as the reference.
@@ -59,7 +105,7 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, double)
                          ((((3.13082909833 * r + -21.06224101826) * r + 23.08336743743) * r + -8.47351093090) * r + 1.0);
         return sup ? -z : z;
     }
-    double r = log(-log(up));
+    double r = log_pos(-log_pos(up));
     r = 0.3374754822726147 +
         r * (0.9761690190917186 +
         r * (0.1607979714918209 +
@@ -86,7 +132,10 @@ __device__ __forceinline__ float inv_normal(uint32_t xi, float)
         const float z = num * __builtin_amdgcn_rcpf(den);
         return sup ? -z : z;
     }
-    float r = __logf(-__logf(up));
+    // ln(-ln(up)) with the hardware log2: up is in [2^-32, 0.08] and -ln(up)
+    // in [2.5, 22.2], normal numbers for which v_log_f32 is accurate
+    const float ln2 = 0.693147180559945f;
+    float r = __builtin_amdgcn_logf(-__builtin_amdgcn_logf(up) * ln2) * ln2;
     r = 0.3374754822726147f +
         r * (0.9761690190917186f +
         r * (0.1607979714918209f +
@@ -133,48 +182,51 @@ struct KnotSearch {
 template <typename R>
 __device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
 {
-    if (!(x >= s.x0)) return 0;
+    // states below the first knot map to bucket 0, whose count is 0 and whose
+    // comparisons all fail; NaN maps there too
     R bf = (x - s.x0) * s.invh;
+    bf = bf > R(0) ? bf : R(0);
     bf = bf < R(s.nb - 1) ? bf : R(s.nb - 1);
     int lo = s.table[int(bf)];
-    for (int k = 0; k < s.fix; ++k) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
         const int c = lo < s.m ? lo : s.m - 1;
-        if (lo < s.m && s.knots[c] <= x) ++lo;
+        lo += (lo < s.m && s.knots[c] <= x) ? 1 : 0;
+    }
+    for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
+        const int c = lo < s.m ? lo : s.m - 1;
+        lo += (lo < s.m && s.knots[c] <= x) ? 1 : 0;
     }
     return lo;
 }
 
+// Branch-free: every load is in range (nodes clamped), the extrapolation
+// cases are selected afterwards.
 template <typename R>
 __device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x)
 {
     const int m = ks.m;
     const int lo = count_le(ks, x);   // number of knots <= x
     int j = lo - 1;
-    j = j < 0 ? 0 : j;
     j = j > m - 2 ? m - 2 : j;
-    j = j < 0 ? 0 : j;                 // m == 1
+    j = j < 0 ? 0 : j;
     const int j2 = j + 1 < m ? j + 1 : j;
     const R y1 = row[j], y2 = row[j2];
-    const bool inner = lo > 0 && lo < m;
-    const R t = (x - ks.knots[j]) * inv_dx[j];
+    const R x1 = ks.knots[j], ih = inv_dx[j];
+    const bool right = lo >= m && m > 1;
+    const bool left = !right && (lo == 0 || m == 1);
+    const R t = (x - x1) * ih;
+    const R dy = y2 - y1;
     Bracket<R> b;
     b.j = j;
-    if (inner) {
-        b.vol = y1 + (y2 - y1) * t;
-        b.slope = (y2 - y1) * inv_dx[j];
-        b.wa = R(1) - t;
-        b.wb = t;
-    } else {
-        const bool right = lo >= m && m > 1;
-        b.vol = right ? y2 : y1;
-        b.slope = R(0);
-        b.wa = right ? R(0) : R(1);
-        b.wb = right ? R(1) : R(0);
-    }
+    b.vol = left ? y1 : (right ? y2 : y1 + dy * t);
+    b.slope = (left || right) ? R(0) : dy * ih;
+    b.wa = left ? R(1) : (right ? R(0) : R(1) - t);
+    b.wb = left ? R(0) : (right ? R(1) : t);
     return b;
 }
 
-__device__ __forceinline__ double recip_mul(double a, double b) { return a / b; }
+__device__ __forceinline__ double recip_mul(double a, double b) { return fast_div(a, b); }
 __device__ __forceinline__ float recip_mul(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // ------------------------------------------------------------------ products
@@ -345,7 +397,7 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
                     if (F > R(P.bar_smooth)) {
                         ps.dead = true;
                     } else if (F > R(P.minus_smooth)) {
-                        const R f = (R(P.bar_smooth) - F) / R(P.two_smooth);
+                        const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
                         ps.alive = ps.alive * f;
                         if (f == R(0)) ps.nzero++; else ps.pnz = ps.pnz * f;
                     }
@@ -429,11 +481,11 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
                 Fb = R(0); nb = R(0); db = R(0);
                 if constexpr (PRD == PRD_BARRIER) {
                     if (!ps.dead && F > R(P.minus_smooth)) {
-                        const R f = (R(P.bar_smooth) - F) / R(P.two_smooth);
+                        const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
                         R others;
-                        if (ps.nzero == 0) others = ps.alive / f;
+                        if (ps.nzero == 0) others = recip_mul(ps.alive, f);
                         else others = (ps.nzero == 1 && f == R(0)) ? ps.pnz : R(0);
-                        Fb += alive_bar * others * (R(-1) / R(P.two_smooth));
+                        Fb -= alive_bar * others * R(P.inv_two_smooth);
                     }
                     if (s == last) {
                         const R k = R(P.strike);
@@ -473,32 +525,43 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
                     }
                 }
                 R xnext = x;     // state at point i + 1
-                for (int i = n - 1; i >= 0; --i) {
-                    const R xi = ck[i * kWave + lane];
-                    const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
-                    // d x_{i+1} / d vol = g - vol, recovered from the two states:
-                    // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
-                    R w;
-                    if (b.vol != R(0)) {
-                        w = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
-                    } else {
-                        const uint32_t u = hi[i] ^ low_word(i);
-                        w = inv_normal(u, R());
-                    }
-                    xnext = xi;
-                    const R vbar = abar * w;
-                    abar = abar + vbar * b.slope;
-                    if (active && vbar != R(0)) {
-                        double* row = ivbar + i * m;
-                        if (b.wa != R(0)) atomicAdd(row + b.j, double(vbar * b.wa));
-                        if (b.wb != R(0)) atomicAdd(row + b.j + 1, double(vbar * b.wb));
-                    }
-                    const int s = step_sample[i];
-                    if (s >= 0) {
-                        const R S = state_spot(xi, s0);
-                        R Fb, nb, db;
-                        sample_adj(s, S, R(1), R(1), Fb, nb, db);
-                        abar = abar + S * Fb;
+                // checkpoint loads run kPrefetch steps ahead of their use
+                constexpr int kPrefetch = sizeof(R) == 8 ? 2 : 4;   // FP64 holds twice the registers
+                R buf[kPrefetch];
+#pragma unroll
+                for (int k = 0; k < kPrefetch; ++k) buf[k] = n - 1 - k >= 0 ? ck[(n - 1 - k) * kWave + lane] : R(0);
+                for (int i0 = n - 1; i0 >= 0; i0 -= kPrefetch) {
+#pragma unroll
+                    for (int k = 0; k < kPrefetch; ++k) {
+                        const int i = i0 - k;
+                        if (i < 0) break;
+                        const R xi = buf[k];
+                        buf[k] = i - kPrefetch >= 0 ? ck[(i - kPrefetch) * kWave + lane] : R(0);
+                        const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
+                        // d x_{i+1} / d vol = g - vol, recovered from the two states:
+                        // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
+                        R w;
+                        if (b.vol != R(0)) {
+                            w = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
+                        } else {
+                            const uint32_t u = hi[i] ^ low_word(i);
+                            w = inv_normal(u, R());
+                        }
+                        xnext = xi;
+                        const R vbar = abar * w;
+                        abar = abar + vbar * b.slope;
+                        if (active) {
+                            double* row = ivbar + i * m;
+                            atomicAdd(row + b.j, double(vbar * b.wa));
+                            if (m > 1) atomicAdd(row + b.j + 1, double(vbar * b.wb));
+                        }
+                        const int s = step_sample[i];
+                        if (s >= 0) {
+                            const R S = state_spot(xi, s0);
+                            R Fb, nb, db;
+                            sample_adj(s, S, R(1), R(1), Fb, nb, db);
+                            abar = abar + S * Fb;
+                        }
                     }
                 }
                 if (active) padj[0] += double(abar) / s0;      // d log(S0) / d S0
@@ -575,14 +638,25 @@ __global__ void __launch_bounds__(1024) path_kernel(const LaunchParams P)
     }
 }
 
-// Column sums of a row-major [rows][cols] matrix, rows summed in order.
-__global__ void reduce_rows_kernel(const double* in, int rows, int cols, double* out)
+// Column sums of a row-major [rows][cols] matrix.  A block takes 64 columns;
+// wave w sums rows w, w + 16, ... in order and the 16 partial sums are added
+// in wave order, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(1024) reduce_rows_kernel(const double* in, int rows, int cols, double* out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ double part[16][kWave];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c = blockIdx.x * kWave + lane;
     double s = 0.0;
-    for (int r = 0; r < rows; ++r) s += in[size_t(r) * cols + c];
-    out[c] = s;
+    if (c < cols)
+        for (int r = w; r < rows; r += 16) s += in[size_t(r) * cols + c];
+    part[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && c < cols) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t += part[k][lane];
+        out[c] = t;
+    }
 }
 
 // Plain streaming copy, 16 bytes per lane per iteration (diagnostics).
@@ -647,8 +721,8 @@ int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid,
 
 int launch_reduce_rows(const double* in, int rows, int cols, double* out, void* stream)
 {
-    const int threads = 256;
-    const int blocks = (cols + threads - 1) / threads;
+    const int threads = 1024;
+    const int blocks = (cols + kWave - 1) / kWave;
     hipLaunchKernelGGL(reduce_rows_kernel, dim3(blocks), dim3(threads), 0, static_cast<hipStream_t>(stream), in, rows,
                        cols, out);
     return int(hipGetLastError());
@@ -658,11 +732,23 @@ int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* 
 {
     const void* f = pick(cfg);
     if (!f) return int(hipErrorInvalidDeviceFunction);
+    // the dynamic LDS limit and the occupancy of a kernel are queried once per size
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t>, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_tuple(f, block, smem);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        *blocks_per_cu = it->second;
+        return 0;
+    }
     if (smem > 65536) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
         if (e != hipSuccess) return int(e);
     }
-    return int(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, smem));
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, block, smem);
+    if (e == hipSuccess) cache[key] = *blocks_per_cu;
+    return int(e);
 }
 
 }  // namespace cfk

@@ -196,6 +196,9 @@ typedef struct cf_kernel_stats {
     int block_threads;
     int lds_bytes;
     int64_t path_steps;      /* paths x simulation steps */
+    double setup_ms;         /* host time from the call to the kernel launch */
+    double host_ms;          /* host time of the whole call */
+    int lean_layout;         /* 1 when the LEAN shared-memory layout was used */
 } cf_kernel_stats;
 int cf_last_kernel_stats(cf_kernel_stats* out);
 
