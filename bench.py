@@ -5,9 +5,12 @@
 Workload (one "step" = one AADriskOne-equivalent call over all paths):
 Dupire local vol on the 30 x 36 grid (1,080 vols), up-and-out call K=120,
 B=150, 5% smoothing, 3y weekly monitoring (156 log-Euler steps), 2^20 Sobol
-paths, price + delta + 1,080 local-vol sensitivities, FP32 (FP64 sums).  The
-same workload in FP64 and the 2^19-path FP64/FP32 pricing run are measured
-beside it.  Inputs are synthetic (the local-vol formula of the
+paths, price + delta + 1,080 local-vol sensitivities, FP32 (FP64 sums).
+Measured beside it: the same workload in FP64 and with the recompute
+adjoint, the 2^19-path FP64/FP32 pricing run, the product book on shared
+paths (30 Europeans, 10 barriers, 5 Asians, aggregated risk, 2^20 paths) and a
+replay of 512 uneven requests at 200/s through the request server (p50/p99
+latency).  Inputs are synthetic (the local-vol formula of the
 configuration); all tables are resident on the device before timing.
 
 Prints one JSON line.  `value` is paths*steps per second of the FP32 AAD
@@ -31,6 +34,7 @@ import compfinance_amd as cf  # noqa: E402
 from compfinance_amd import workloads as W  # noqa: E402
 
 METRIC = "Dupire price+1080 risks wall-ms & paths·steps/s FP32 vs FP64; serve p99 @QPS"
+_BOOK_CLS = {"europeans": cf.Europeans, "barrier": cf.UOC, "asian": cf.Asian}
 
 # Algorithmic work per path-step (FMA = 2 FLOP, FP32 transcendentals weighted
 # 4, FP64 divide 8 and exp/log 20): forward 41/69, adjoint 26/46 FLOP.
@@ -90,6 +94,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-server replay")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine shards paths across GPUs only by running one "
@@ -124,6 +129,28 @@ def main():
     wp64, kp64 = _time_calls(price(cf.FP64), s64, 1)
     p32 = price(cf.FP32)()
     p64 = price(cf.FP64)()
+
+    # recompute adjoint (no per-step checkpoints)
+    def aad_rc(prec):
+        return lambda: cf.mcParallelSimulAAD(prd, mdl, weights=w, num=cf.NumericalParam(
+            numPath=n3, precision=prec, aadMode=cf.AAD_RECOMPUTE))
+
+    wr32, kr32 = _time_calls(aad_rc(cf.FP32), s64, 1)
+    wr64, kr64 = _time_calls(aad_rc(cf.FP64), s64, 1)
+    rr32 = aad_rc(cf.FP32)()
+
+    # book on shared paths: 30 Europeans, 10 barriers, 5 Asians, aggregated risk
+    book = cf.Book([_BOOK_CLS[k](**kw) for k, kw in W.c4_parts()])
+    bw = W.c4_notionals(len(book.payoffLabels()))
+
+    def c4(prec):
+        return lambda: cf.mcParallelSimulAAD(book, mdl, weights=bw, num=cf.NumericalParam(
+            numPath=W.C4_PATHS, precision=prec))
+
+    wb32, kb32 = _time_calls(c4(cf.FP32), s64, 1)
+    wb64, kb64 = _time_calls(c4(cf.FP64), max(2, s64 // 2), 1)
+    b32, b64 = c4(cf.FP32)(), c4(cf.FP64)()
+    nb_steps = int(len(cf.simulation_timeline(mdl, book)[0]) - 1)
 
     path_steps = n3 * n_steps
     kavg = float(np.mean(k32))
@@ -164,6 +191,16 @@ def main():
                      "fp32_kernel_ms": float(np.mean(kp32)), "fp64_kernel_ms": float(np.mean(kp64)),
                      "fp64_value": float(p64.means[0]), "fp64_std_err": se64,
                      "fp32_minus_fp64_in_se": float((p32.means[0] - p64.means[0]) / se64)},
+        "aad_recompute": {"fp32_wall_ms": 1e3 * wr32 / s64, "fp64_wall_ms": 1e3 * wr64 / s64,
+                          "fp32_kernel_ms": float(np.mean(kr32)), "fp64_kernel_ms": float(np.mean(kr64)),
+                          "fp32_max_risk_diff_vs_checkpoint": float(np.abs(rr32.risks - r32.risks).max()),
+                          "fp32_risk_scale": float(np.abs(r32.risks).max())},
+        "book_c4": {"paths": W.C4_PATHS, "time_steps": nb_steps, "payoffs": int(len(bw)),
+                    "risks": int(b32.risks.size),
+                    "fp32_wall_ms": 1e3 * wb32 / s64, "fp64_wall_ms": 1e3 * wb64 / max(2, s64 // 2),
+                    "fp32_kernel_ms": float(np.mean(kb32)), "fp64_kernel_ms": float(np.mean(kb64)),
+                    "fp32_paths_steps_per_s": W.C4_PATHS * nb_steps / (wb32 / s64),
+                    "aggregate_fp32": float(b32.agg_mean), "aggregate_fp64": float(b64.agg_mean)},
         "aad_values": {"fp32": float(r32.agg_mean), "fp64": float(r64.agg_mean),
                        "delta_fp32": float(r32.risks[0]), "delta_fp64": float(r64.risks[0])},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[cf.FP32], "unit": "TFLOP/s",
@@ -171,6 +208,9 @@ def main():
                      "kernel": "path_kernel<float, Dupire, barrier, adjoint>", "kernel_ms": kavg,
                      "flop_per_path_step": FLOP_PER_PATH_STEP[("aad", cf.FP32)], "path_steps": path_steps},
     }
+    if not args.no_serving:
+        from compfinance_amd import serving
+        out["serving_c5"] = serving.run_c5()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = _cpu_baseline(n_steps)
     print(json.dumps(out))

@@ -21,11 +21,11 @@ import numpy as np
 
 __all__ = [
     "FP64", "FP32", "AAD_CHECKPOINT", "AAD_RECOMPUTE",
-    "BlackScholes", "Dupire", "European", "UOC", "Europeans", "NumericalParam",
+    "BlackScholes", "Dupire", "European", "UOC", "Europeans", "Asian", "Book", "NumericalParam",
     "mcParallelSimul", "mcParallelSimulAAD", "SimulResults", "AADSimulResults",
-    "putBlackScholes", "putDupire", "putEuropean", "putBarrier", "putEuropeans",
+    "putBlackScholes", "putDupire", "putEuropean", "putBarrier", "putEuropeans", "putAsian", "putBook",
     "value", "AADriskOne", "AADriskAggregate", "dupireAADRisk", "clearStore",
-    "CfError", "lib", "last_kernel_stats", "LIB_PATH",
+    "CfError", "lib", "last_kernel_stats", "LIB_PATH", "Server", "dupireCalib", "dupireSuperbucket",
 ]
 
 FP64, FP32 = 0, 1
@@ -48,12 +48,17 @@ class cf_model(ctypes.Structure):
 
 
 class cf_product(ctypes.Structure):
-    _fields_ = [
-        ("kind", ctypes.c_int), ("strike", ctypes.c_double), ("exercise", ctypes.c_double),
-        ("settlement", ctypes.c_double), ("barrier", ctypes.c_double), ("maturity", ctypes.c_double),
-        ("monitor_freq", ctypes.c_double), ("smooth", ctypes.c_double), ("call_put", ctypes.c_int),
-        ("n_options", ctypes.c_int), ("maturities", _c_double_p), ("strikes", _c_double_p),
-    ]
+    pass
+
+
+cf_product._fields_ = [
+    ("kind", ctypes.c_int), ("strike", ctypes.c_double), ("exercise", ctypes.c_double),
+    ("settlement", ctypes.c_double), ("barrier", ctypes.c_double), ("maturity", ctypes.c_double),
+    ("monitor_freq", ctypes.c_double), ("smooth", ctypes.c_double), ("call_put", ctypes.c_int),
+    ("n_options", ctypes.c_int), ("maturities", _c_double_p), ("strikes", _c_double_p),
+    ("fixing_freq", ctypes.c_double), ("snap_freq", ctypes.c_double),
+    ("n_parts", ctypes.c_int), ("parts", ctypes.POINTER(cf_product)),
+]
 
 
 class cf_num(ctypes.Structure):
@@ -274,6 +279,54 @@ class Europeans:
         return _product_timeline(self)
 
 
+class Asian:
+    """Arithmetic-average option (no reference class; Product<T> interface,
+    mcBase.h:122-152): max(mean of the spot on the fixing dates - strike, 0),
+    paid at maturity.  Fixings at fixingFreq, 2 fixingFreq, ... up to the
+    maturity, each moved to the nearest date of the grid 0, snapFreq, ...,
+    maturity when snapFreq > 0 (monthly fixings on a weekly simulation grid)."""
+
+    def __init__(self, strike, maturity, fixingFreq, snapFreq=0.0, callPut=False):
+        self.strike, self.maturity = float(strike), float(maturity)
+        self.fixingFreq, self.snapFreq, self.callPut = float(fixingFreq), float(snapFreq), bool(callPut)
+
+    def descriptor(self):
+        d = cf_product()
+        d.kind, d.strike, d.maturity, d.call_put = 3, self.strike, self.maturity, int(self.callPut)
+        d.fixing_freq, d.snap_freq = self.fixingFreq, self.snapFreq
+        return d
+
+    def payoffLabels(self):
+        return _payoff_labels(self)
+
+    def timeline(self):
+        return _product_timeline(self)
+
+
+class Book:
+    """Several products on the same paths (no reference class): payoffs
+    concatenated in part order, labelled "<part index>:<part label>".  Parts:
+    European (settled at exercise), UOC, Europeans, Asian."""
+
+    def __init__(self, parts):
+        self.parts = list(parts)
+        if not self.parts:
+            raise ValueError("empty book")
+
+    def descriptor(self):
+        d = cf_product()
+        arr = (cf_product * len(self.parts))(*[p.descriptor() for p in self.parts])
+        self._keep = arr
+        d.kind, d.n_parts, d.parts = 4, len(self.parts), ctypes.cast(arr, ctypes.POINTER(cf_product))
+        return d
+
+    def payoffLabels(self):
+        return _payoff_labels(self)
+
+    def timeline(self):
+        return _product_timeline(self)
+
+
 class NumericalParam:
     """NumericalParam (main.h:33-40) plus precision, AAD variant and first path."""
 
@@ -463,6 +516,23 @@ def putEuropeans(maturities, strikes, store):
     _product_store[store] = p
 
 
+def putAsian(strike, maturity, fixingFreq, snapFreq, callPut, store):
+    """Arithmetic-average option (no reference counterpart)."""
+    L = lib()
+    L.cf_put_asian.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_char_p]
+    _check(L.cf_put_asian(float(strike), float(maturity), float(fixingFreq), float(snapFreq), int(bool(callPut)),
+                          _id(store)))
+    _product_store[store] = Asian(strike, maturity, fixingFreq, snapFreq, callPut)
+
+
+def putBook(parts, store):
+    """Book of stored products on shared paths (no reference counterpart)."""
+    L = lib()
+    L.cf_put_book.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_char_p]
+    _check(L.cf_put_book(_label_array(parts), len(parts), _id(store)))
+    _product_store[store] = Book([_product_store[p] for p in parts])
+
+
 def clearStore():
     lib().cf_store_clear()
     _model_store.clear()
@@ -542,3 +612,174 @@ def dupireAADRisk(modelId, productId, notionals, num):
                                           ctypes.byref(n), _dptr(val), _dptr(delta), _dptr(vega), len(vega)))
     shape = (len(m.spots), len(m.times)) if isinstance(m, Dupire) else (got,)
     return OrderedDict(value=float(val[0]), delta=float(delta[0]), vega=vega[:got].reshape(shape))
+
+
+# ---------------------------------------------------------------- request server
+# Continuous batching of pricing and risk requests on one GPU (cf_server_*
+# in cfamd.h); no counterpart in the reference.
+
+
+class cf_server_opts(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_int64), ("auto_recompute", ctypes.c_int)]
+
+
+class cf_request(ctypes.Structure):
+    _fields_ = [("model", cf_model), ("product", cf_product), ("num", cf_num), ("aad", ctypes.c_int),
+                ("weights", _c_double_p)]
+
+
+class cf_result(ctypes.Structure):
+    _fields_ = [
+        ("means", _c_double_p), ("std_errs", _c_double_p), ("cap_pay", ctypes.c_int),
+        ("risks", _c_double_p), ("cap_risk", ctypes.c_int),
+        ("status", ctypes.c_int), ("error", ctypes.c_char * 256), ("n_pay", ctypes.c_int), ("n_param", ctypes.c_int),
+        ("agg_mean", ctypes.c_double), ("submit_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("device_bytes", ctypes.c_int64),
+        ("recomputed", ctypes.c_int),
+    ]
+
+
+class cf_server_stats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in (
+        "submitted", "completed", "failed", "queued", "running", "peak_running", "admitted_busy", "memory_waits",
+        "recompute_switches", "arena_bytes", "peak_arena_bytes")]
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_server_sigs", False):
+        P = ctypes.POINTER
+        for name, res, args in [
+            ("cf_server_create", ctypes.c_int, [P(cf_server_opts), P(ctypes.c_void_p)]),
+            ("cf_server_submit", ctypes.c_int, [ctypes.c_void_p, P(cf_request), P(ctypes.c_int64)]),
+            ("cf_server_poll", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+            ("cf_server_wait", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, P(cf_result)]),
+            ("cf_server_get_stats", ctypes.c_int, [ctypes.c_void_p, P(cf_server_stats)]),
+            ("cf_server_destroy", ctypes.c_int, [ctypes.c_void_p]),
+        ]:
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        L._server_sigs = True
+    return L
+
+
+class Server:
+    """Request server on the current GPU: `slots` requests on the device at
+    once, device blocks from an arena of `hbmBudget` bytes; AAD requests that
+    do not fit switch to the recompute variant when `autoRecompute`."""
+
+    def __init__(self, slots=4, hbmBudget=16 << 30, autoRecompute=True):
+        L = _server_lib()
+        o = cf_server_opts(int(slots), int(hbmBudget), int(bool(autoRecompute)))
+        h = ctypes.c_void_p()
+        _check(L.cf_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._sizes = {}
+
+    def submit(self, prd, mdl, num=None, aad=False, weights=None):
+        """Queues one request; returns its id.  aad: prices, aggregate of
+        sum_k weights[k] payoff_k (default: first payoff) and risks."""
+        r = cf_request()
+        r.model, r.product = mdl.descriptor(), prd.descriptor()
+        r.num = (num or NumericalParam()).descriptor()
+        r.aad = int(bool(aad))
+        npay = _check(lib().cf_num_payoffs(ctypes.byref(r.product)))
+        npar = _check(lib().cf_num_params(ctypes.byref(r.model))) if aad else 0
+        w = None
+        if aad and weights is not None:
+            w = _f64(weights)
+            if w.shape != (npay,):
+                raise ValueError("weights must have one entry per payoff")
+            r.weights = _dptr(w)
+        rid = ctypes.c_int64()
+        _check(_server_lib().cf_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        self._sizes[rid.value] = (npay, npar)
+        return rid.value
+
+    def poll(self, rid):
+        return bool(_check(_server_lib().cf_server_poll(self._h, int(rid))))
+
+    def wait(self, rid):
+        npay, npar = self._sizes.pop(rid)
+        means, ses, risks = np.zeros(npay), np.zeros(npay), np.zeros(max(npar, 1))
+        out = cf_result()
+        out.means, out.std_errs, out.cap_pay = _dptr(means), _dptr(ses), npay
+        out.risks, out.cap_risk = _dptr(risks), npar
+        rc = _server_lib().cf_server_wait(self._h, int(rid), ctypes.byref(out))
+        if rc < 0:
+            raise CfError(rc, out.error.decode() or lib().cf_last_error().decode())
+        return OrderedDict(means=means, std_errs=ses, agg_mean=out.agg_mean, risks=risks[:npar],
+                           submit_ms=out.submit_ms, queue_ms=out.queue_ms, latency_ms=out.latency_ms,
+                           kernel_ms=out.kernel_ms, device_bytes=out.device_bytes, recomputed=bool(out.recomputed))
+
+    def stats(self):
+        s = cf_server_stats()
+        _check(_server_lib().cf_server_get_stats(self._h, ctypes.byref(s)))
+        return OrderedDict((f, getattr(s, f)) for f, _ in cf_server_stats._fields_)
+
+    def close(self):
+        if self._h:
+            _check(_server_lib().cf_server_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+# ---------------------------------------------------------------- local-vol calibration (FP64, host)
+
+
+class cf_merton_ivs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("spot", "vol", "jmp_intens", "jmp_average", "jmp_std")]
+
+
+def _calib_lib():
+    L = lib()
+    if not getattr(L, "_calib_sigs", False):
+        P, d, i = ctypes.POINTER, _c_double_p, ctypes.c_int
+        L.cf_dupire_calib.argtypes = [P(cf_merton_ivs), d, i, ctypes.c_double, d, i, ctypes.c_double, d, d, d,
+                                      P(i), P(i), i]
+        L.cf_superbucket_vega.argtypes = [P(cf_merton_ivs), d, i, ctypes.c_double, d, i, ctypes.c_double, d, i, d,
+                                          i, d, i, d]
+        L.cf_dupire_superbucket.argtypes = [P(cf_merton_ivs), ctypes.c_double, ctypes.c_char_p,
+                                            P(ctypes.c_char_p), d, i, d, i, ctypes.c_double, d, i, ctypes.c_double,
+                                            d, i, d, i, P(cf_num), d, d, d]
+        L._calib_sigs = True
+    return L
+
+
+def dupireCalib(inclSpots, maxDs, inclTimes, maxDt, spot, vol, jmpIntens=0.0, jmpAverage=0.0, jmpStd=0.0):
+    """dupireCalib (main.h:414-438): local vols on a Merton implied surface;
+    returns {spots, times, lVols [spots x times]}."""
+    L = _calib_lib()
+    ivs = cf_merton_ivs(spot, vol, jmpIntens, jmpAverage, jmpStd)
+    s, t = _f64(inclSpots), _f64(inclTimes)
+    ns, nt = ctypes.c_int(), ctypes.c_int()
+    args = (ctypes.byref(ivs), _dptr(s), len(s), float(maxDs), _dptr(t), len(t), float(maxDt))
+    _check(L.cf_dupire_calib(*args, None, None, None, ctypes.byref(ns), ctypes.byref(nt), 0))
+    S, T, V = np.zeros(ns.value), np.zeros(nt.value), np.zeros(ns.value * nt.value)
+    _check(L.cf_dupire_calib(*args, _dptr(S), _dptr(T), _dptr(V), ctypes.byref(ns), ctypes.byref(nt), len(V)))
+    return OrderedDict(spots=S, times=T, lVols=V.reshape(len(S), len(T)))
+
+
+def dupireSuperbucket(spot, maxDt, productId, notionals, inclSpots, maxDs, inclTimes, maxDtVol, strikes, mats, vol,
+                      jmpIntens, jmpAverage, jmpStd, num):
+    """dupireSuperbucket (main.h:453-569): value, delta and the risks to the
+    implied-vol spreads on strikes x mats, through the local-vol calibration
+    and the GPU's local-vol risks."""
+    L = _calib_lib()
+    ivs = cf_merton_ivs(spot, vol, jmpIntens, jmpAverage, jmpStd)
+    s, t, k, m = _f64(inclSpots), _f64(inclTimes), _f64(strikes), _f64(mats)
+    labs = list(notionals.keys())
+    nots = _f64([notionals[x] for x in labs]) if labs else np.zeros(1)
+    val, delta, vega = np.zeros(1), np.zeros(1), np.zeros(len(k) * len(m))
+    n = num.descriptor()
+    _check(L.cf_dupire_superbucket(ctypes.byref(ivs), float(maxDt), _id(productId), _label_array(labs), _dptr(nots),
+                                   len(labs), _dptr(s), len(s), float(maxDs), _dptr(t), len(t), float(maxDtVol),
+                                   _dptr(k), len(k), _dptr(m), len(m), ctypes.byref(n), _dptr(val), _dptr(delta),
+                                   _dptr(vega)))
+    return OrderedDict(value=float(val[0]), delta=float(delta[0]), strikes=k, mats=m,
+                       vega=vega.reshape(len(k), len(m)))

@@ -14,7 +14,11 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfamd.so")
 
-SOURCES = ["cf_kernels.hip", "cf_engine.cpp", "cf_store.cpp"]
+# (source, object, defines): the path kernel families compile in parallel
+SOURCES = ([("cf_kern_inst.hip", "cf_kern_r%d_a%d" % (r, a), ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a])
+            for r in (0, 1) for a in (0, 1, 2)] +
+           [(s, os.path.splitext(s)[0], []) for s in ("cf_kernels.hip", "cf_engine.cpp", "cf_store.cpp",
+                                                      "cf_server.cpp", "cf_calib.cpp")])
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = [
@@ -36,28 +40,41 @@ def hipcc():
 
 
 def build(verbose=False):
-    objs = []
+    """Compiles every source (concurrently) and links the library."""
     tmpdir = os.path.join(HERE, "_build")
     os.makedirs(tmpdir, exist_ok=True)
-    for src in SOURCES:
+    jobs = []
+    for src, name, defs in SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
-        obj = os.path.join(tmpdir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc()] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-c", path, "-o", obj]
-        _run(cmd, verbose)
-        objs.append(obj)
+        obj = os.path.join(tmpdir, name + ".o")
+        cmd = [hipcc()] + HIP_FLAGS + defs + ["-I", os.path.join(ROOT, "include"), "-c", path, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        jobs.append((cmd, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    failed = []
+    for cmd, obj, proc in jobs:
+        out = _filter(proc.communicate()[0])
+        if proc.returncode != 0:
+            failed.append("build failed: " + " ".join(cmd) + "\n" + out)
+        elif verbose and out.strip():
+            print(out)
+    if failed:
+        raise RuntimeError("\n".join(failed))
     tmp_lib = LIB + ".tmp"
-    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp_lib] + objs, verbose)
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp_lib] + [j[1] for j in jobs], verbose)
     os.replace(tmp_lib, LIB)
     return LIB
+
+
+def _filter(text):
+    return "\n".join(l for l in text.splitlines() if "packed-fp32-ops" not in l)
 
 
 def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
-    out = "\n".join(l for l in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in l)
+    out = _filter(res.stdout + res.stderr)
     if res.returncode != 0:
         raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + out)
     if verbose and out.strip():

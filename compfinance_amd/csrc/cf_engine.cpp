@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <iomanip>
 #include <map>
 #include <memory>
@@ -28,21 +29,16 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/cfamd.h"
+#include "cf_host.h"
 #include "cf_internal.h"
 #include "sobol_init.inc"
 
-namespace {
+namespace cfh {
 
-using std::size_t;
-using std::string;
-using std::vector;
-
-constexpr double kHalfDay = 0.00136986301369863;   // mcMdlDupire.h:26
-constexpr double kOneHour = 0.000114469;           // mcPrd.h:26
-constexpr double kEps = 1.0e-08;                   // utility.h:5, store.h:153
 constexpr int kDefaultBlockThreads = 1024;
 
 // Launch-shape overrides for tuning sweeps: CF_BLOCK_THREADS (64..1024, a
@@ -74,29 +70,9 @@ int forced_layout()   // -1 automatic, 0 FULL, 1 LEAN
 thread_local string g_err;
 thread_local cf_kernel_stats g_stats{};
 
-struct CfError : std::runtime_error {
-    int code;
-    CfError(int c, const string& m) : std::runtime_error(m), code(c) {}
-};
 
-[[noreturn]] void fail(int code, const string& msg) { throw CfError(code, msg); }
+void fail(int code, const string& msg) { throw CfError(code, msg); }
 
-template <class F>
-int guarded(F&& f)
-{
-    try {
-        return f();
-    } catch (const CfError& e) {
-        g_err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_err = "out of host memory";
-        return CF_ERR_CAPACITY;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return CF_ERR_ARG;
-    }
-}
 
 void hip_check(hipError_t e, const char* what)
 {
@@ -141,12 +117,6 @@ vector<double> fill_data(const vector<double>& original, double maxDx, double mi
     return out;
 }
 
-// linear interpolation with flat extrapolation (interp.h:26-63) and its
-// two-node weights
-struct Interp1 {
-    int j0, j1;
-    double w0, w1;
-};
 
 Interp1 interp_weights(const double* xs, int n, double x)
 {
@@ -171,29 +141,53 @@ double interp_value(const double* xs, int n, const double* ys, double x)
 
 // ------------------------------------------------------------------ plans
 
-struct SampleDefH {
-    bool numeraire = true;
-    bool has_fwd = false, has_disc = false;
-    double fwd_mat = 0.0, disc_mat = 0.0;
-};
 
-struct ProductPlan {
-    int kind = 0;
-    vector<double> timeline;
-    vector<SampleDefH> defs;
-    double strike = 0.0, barrier = 0.0, smooth = 0.0, freq = 0.0, maturity = 0.0;
-    double exercise = 0.0, settlement = 0.0;
-    bool put = false;
-    vector<double> mats;              // Europeans: distinct maturities, increasing
-    vector<vector<double>> strikes;   // Europeans: strikes per maturity, input order
-    vector<int> pay_first;
-    vector<double> pay_strike;
-    int n_pay = 0;
-};
 
 void check_finite(double x, const char* what)
 {
     if (!std::isfinite(x)) fail(CF_ERR_ARG, string(what) + " must be finite");
+}
+
+// Dates freq, 2 freq, ... accumulated as the barrier's monitoring dates
+// (mcPrd.h:162-193) up to the maturity, which closes the list; today first
+// when asked.
+vector<double> periodic_dates(double maturity, double freq, bool with_today)
+{
+    vector<double> out;
+    if (with_today) out.push_back(0.0);
+    double t = 0.0 + freq;
+    while (maturity - t > kOneHour) {
+        out.push_back(t);
+        t += freq;
+    }
+    out.push_back(maturity);
+    return out;
+}
+
+// Asian fixing dates, moved to the nearest date of the snapping grid when
+// snap > 0 (ties to the earlier date).  From the last fixing back, a fixing
+// whose nearest date is not before the next fixing's takes the grid date just
+// before it (a last monthly fixing a few days before maturity), so fixings
+// stay distinct.
+vector<double> asian_fixings(double maturity, double freq, double snap)
+{
+    vector<double> fix = periodic_dates(maturity, freq, false);
+    if (snap > 0.0) {
+        const vector<double> grid = periodic_dates(maturity, snap, true);
+        for (size_t k = fix.size(); k-- > 0;) {
+            double& t = fix[k];
+            auto it = std::lower_bound(grid.begin(), grid.end(), t);
+            if (it == grid.end()) t = grid.back();
+            else if (it != grid.begin() && t - *(it - 1) <= *it - t) t = *(it - 1);
+            else t = *it;
+            if (k + 1 < fix.size() && !(t < fix[k + 1])) {
+                auto nx = std::lower_bound(grid.begin(), grid.end(), fix[k + 1]);
+                t = nx == grid.begin() ? 0.0 : *(nx - 1);
+            }
+            if (t <= 0.0) fail(CF_ERR_ARG, "Asian fixings closer than the snapping grid");
+        }
+    }
+    return fix;
 }
 
 ProductPlan plan_product(const cf_product* p)
@@ -233,13 +227,7 @@ ProductPlan plan_product(const cf_product* p)
         d.put = p->call_put != 0;
         d.freq = p->monitor_freq;
         d.maturity = p->maturity;
-        d.timeline.push_back(0.0);
-        double t = 0.0 + p->monitor_freq;
-        while (p->maturity - t > kOneHour) {
-            d.timeline.push_back(t);
-            t += p->monitor_freq;
-        }
-        d.timeline.push_back(p->maturity);
+        d.timeline = periodic_dates(p->maturity, p->monitor_freq, true);
         for (size_t i = 0; i < d.timeline.size(); ++i) {
             SampleDefH s;
             s.numeraire = false;
@@ -275,6 +263,66 @@ ProductPlan plan_product(const cf_product* p)
             d.pay_first.push_back(int(d.pay_strike.size()));
         }
         d.n_pay = int(d.pay_strike.size());
+    } else if (p->kind == CF_PRODUCT_ASIAN) {
+        check_finite(p->strike, "strike");
+        check_finite(p->maturity, "maturity");
+        check_finite(p->fixing_freq, "fixing frequency");
+        check_finite(p->snap_freq, "snapping frequency");
+        if (!(p->maturity > 0.0)) fail(CF_ERR_ARG, "maturity must be positive");
+        if (!(p->fixing_freq > 0.0)) fail(CF_ERR_ARG, "fixing frequency must be positive");
+        if (p->snap_freq < 0.0) fail(CF_ERR_ARG, "snapping frequency must be non-negative");
+        if (p->maturity / p->fixing_freq > 20000.0) fail(CF_ERR_CAPACITY, "too many fixing dates");
+        if (p->snap_freq > 0.0 && p->maturity / p->snap_freq > 20000.0) fail(CF_ERR_CAPACITY, "snapping grid too fine");
+        d.strike = p->strike;
+        d.maturity = p->maturity;
+        d.put = p->call_put != 0;
+        d.fix_freq = p->fixing_freq;
+        d.snap = p->snap_freq;
+        d.timeline = asian_fixings(p->maturity, p->fixing_freq, p->snap_freq);
+        for (size_t i = 0; i < d.timeline.size(); ++i) {
+            SampleDefH s;
+            s.numeraire = i + 1 == d.timeline.size();
+            s.has_fwd = true;
+            s.fwd_mat = d.timeline[i];
+            d.defs.push_back(s);
+        }
+        d.n_pay = 1;
+    } else if (p->kind == CF_PRODUCT_BOOK) {
+        if (p->n_parts < 1 || !p->parts) fail(CF_ERR_ARG, "empty book");
+        if (p->n_parts > 4096) fail(CF_ERR_CAPACITY, "more than 4096 products in a book");
+        vector<double> all;
+        for (int i = 0; i < p->n_parts; ++i) {
+            const cf_product* q = p->parts + i;
+            if (q->kind == CF_PRODUCT_BOOK) fail(CF_ERR_UNSUPPORTED, "a book cannot hold a book");
+            ProductPlan part = plan_product(q);
+            if (part.kind == CF_PRODUCT_EUROPEAN && part.settlement != part.exercise)
+                fail(CF_ERR_UNSUPPORTED, "a book holds Europeans settled on their exercise date only");
+            all.insert(all.end(), part.timeline.begin(), part.timeline.end());
+            d.n_pay += part.n_pay;
+            d.parts.push_back(std::move(part));
+        }
+        int n_bar = 0;
+        vector<vector<double>> sched;
+        for (const auto& q : d.parts) {
+            if (q.kind == CF_PRODUCT_BARRIER) ++n_bar;
+            if (q.kind == CF_PRODUCT_ASIAN && std::find(sched.begin(), sched.end(), q.timeline) == sched.end())
+                sched.push_back(q.timeline);
+        }
+        if (n_bar > cfk::kMaxBookBarriers)
+            fail(CF_ERR_CAPACITY, "a book holds at most " + std::to_string(cfk::kMaxBookBarriers) + " barriers");
+        if (int(sched.size()) > cfk::kMaxBookGroups)
+            fail(CF_ERR_CAPACITY,
+                 "a book holds at most " + std::to_string(cfk::kMaxBookGroups) + " Asian fixing schedules");
+        std::sort(all.begin(), all.end());
+        all.erase(std::unique(all.begin(), all.end()), all.end());
+        d.timeline = all;
+        for (double t : all) {
+            SampleDefH s;
+            s.numeraire = true;
+            s.has_fwd = true;
+            s.fwd_mat = t;
+            d.defs.push_back(s);
+        }
     } else {
         fail(CF_ERR_UNSUPPORTED, "product kind not supported by the GPU engine");
     }
@@ -284,6 +332,15 @@ ProductPlan plan_product(const cf_product* p)
 vector<string> payoff_labels(const ProductPlan& d)
 {
     vector<string> out;
+    if (d.kind == CF_PRODUCT_ASIAN) {
+        out.push_back(string(d.put ? "asian put " : "asian call ") + fmt2(d.maturity) + " " + fmt2(d.strike));
+        return out;
+    }
+    if (d.kind == CF_PRODUCT_BOOK) {
+        for (size_t i = 0; i < d.parts.size(); ++i)
+            for (const string& l : payoff_labels(d.parts[i])) out.push_back(std::to_string(i) + ":" + l);
+        return out;
+    }
     if (d.kind == CF_PRODUCT_EUROPEAN) {
         if (d.settlement == d.exercise) out.push_back("call " + fmt2(d.strike) + " " + fmt2(d.exercise));
         else out.push_back("call " + fmt2(d.strike) + " " + fmt2(d.exercise) + " " + fmt2(d.settlement));
@@ -299,23 +356,126 @@ vector<string> payoff_labels(const ProductPlan& d)
     return out;
 }
 
-struct ModelPlan {
-    int kind = 0;
-    double spot = 0.0;
-    vector<double> timeline;
-    vector<int> step_sample;   // [n_steps + 1]
-    int n_steps = 0;
-    // Dupire
-    int n_spots = 0, n_times = 0;
-    vector<double> spots, times, log_spots, inv_dx, iv;   // iv [n_steps][n_spots]
-    // Black-Scholes
-    double vol = 0.0, rate = 0.0, div = 0.0;
-    bool spot_measure = false;
-    vector<double> drift, stdv;
-    vector<double> ff, disc, num;   // per product sample
-    vector<double> bs_aux;
-    double s_t0 = 0.0;              // first sample's forward (barrier smoothing width)
-};
+// Kernel product of a plan.
+int kernel_product(const ProductPlan& d)
+{
+    switch (d.kind) {
+    case CF_PRODUCT_EUROPEAN: return cfk::PRD_EUROPEAN;
+    case CF_PRODUCT_BARRIER: return cfk::PRD_BARRIER;
+    case CF_PRODUCT_EUROPEANS: return cfk::PRD_EUROPEANS;
+    default: return cfk::PRD_BOOK;
+    }
+}
+
+
+BookTables build_book(const ProductPlan& d, double width0)
+{
+    vector<const ProductPlan*> parts;
+    if (d.kind == CF_PRODUCT_BOOK) {
+        for (const auto& q : d.parts) parts.push_back(&q);
+    } else {
+        parts.push_back(&d);
+    }
+    const vector<double>& tl = d.timeline;
+    const int ns = int(tl.size());
+    auto at = [&](double t) {
+        auto it = std::lower_bound(tl.begin(), tl.end(), t);
+        if (it == tl.end() || *it != t) fail(CF_ERR_ARG, "book date not on its timeline");
+        return int(it - tl.begin());
+    };
+    vector<vector<std::pair<double, int>>> van(ns);
+    vector<uint32_t> bar_mask(ns, 0u), bar_last(ns, 0u), grp_mask(ns, 0u), grp_last(ns, 0u);
+    vector<double> bar_par;
+    vector<int> bar_col, bar_put;
+    vector<vector<double>> sched;                       // fixing schedules
+    vector<vector<std::tuple<double, int, int>>> members;   // per schedule: strike, column, put
+    int col = 0;
+    for (const ProductPlan* q : parts) {
+        if (q->kind == CF_PRODUCT_EUROPEAN) {
+            van[at(q->exercise)].push_back({q->strike, col++});
+        } else if (q->kind == CF_PRODUCT_EUROPEANS) {
+            for (size_t i = 0; i < q->mats.size(); ++i)
+                for (double k : q->strikes[i]) van[at(q->mats[i])].push_back({k, col++});
+        } else if (q->kind == CF_PRODUCT_BARRIER) {
+            const int b = int(bar_col.size());
+            if (b >= cfk::kMaxBookBarriers)
+                fail(CF_ERR_CAPACITY, "a book holds at most " + std::to_string(cfk::kMaxBookBarriers) + " barriers");
+            for (double t : q->timeline) bar_mask[at(t)] |= 1u << b;
+            bar_last[at(q->timeline.back())] |= 1u << b;
+            const double w = width0 * q->smooth;
+            bar_par.insert(bar_par.end(), {q->strike, q->barrier + w, q->barrier - w, 1.0 / (2 * w)});
+            bar_col.push_back(col);
+            bar_put.push_back(q->put ? 1 : 0);
+            col += 2;
+        } else if (q->kind == CF_PRODUCT_ASIAN) {
+            size_t g = 0;
+            while (g < sched.size() && sched[g] != q->timeline) ++g;
+            if (g == sched.size()) {
+                if (int(g) >= cfk::kMaxBookGroups)
+                    fail(CF_ERR_CAPACITY,
+                         "a book holds at most " + std::to_string(cfk::kMaxBookGroups) + " Asian fixing schedules");
+                sched.push_back(q->timeline);
+                members.emplace_back();
+            }
+            members[g].push_back({q->strike, col++, q->put ? 1 : 0});
+        } else {
+            fail(CF_ERR_UNSUPPORTED, "product kind cannot be held in a book");
+        }
+    }
+    for (size_t g = 0; g < sched.size(); ++g) {
+        for (double t : sched[g]) grp_mask[at(t)] |= 1u << g;
+        grp_last[at(sched[g].back())] |= 1u << g;
+    }
+    vector<int> van_first(1, 0), van_col, grp_first(1, 0), as_col, as_put;
+    vector<double> van_strike, grp_inv, as_strike;
+    for (const auto& v : van) {
+        for (const auto& kc : v) {
+            van_strike.push_back(kc.first);
+            van_col.push_back(kc.second);
+        }
+        van_first.push_back(int(van_strike.size()));
+    }
+    for (size_t g = 0; g < sched.size(); ++g) {
+        grp_inv.push_back(1.0 / double(sched[g].size()));
+        for (const auto& m : members[g]) {
+            as_strike.push_back(std::get<0>(m));
+            as_col.push_back(std::get<1>(m));
+            as_put.push_back(std::get<2>(m));
+        }
+        grp_first.push_back(int(as_strike.size()));
+    }
+    BookTables B;
+    auto put = [&B](const void* p, size_t bytes) {
+        const size_t off = (B.blob.size() + 7) & ~size_t(7);
+        B.blob.resize(off + bytes);
+        if (bytes) std::memcpy(B.blob.data() + off, p, bytes);
+        return int(off);
+    };
+    cfk::BookLayout& L = B.L;
+    L.n_van = int(van_strike.size());
+    L.n_bar = int(bar_col.size());
+    L.n_grp = int(sched.size());
+    L.n_asian = int(as_strike.size());
+    L.van_first = put(van_first.data(), van_first.size() * 4);
+    L.van_strike = put(van_strike.data(), van_strike.size() * 8);
+    L.van_col = put(van_col.data(), van_col.size() * 4);
+    L.bar_mask = put(bar_mask.data(), bar_mask.size() * 4);
+    L.bar_last = put(bar_last.data(), bar_last.size() * 4);
+    L.bar_par = put(bar_par.data(), bar_par.size() * 8);
+    L.bar_col = put(bar_col.data(), bar_col.size() * 4);
+    L.bar_put = put(bar_put.data(), bar_put.size() * 4);
+    L.grp_mask = put(grp_mask.data(), grp_mask.size() * 4);
+    L.grp_last = put(grp_last.data(), grp_last.size() * 4);
+    L.grp_inv = put(grp_inv.data(), grp_inv.size() * 8);
+    L.grp_first = put(grp_first.data(), grp_first.size() * 4);
+    L.as_strike = put(as_strike.data(), as_strike.size() * 8);
+    L.as_col = put(as_col.data(), as_col.size() * 4);
+    L.as_put = put(as_put.data(), as_put.size() * 4);
+    B.blob.resize((B.blob.size() + 15) & ~size_t(15));
+    L.bytes = int(B.blob.size());
+    return B;
+}
+
 
 void validate_model(const cf_model* m)
 {
@@ -449,14 +609,6 @@ vector<string> param_labels(const cf_model* m)
 
 // ------------------------------------------------------------------ knot search
 
-// Bucket table of the kernels' upper_bound (see KnotSearch in cf_kernels.hip),
-// built with the same real type and the same two rounded operations the
-// kernel uses to map a state to its bucket.
-struct BucketTable {
-    vector<uint16_t> table;
-    int nb = 1, fix = 1;
-    double x0 = 0.0, invh = 0.0;
-};
 
 template <class R>
 BucketTable build_buckets(const vector<R>& knots)
@@ -489,92 +641,10 @@ BucketTable build_buckets(const vector<R>& knots)
 
 // ------------------------------------------------------------------ device
 
-// Grow-only device buffer.
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    void* ensure(size_t bytes)
-    {
-        if (bytes <= cap && p) return p;
-        if (p) hip_check(hipFree(p), "hipFree");
-        p = nullptr;
-        cap = 0;
-        size_t want = std::max<size_t>(bytes, 256);
-        hip_check(hipMalloc(&p, want), "hipMalloc");
-        cap = want;
-        return p;
-    }
-};
 
-// Grow-only pinned host buffer (uploads and downloads without staging copies).
-struct PinnedBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    void* ensure(size_t bytes)
-    {
-        if (bytes <= cap && p) return p;
-        if (p) hip_check(hipHostFree(p), "hipHostFree");
-        p = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(bytes, 4096);
-        hip_check(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc");
-        cap = want;
-        return p;
-    }
-};
 
-// Host staging of a prepared plan's tables, one upload.
-struct Staging {
-    vector<unsigned char> bytes;
-    template <class T>
-    size_t add(const T* data, size_t n)
-    {
-        const size_t off = (bytes.size() + 255) & ~size_t(255);
-        bytes.resize(off + n * sizeof(T));
-        if (n) std::memcpy(bytes.data() + off, data, n * sizeof(T));
-        return off;
-    }
-    template <class T>
-    size_t add(const vector<T>& v) { return add(v.data(), v.size()); }
-};
 
-// A model and product prepared for one precision and one kernel: the plans,
-// the knot search table, the shared-memory layout and the tables resident in
-// device memory.  Prepared once per distinct (model, product, precision,
-// adjoint) and reused by later calls.
-struct Prepared {
-    ProductPlan pd;
-    ModelPlan md;
-    BucketTable bt;
-    cfk::KernelConfig cfg{};
-    size_t smem = 0;
-    int per_cu = 0;
-    int block = 0;
-    DevBuf tables;
-    size_t o_ss = 0, o_knots = 0, o_invdx = 0, o_iv = 0, o_drift = 0, o_std = 0, o_bucket = 0;
-    size_t o_ff = 0, o_disc = 0, o_num = 0, o_aux = 0, o_pf = 0, o_pk = 0;
-    // adjoint of the local-vol pre-interpolation: sqrt(dt_i) and the time
-    // interpolation weights of each step
-    vector<double> sqrtdt;
-    vector<Interp1> tw;
-    ~Prepared()
-    {
-        if (tables.p) (void)hipFree(tables.p);
-    }
-};
 
-struct Engine {
-    int device = -1;
-    int num_cus = 0;
-    int max_lds = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};
-    uint32_t* d_dirs = nullptr;
-    DevBuf block_out, ivbar_blocks, reduced, ivbar, per_path, per_path_agg, ckpt, weights;
-    PinnedBuf pin_in, pin_out;
-    std::list<std::pair<string, std::shared_ptr<Prepared>>> plans;   // most recent first
-    std::mutex mu;
-};
 
 constexpr size_t kMaxPlans = 64;
 
@@ -622,18 +692,6 @@ Engine& engine()
     return *slot;
 }
 
-struct Call {
-    bool adjoint = false;
-    const cf_num* num = nullptr;
-    const double* weights = nullptr;
-    // outputs
-    double* means = nullptr;
-    double* std_errs = nullptr;
-    double* per_path = nullptr;
-    double* agg_mean = nullptr;
-    double* risks = nullptr;
-    double* per_path_agg = nullptr;
-};
 
 void validate_num(const cf_num* num)
 {
@@ -648,14 +706,15 @@ void validate_num(const cf_num* num)
 }
 
 // Byte image of everything a prepared plan depends on.
-string plan_key(const cf_model* m, const cf_product* p, int precision, bool adjoint)
+// adj: cfk::AdjointKind
+string plan_key(const cf_model* m, const cf_product* p, int precision, int adj)
 {
     string k;
     auto put = [&k](const void* d, size_t n) { k.append(static_cast<const char*>(d), n); };
     auto putd = [&](double x) { put(&x, sizeof x); };
     auto puti = [&](int x) { put(&x, sizeof x); };
     puti(precision);
-    puti(adjoint ? 1 : 0);
+    puti(adj);
     puti(m->kind);
     putd(m->spot);
     if (m->kind == CF_MODEL_BLACK_SCHOLES) {
@@ -671,28 +730,37 @@ string plan_key(const cf_model* m, const cf_product* p, int precision, bool adjo
         put(m->times, sizeof(double) * size_t(std::max(m->n_times, 0)));
         put(m->vols, sizeof(double) * size_t(std::max(m->n_spots, 0)) * size_t(std::max(m->n_times, 0)));
     }
-    puti(p->kind);
-    if (p->kind == CF_PRODUCT_EUROPEANS) {
-        puti(p->n_options);
-        put(p->maturities, sizeof(double) * size_t(std::max(p->n_options, 0)));
-        put(p->strikes, sizeof(double) * size_t(std::max(p->n_options, 0)));
-    } else {
-        putd(p->strike);
-        putd(p->exercise);
-        putd(p->settlement);
-        putd(p->barrier);
-        putd(p->maturity);
-        putd(p->monitor_freq);
-        putd(p->smooth);
-        puti(p->call_put);
-    }
+    std::function<void(const cf_product*)> put_product = [&](const cf_product* q) {
+        puti(q->kind);
+        if (q->kind == CF_PRODUCT_EUROPEANS) {
+            puti(q->n_options);
+            put(q->maturities, sizeof(double) * size_t(std::max(q->n_options, 0)));
+            put(q->strikes, sizeof(double) * size_t(std::max(q->n_options, 0)));
+        } else if (q->kind == CF_PRODUCT_BOOK) {
+            puti(q->n_parts);
+            for (int i = 0; i < q->n_parts; ++i) put_product(q->parts + i);
+        } else {
+            putd(q->strike);
+            putd(q->exercise);
+            putd(q->settlement);
+            putd(q->barrier);
+            putd(q->maturity);
+            putd(q->monitor_freq);
+            putd(q->smooth);
+            puti(q->call_put);
+            putd(q->fixing_freq);
+            putd(q->snap_freq);
+        }
+    };
+    put_product(p);
     return k;
 }
 
 // Plans, tables and layout of a model and product; FULL layout when it fits,
 // LEAN otherwise.
-std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, bool adjoint)
+std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj)
 {
+    const bool adjoint = adj != cfk::ADJ_NONE;
     auto P = std::make_shared<Prepared>();
     P->pd = plan_product(prd);
     P->md = plan_model(mdl, P->pd);
@@ -712,15 +780,18 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
             P->bt = build_buckets(knots_f);
         }
     }
-    P->cfg = cfk::KernelConfig{fp64 ? 0 : 1, md.kind, pd.kind, adjoint, false};
-    P->block = block_threads();
+    const bool book = kernel_product(pd) == cfk::PRD_BOOK;
+    if (book) P->book = build_book(pd, md.s_t0);
+    P->cfg = cfk::KernelConfig{fp64 ? 0 : 1, md.kind, kernel_product(pd), adjoint, false, adj == cfk::ADJ_RECOMPUTE};
+    P->block = book ? std::min(block_threads(), cfk::kBookBlock) : block_threads();
     const int waves = P->block / cfk::kWave;
     const int forced = forced_layout();
     for (int lean = forced == 1 ? 1 : 0; lean < 2; ++lean) {
         P->cfg.lean = lean != 0;
         const cfk::SmemLayout L =
             cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves, fp64 ? 8 : 4, dup,
-                             pd.kind == CF_PRODUCT_EUROPEANS, adjoint, P->cfg.lean, dup ? P->bt.nb : 0);
+                             pd.kind == CF_PRODUCT_EUROPEANS, adjoint, P->cfg.lean, dup ? P->bt.nb : 0,
+                             book ? P->book.L.bytes : 0);
         P->smem = L.total;
         if (P->smem <= size_t(E.max_lds)) break;
     }
@@ -763,8 +834,8 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
         P->o_pf = st.add(pd.pay_first);
         P->o_pk = st.add(pd.pay_strike);
     }
-    P->tables.ensure(st.bytes.size());
-    hip_check(hipMemcpy(P->tables.p, st.bytes.data(), st.bytes.size(), hipMemcpyHostToDevice), "table upload");
+    if (book) P->o_book = st.add(P->book.blob);
+    P->staged = std::move(st.bytes);
 
     if (dup && adjoint) {
         P->sqrtdt.resize(md.n_steps);
@@ -777,20 +848,30 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
     return P;
 }
 
-std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, bool adjoint)
+std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj,
+                                 bool with_tables)
 {
     if (!mdl) fail(CF_ERR_ARG, "null model");
     if (!prd) fail(CF_ERR_ARG, "null product");
-    const string key = plan_key(mdl, prd, fp64 ? CF_FP64 : CF_FP32, adjoint);
+    const string key = plan_key(mdl, prd, fp64 ? CF_FP64 : CF_FP32, adj);
+    std::lock_guard<std::mutex> lk(E.plan_mu);
+    std::shared_ptr<Prepared> P;
     for (auto it = E.plans.begin(); it != E.plans.end(); ++it) {
         if (it->first == key) {
             E.plans.splice(E.plans.begin(), E.plans, it);
-            return E.plans.front().second;
+            P = E.plans.front().second;
+            break;
         }
     }
-    auto P = prepare(E, mdl, prd, fp64, adjoint);
-    E.plans.emplace_front(key, P);
-    if (E.plans.size() > kMaxPlans) E.plans.pop_back();
+    if (!P) {
+        P = prepare(E, mdl, prd, fp64, adj);
+        E.plans.emplace_front(key, P);
+        if (E.plans.size() > kMaxPlans) E.plans.pop_back();
+    }
+    if (with_tables && !P->tables.p) {
+        P->tables.ensure(P->staged.size());
+        hip_check(hipMemcpy(P->tables.p, P->staged.data(), P->staged.size(), hipMemcpyHostToDevice), "table upload");
+    }
     return P;
 }
 
@@ -799,66 +880,54 @@ double ms_since(std::chrono::steady_clock::time_point t0)
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-int run(const cf_model* mdl, const cf_product* prd, Call& c)
+CallShape shape_call(const Engine& E, const Prepared& PR, const cf_num* num, bool adjoint)
 {
-    const auto t_start = std::chrono::steady_clock::now();
-    validate_num(c.num);
-    if (c.adjoint && !c.weights) fail(CF_ERR_ARG, "null aggregation weights");
-    if (c.adjoint && c.num->aad_mode == CF_AAD_RECOMPUTE)
-        fail(CF_ERR_UNSUPPORTED, "the recompute AAD variant is not available; use the checkpoint variant");
-    const int64_t n_path = c.num->num_path, first = c.num->first_path;
-    const bool fp64 = c.num->precision == CF_FP64;
+    CallShape S;
+    S.first = num->first_path;
+    S.n_path = num->num_path;
+    S.adjoint = adjoint;
+    S.fp64 = PR.cfg.precision == 0;
+    S.waves = PR.block / cfk::kWave;
+    const int64_t q0 = (S.first + 1) >> 6, q1 = (S.first + S.n_path) >> 6;
+    const int64_t want = (q1 - q0 + 1 + S.waves - 1) / S.waves;
+    S.grid = int(std::min<int64_t>(want, int64_t(PR.per_cu) * E.num_cus));
+    S.slots = S.grid * S.waves;
+    S.npay = PR.pd.n_pay;
+    S.stride = cfk::out_stride(S.npay);
+    const bool dup = PR.md.kind == CF_MODEL_DUPIRE;
+    S.ivn = adjoint && dup ? size_t(PR.md.n_steps) * PR.md.n_spots : 0;
+    S.ck_rows = adjoint ? cfk::ckpt_rows(PR.md.n_steps, dup, PR.cfg.recompute, S.fp64 ? 8 : 4) : 0;
+    return S;
+}
 
-    Engine& E = engine();
-    std::lock_guard<std::mutex> lk(E.mu);
-    hip_check(hipSetDevice(E.device), "hipSetDevice");
-    const std::shared_ptr<Prepared> PP = lookup(E, mdl, prd, fp64, c.adjoint);
-    const Prepared& PR = *PP;
+static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+
+size_t CallBytes::total() const
+{
+    return al256(block_out) + al256(reduced) + al256(ivbar_blocks) + al256(ivbar) + al256(weights) + al256(ckpt);
+}
+
+CallBytes call_bytes(const CallShape& S)
+{
+    CallBytes b{};
+    b.block_out = size_t(S.grid) * S.stride * 8;
+    b.reduced = size_t(S.stride) * 8;
+    b.ivbar_blocks = size_t(S.grid) * S.ivn * 8;
+    b.ivbar = S.ivn * 8;
+    b.weights = S.adjoint ? size_t(S.npay) * 8 : 0;
+    b.ckpt = size_t(S.slots) * S.ck_rows * cfk::kWave * (S.fp64 ? 8 : 4);
+    return b;
+}
+
+cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallShape& S, const CallBuffers& B)
+{
     const ProductPlan& pd = PR.pd;
     const ModelPlan& md = PR.md;
-    const int npay = pd.n_pay;
     const bool dup = md.kind == CF_MODEL_DUPIRE;
-    const int stride = cfk::out_stride(npay);
-    const int ns = int(pd.timeline.size());
-
-    // persistent grid: enough waves for the 64-path Sobol blocks, at most the
-    // resident blocks of the device
-    const int waves = PR.block / cfk::kWave;
-    const int64_t q0 = (first + 1) >> 6, q1 = (first + n_path) >> 6;
-    const int64_t want = (q1 - q0 + 1 + waves - 1) / waves;
-    const int grid = int(std::min<int64_t>(want, int64_t(PR.per_cu) * E.num_cus));
-    const int slots = grid * waves;
-
-    double* d_block = static_cast<double*>(E.block_out.ensure(size_t(grid) * stride * 8));
-    double* d_red = static_cast<double*>(E.reduced.ensure(size_t(stride) * 8));
-    const size_t ivn = dup ? size_t(md.n_steps) * md.n_spots : 0;
-    double* d_ivb = nullptr;
-    double* d_iv_red = nullptr;
-    void* d_ck = nullptr;
-    double* d_w = static_cast<double*>(E.weights.ensure(size_t(npay) * 8));
-    if (c.adjoint) {
-        if (dup && ivn) {
-            d_ivb = static_cast<double*>(E.ivbar_blocks.ensure(size_t(grid) * ivn * 8));
-            d_iv_red = static_cast<double*>(E.ivbar.ensure(ivn * 8));
-        }
-        const size_t ck_bytes = size_t(slots) * std::max(md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4);
-        d_ck = E.ckpt.ensure(ck_bytes);
-    }
-    double* d_pp = c.per_path ? static_cast<double*>(E.per_path.ensure(size_t(n_path) * npay * 8)) : nullptr;
-    double* d_ppa = c.per_path_agg ? static_cast<double*>(E.per_path_agg.ensure(size_t(n_path) * 8)) : nullptr;
-    double* h_in = static_cast<double*>(E.pin_in.ensure(size_t(npay) * 8));
-    double* h_out = static_cast<double*>(E.pin_out.ensure((size_t(stride) + ivn) * 8));
-
-    hip_check(hipEventRecord(E.ev[0], E.stream), "hipEventRecord");
-    if (c.adjoint) {
-        std::memcpy(h_in, c.weights, size_t(npay) * 8);
-        hip_check(hipMemcpyAsync(d_w, h_in, size_t(npay) * 8, hipMemcpyHostToDevice, E.stream), "upload");
-    }
-
-    const unsigned char* dt = static_cast<const unsigned char*>(PR.tables.p);
+    const unsigned char* dt = B.tables;
     cfk::LaunchParams P{};
-    P.first_path = first;
-    P.n_path = n_path;
+    P.first_path = S.first;
+    P.n_path = S.n_path;
     P.n_steps = md.n_steps;
     P.dirs = E.d_dirs;
     P.n_spots = md.n_spots;
@@ -873,16 +942,16 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     P.drift = dt + PR.o_drift;
     P.stdv = dt + PR.o_std;
     P.spot = md.spot;
-    P.x0 = fp64 ? std::log(md.spot) : 0.0;
+    P.x0 = S.fp64 ? std::log(md.spot) : 0.0;
     P.spot_measure = md.spot_measure ? 1 : 0;
     P.bs_vol = md.vol;
     P.bs_aux = reinterpret_cast<const double*>(dt + PR.o_aux);
-    P.n_samples = ns;
+    P.n_samples = int(pd.timeline.size());
     P.step_sample = reinterpret_cast<const int*>(dt + PR.o_ss);
     P.ff = reinterpret_cast<const double*>(dt + PR.o_ff);
     P.disc = reinterpret_cast<const double*>(dt + PR.o_disc);
     P.num = reinterpret_cast<const double*>(dt + PR.o_num);
-    P.n_pay = npay;
+    P.n_pay = pd.n_pay;
     P.strike = pd.strike;
     if (pd.kind == CF_PRODUCT_BARRIER) {
         // mcPrd.h:247-250: width from the spot written at t = 0, no derivative
@@ -895,45 +964,36 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     P.put = pd.put ? 1 : 0;
     P.pay_first = reinterpret_cast<const int*>(dt + PR.o_pf);
     P.pay_strike = reinterpret_cast<const double*>(dt + PR.o_pk);
-    P.block_out = d_block;
-    P.per_path = d_pp;
-    P.weights = d_w;
-    P.ckpt = d_ck;
-    P.ivbar_blocks = d_ivb;
-    P.per_path_agg = d_ppa;
+    P.book = dt + PR.o_book;
+    P.bk = PR.book.L;
+    P.block_out = B.block_out;
+    P.per_path = B.per_path;
+    P.weights = B.weights;
+    P.ckpt = B.ckpt;
+    P.ivbar_blocks = B.ivbar_blocks;
+    P.per_path_agg = B.per_path_agg;
+    return P;
+}
 
-    hip_check(hipEventRecord(E.ev[1], E.stream), "hipEventRecord");
-    const double setup_ms = ms_since(t_start);
-    int rc = cfk::launch_path_kernel(PR.cfg, P, grid, PR.block, PR.smem, E.stream);
+void enqueue_call(const Prepared& PR, const CallShape& S, const CallBuffers& B, const cfk::LaunchParams& P,
+                  hipStream_t stream, hipEvent_t ev_kernel0, hipEvent_t ev_kernel1)
+{
+    if (ev_kernel0) hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
+    int rc = cfk::launch_path_kernel(PR.cfg, P, S.grid, PR.block, PR.smem, stream);
     if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
-    hip_check(hipEventRecord(E.ev[2], E.stream), "hipEventRecord");
-    rc = cfk::launch_reduce_rows(d_block, grid, stride, d_red, E.stream);
+    if (ev_kernel1) hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
+    rc = cfk::launch_reduce_rows(B.block_out, S.grid, S.stride, B.reduced, stream);
     if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
-    if (d_ivb) {
-        rc = cfk::launch_reduce_rows(d_ivb, grid, int(ivn), d_iv_red, E.stream);
+    if (S.ivn) {
+        rc = cfk::launch_reduce_rows(B.ivbar_blocks, S.grid, int(S.ivn), B.ivbar, stream);
         if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
     }
-    hip_check(hipEventRecord(E.ev[3], E.stream), "hipEventRecord");
+}
 
-    hip_check(hipMemcpyAsync(h_out, d_red, size_t(stride) * 8, hipMemcpyDeviceToHost, E.stream), "download");
-    if (d_ivb)
-        hip_check(hipMemcpyAsync(h_out + stride, d_iv_red, ivn * 8, hipMemcpyDeviceToHost, E.stream), "download");
-    if (d_pp)
-        hip_check(hipMemcpyAsync(c.per_path, d_pp, size_t(n_path) * npay * 8, hipMemcpyDeviceToHost, E.stream),
-                  "download");
-    if (d_ppa)
-        hip_check(hipMemcpyAsync(c.per_path_agg, d_ppa, size_t(n_path) * 8, hipMemcpyDeviceToHost, E.stream),
-                  "download");
-    hip_check(hipStreamSynchronize(E.stream), "kernel execution");
-
-    float t_main = 0, t_red = 0, t_all = 0;
-    hip_check(hipEventElapsedTime(&t_main, E.ev[1], E.ev[2]), "event");
-    hip_check(hipEventElapsedTime(&t_red, E.ev[2], E.ev[3]), "event");
-    hip_check(hipEventElapsedTime(&t_all, E.ev[0], E.ev[3]), "event");
-
-    // ---- results
-    const double* red = h_out;
-    const double np = double(n_path);
+void collect_results(const Prepared& PR, const CallShape& S, const double* red, const double* ivbar, const Call& c)
+{
+    const int npay = S.npay;
+    const double np = double(S.n_path);
     for (int k = 0; k < npay; ++k) {
         const double mean = red[k] / np;
         if (c.means) c.means[k] = mean;
@@ -942,44 +1002,114 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
             c.std_errs[k] = std::sqrt(var / np);
         }
     }
-    if (c.adjoint) {
-        if (c.agg_mean) c.agg_mean[0] = red[2 * npay] / np;
-        const double* padj = red + 2 * npay + 2;
-        if (dup) {
-            const int nS = md.n_spots, nT = md.n_times;
-            const double* ivbar = h_out + stride;
-            double* r = c.risks;
-            std::fill(r, r + 1 + size_t(nS) * nT, 0.0);
-            r[0] = padj[0] / np;
-            // adjoint of interpVols[i][j] = sqrt(dt_i) * interp_t(vols[j], t_i)
-            for (int i = 0; i < md.n_steps; ++i) {
-                const Interp1& ip = PR.tw[i];
-                for (int j = 0; j < nS; ++j) {
-                    const double a = ivbar[size_t(i) * nS + j] * PR.sqrtdt[i];
-                    if (a == 0.0) continue;
-                    r[1 + size_t(j) * nT + ip.j0] += a * ip.w0;
-                    if (ip.w1 != 0.0) r[1 + size_t(j) * nT + ip.j1] += a * ip.w1;
-                }
+    if (!S.adjoint) return;
+    if (c.agg_mean) c.agg_mean[0] = red[2 * npay] / np;
+    const double* padj = red + 2 * npay + 2;
+    const ModelPlan& md = PR.md;
+    if (md.kind == CF_MODEL_DUPIRE) {
+        const int nS = md.n_spots, nT = md.n_times;
+        double* r = c.risks;
+        std::fill(r, r + 1 + size_t(nS) * nT, 0.0);
+        r[0] = padj[0] / np;
+        // adjoint of interpVols[i][j] = sqrt(dt_i) * interp_t(vols[j], t_i)
+        for (int i = 0; i < md.n_steps; ++i) {
+            const Interp1& ip = PR.tw[i];
+            for (int j = 0; j < nS; ++j) {
+                const double a = ivbar[size_t(i) * nS + j] * PR.sqrtdt[i];
+                if (a == 0.0) continue;
+                r[1 + size_t(j) * nT + ip.j0] += a * ip.w0;
+                if (ip.w1 != 0.0) r[1 + size_t(j) * nT + ip.j1] += a * ip.w1;
             }
-            for (size_t k = 1; k < 1 + size_t(nS) * nT; ++k) r[k] /= np;
-        } else {
-            for (int k = 0; k < 4; ++k) c.risks[k] = padj[k] / np;
         }
+        for (size_t k = 1; k < 1 + size_t(nS) * nT; ++k) r[k] /= np;
+    } else {
+        for (int k = 0; k < 4; ++k) c.risks[k] = padj[k] / np;
     }
+}
+
+int adjoint_kind(const Call& c)
+{
+    if (!c.adjoint) return cfk::ADJ_NONE;
+    return c.num->aad_mode == CF_AAD_RECOMPUTE ? cfk::ADJ_RECOMPUTE : cfk::ADJ_CHECKPOINT;
+}
+
+// Synchronous call on the engine's own stream and buffers.
+int run(const cf_model* mdl, const cf_product* prd, Call& c)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    validate_num(c.num);
+    if (c.adjoint && !c.weights) fail(CF_ERR_ARG, "null aggregation weights");
+    const bool fp64 = c.num->precision == CF_FP64;
+
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lk(E.sync_mu);
+    hip_check(hipSetDevice(E.device), "hipSetDevice");
+    const std::shared_ptr<Prepared> PP = lookup(E, mdl, prd, fp64, adjoint_kind(c), true);
+    const Prepared& PR = *PP;
+    const CallShape S = shape_call(E, PR, c.num, c.adjoint);
+    const CallBytes nb = call_bytes(S);
+
+    CallBuffers B;
+    B.tables = static_cast<const unsigned char*>(PR.tables.p);
+    B.block_out = static_cast<double*>(E.block_out.ensure(nb.block_out));
+    B.reduced = static_cast<double*>(E.reduced.ensure(nb.reduced));
+    if (S.ivn) {
+        B.ivbar_blocks = static_cast<double*>(E.ivbar_blocks.ensure(nb.ivbar_blocks));
+        B.ivbar = static_cast<double*>(E.ivbar.ensure(nb.ivbar));
+    }
+    if (c.adjoint) {
+        B.weights = static_cast<double*>(E.weights.ensure(nb.weights));
+        B.ckpt = E.ckpt.ensure(nb.ckpt);
+    }
+    if (c.per_path) B.per_path = static_cast<double*>(E.per_path.ensure(size_t(S.n_path) * S.npay * 8));
+    if (c.per_path_agg) B.per_path_agg = static_cast<double*>(E.per_path_agg.ensure(size_t(S.n_path) * 8));
+    double* h_in = static_cast<double*>(E.pin_in.ensure(size_t(S.npay) * 8));
+    double* h_out = static_cast<double*>(E.pin_out.ensure((size_t(S.stride) + S.ivn) * 8));
+
+    hip_check(hipEventRecord(E.ev[0], E.stream), "hipEventRecord");
+    if (c.adjoint) {
+        std::memcpy(h_in, c.weights, size_t(S.npay) * 8);
+        hip_check(hipMemcpyAsync(B.weights, h_in, size_t(S.npay) * 8, hipMemcpyHostToDevice, E.stream), "upload");
+    }
+    const cfk::LaunchParams P = launch_params(E, PR, S, B);
+    const double setup_ms = ms_since(t_start);
+    enqueue_call(PR, S, B, P, E.stream, E.ev[1], E.ev[2]);
+    hip_check(hipEventRecord(E.ev[3], E.stream), "hipEventRecord");
+
+    hip_check(hipMemcpyAsync(h_out, B.reduced, size_t(S.stride) * 8, hipMemcpyDeviceToHost, E.stream), "download");
+    if (S.ivn)
+        hip_check(hipMemcpyAsync(h_out + S.stride, B.ivbar, S.ivn * 8, hipMemcpyDeviceToHost, E.stream), "download");
+    if (B.per_path)
+        hip_check(hipMemcpyAsync(c.per_path, B.per_path, size_t(S.n_path) * S.npay * 8, hipMemcpyDeviceToHost,
+                                 E.stream), "download");
+    if (B.per_path_agg)
+        hip_check(hipMemcpyAsync(c.per_path_agg, B.per_path_agg, size_t(S.n_path) * 8, hipMemcpyDeviceToHost,
+                                 E.stream), "download");
+    hip_check(hipStreamSynchronize(E.stream), "kernel execution");
+
+    float t_main = 0, t_red = 0, t_all = 0;
+    hip_check(hipEventElapsedTime(&t_main, E.ev[1], E.ev[2]), "event");
+    hip_check(hipEventElapsedTime(&t_red, E.ev[2], E.ev[3]), "event");
+    hip_check(hipEventElapsedTime(&t_all, E.ev[0], E.ev[3]), "event");
+
+    collect_results(PR, S, h_out, h_out + S.stride, c);
+
     g_stats.main_kernel_ms = t_main;
     g_stats.reduce_ms = t_red;
     g_stats.total_ms = t_all;
-    g_stats.grid_blocks = grid;
+    g_stats.grid_blocks = S.grid;
     g_stats.block_threads = PR.block;
     g_stats.lds_bytes = int(PR.smem);
-    g_stats.path_steps = n_path * int64_t(md.n_steps);
+    g_stats.path_steps = S.n_path * int64_t(PR.md.n_steps);
     g_stats.setup_ms = setup_ms;
     g_stats.host_ms = ms_since(t_start);
     g_stats.lean_layout = PR.cfg.lean ? 1 : 0;
     return CF_OK;
 }
 
-}  // namespace
+}  // namespace cfh
+
+using namespace cfh;
 
 void cfk::set_last_error(const char* msg) { g_err = msg; }
 
@@ -996,7 +1126,7 @@ int cf_diag_copy(size_t bytes, int reps)
     return guarded([&] {
         if (bytes < 16 || reps < 1) fail(CF_ERR_ARG, "bad copy size");
         Engine& E = engine();
-        std::lock_guard<std::mutex> lk(E.mu);
+        std::lock_guard<std::mutex> lk(E.sync_mu);
         void *a = nullptr, *b = nullptr;
         hip_check(hipMalloc(&a, bytes), "hipMalloc");
         hip_check(hipMalloc(&b, bytes), "hipMalloc");
@@ -1132,15 +1262,10 @@ int64_t cf_aad_footprint(const cf_model* mdl, const cf_product* prd, const cf_nu
     const int rc = guarded([&] {
         validate_num(num);
         Engine& E = engine();
-        std::lock_guard<std::mutex> lk(E.mu);
         const bool fp64 = num->precision == CF_FP64;
-        const std::shared_ptr<Prepared> P = lookup(E, mdl, prd, fp64, true);
-        const int waves = P->block / cfk::kWave;
-        const int64_t q0 = (int64_t(num->first_path) + 1) >> 6, q1 = (int64_t(num->first_path) + num->num_path) >> 6;
-        const int64_t grid = std::min<int64_t>((q1 - q0 + 1 + waves - 1) / waves, int64_t(P->per_cu) * E.num_cus);
-        const int64_t ivn = P->md.kind == CF_MODEL_DUPIRE ? int64_t(P->md.n_steps) * P->md.n_spots : 0;
-        bytes = grid * waves * std::max(P->md.n_steps, 1) * cfk::kWave * (fp64 ? 8 : 4) +
-                grid * (ivn + cfk::out_stride(P->pd.n_pay)) * 8;
+        const int adj = num->aad_mode == CF_AAD_RECOMPUTE ? cfk::ADJ_RECOMPUTE : cfk::ADJ_CHECKPOINT;
+        const std::shared_ptr<Prepared> P = lookup(E, mdl, prd, fp64, adj, false);
+        bytes = int64_t(call_bytes(shape_call(E, *P, num, true)).total() + P->staged.size());
         return CF_OK;
     });
     return rc == CF_OK ? bytes : int64_t(rc);

@@ -14,10 +14,53 @@
 namespace cfk {
 
 enum ModelKind { MODEL_BS = 0, MODEL_DUPIRE = 1 };
-enum ProductKind { PRD_EUROPEAN = 0, PRD_BARRIER = 1, PRD_EUROPEANS = 2 };
+enum ProductKind { PRD_EUROPEAN = 0, PRD_BARRIER = 1, PRD_EUROPEANS = 2, PRD_BOOK = 3 };
+enum AdjointKind { ADJ_NONE = 0, ADJ_CHECKPOINT = 1, ADJ_RECOMPUTE = 2 };
 
 constexpr int kWave = 64;
 constexpr int kSobolDims = 1101;     // direction-number table stride
+
+// Recompute variant: the Dupire state is kept at every kSeg-th timeline
+// point (8 steps in FP32, 4 in FP64: the re-simulated segment lives in
+// registers); Black-Scholes keeps nothing (normals are regenerated).
+constexpr int kSegF32 = 8, kSegF64 = 4;
+
+// Book of products on shared paths (also an Asian on its own): vanilla calls
+// attached to samples, up-and-out barriers and arithmetic-average groups.  Each
+// barrier and each distinct fixing schedule keeps per-lane state in registers,
+// hence the bounds; the book kernels run 512-thread blocks for the registers.
+constexpr int kMaxBookBarriers = 16, kMaxBookGroups = 4;
+constexpr int kBookBlock = 512;
+
+// Byte offsets of the book tables inside one blob (staged whole in LDS).
+struct BookLayout {
+    int n_van, n_bar, n_grp, n_asian;
+    int bytes;
+    int van_first;    // [n_samples + 1] int: vanillas of sample s
+    int van_strike;   // [n_van] double
+    int van_col;      // [n_van] int: payoff column
+    int bar_mask;     // [n_samples] uint32: barriers observing at s
+    int bar_last;     // [n_samples] uint32: barriers maturing at s
+    int bar_par;      // [n_bar][4] double: strike, B + w, B - w, 1 / 2w
+    int bar_col;      // [n_bar] int: column of the barrier payoff (vanilla next)
+    int bar_put;      // [n_bar] int
+    int grp_mask;     // [n_samples] uint32: schedules fixing at s
+    int grp_last;     // [n_samples] uint32: schedules ending at s
+    int grp_inv;      // [n_grp] double: 1 / number of fixings
+    int grp_first;    // [n_grp + 1] int: Asians of each schedule
+    int as_strike;    // [n_asian] double
+    int as_col;       // [n_asian] int
+    int as_put;       // [n_asian] int
+};
+
+// Rows of 64 reals in a wave's checkpoint slot.
+CF_HD inline int ckpt_rows(int n_steps, bool dupire, bool recompute, int real_bytes)
+{
+    if (!recompute) return n_steps;
+    if (!dupire) return 0;
+    const int seg = real_bytes == 4 ? kSegF32 : kSegF64;
+    return (n_steps + seg - 1) / seg;
+}
 
 // Per-block accumulator row: payoff sums, payoff sums of squares, aggregate
 // sum, aggregate sum of squares, then up to kMaxParamAdj parameter adjoints
@@ -65,6 +108,8 @@ struct LaunchParams {
     int put;
     const int* pay_first;        // Europeans [n_samples + 1]: payoffs of sample s
     const double* pay_strike;    // Europeans [n_pay]
+    const void* book;            // book tables, layout bk
+    BookLayout bk;
 
     // outputs
     double* block_out;           // [gridDim][out_stride(n_pay)]
@@ -72,14 +117,15 @@ struct LaunchParams {
 
     // adjoint
     const double* weights;       // [n_pay] aggregation weights
-    void* ckpt;                  // [slots][n_steps][64] path states
+    void* ckpt;                  // [slots][ckpt_rows][64] path states or normals
     double* ivbar_blocks;        // [gridDim][n_steps][n_spots] local-vol adjoints
     double* per_path_agg;        // [n_path] or null
 };
 
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
-    size_t lo, hi, step_sample, knots, inv_dx, bucket, iv, drift, stdv, samples, pay_first, pay_strike, acc, ivbar, total;
+    size_t lo, hi, step_sample, knots, inv_dx, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book, acc,
+        ivbar, total;
 };
 
 CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -91,7 +137,7 @@ CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 // table from global memory through the caches.
 CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int n_pay, int waves,
                                     int real_bytes, bool dupire, bool europeans, bool adjoint, bool lean,
-                                    int n_buckets)
+                                    int n_buckets, int book_bytes)
 {
     SmemLayout L{};
     size_t o = 0;
@@ -107,6 +153,7 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.samples = o;     o = cf_align16(o + (dupire ? 0 : size_t(n_samples) * 3 * 8));
     L.pay_first = o;   o = cf_align16(o + (europeans ? size_t(n_samples + 1) * 4 : 0));
     L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
+    L.book = o;        o = cf_align16(o + size_t(book_bytes));
     L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
     L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(n_steps) * n_spots * 8 : 0));
     L.total = o;
@@ -120,7 +167,19 @@ struct KernelConfig {
     int product;
     bool adjoint;
     bool lean;       // LEAN shared-memory layout
+    bool recompute;  // adjoint by re-simulation instead of per-step checkpoints
 };
+
+// Path kernel families (precision r: 0 FP64, 1 FP32; adjoint a: AdjointKind),
+// one per build of cf_kern_inst.hip: kernel for a model, product and layout.
+#define CF_PICK_CAT(r, a) pick_r##r##_a##a
+#define CF_PICK_NAME(r, a) CF_PICK_CAT(r, a)
+const void* pick_r0_a0(int model, int prd, bool lean);
+const void* pick_r0_a1(int model, int prd, bool lean);
+const void* pick_r0_a2(int model, int prd, bool lean);
+const void* pick_r1_a0(int model, int prd, bool lean);
+const void* pick_r1_a1(int model, int prd, bool lean);
+const void* pick_r1_a2(int model, int prd, bool lean);
 
 // Returns 0 on success, else a hipError_t value.
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block,

@@ -1,0 +1,45 @@
+// One (precision, adjoint) family of path kernels, instantiated for every
+// model, product and LDS layout.  Built once per family with
+// -DCF_INST_REAL=0|1 (FP64, FP32) and -DCF_INST_ADJ=0|1|2 (cfk::AdjointKind),
+// so the families compile in parallel.
+#include <type_traits>
+
+#include "cf_path_kernel.h"
+
+#if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ)
+#error "CF_INST_REAL and CF_INST_ADJ must be defined"
+#endif
+
+namespace cfk {
+namespace {
+
+using Real = std::conditional_t<CF_INST_REAL == 0, double, float>;
+constexpr int kAdj = CF_INST_ADJ;
+
+template <int MODEL, int PRD>
+const void* ptr(bool lean)
+{
+    return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true>)
+                : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false>);
+}
+
+template <int MODEL>
+const void* by_product(int prd, bool lean)
+{
+    switch (prd) {
+    case PRD_EUROPEAN: return ptr<MODEL, PRD_EUROPEAN>(lean);
+    case PRD_BARRIER: return ptr<MODEL, PRD_BARRIER>(lean);
+    case PRD_EUROPEANS: return ptr<MODEL, PRD_EUROPEANS>(lean);
+    case PRD_BOOK: return ptr<MODEL, PRD_BOOK>(lean);
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+const void* CF_PICK_NAME(CF_INST_REAL, CF_INST_ADJ)(int model, int prd, bool lean)
+{
+    return model == MODEL_DUPIRE ? by_product<MODEL_DUPIRE>(prd, lean) : by_product<MODEL_BS>(prd, lean);
+}
+
+}  // namespace cfk

@@ -1,0 +1,923 @@
+// HIP path kernels for gfx950 (MI355X): Sobol -> inverse normal -> model
+// step -> product, one path per lane, forward (pricing) and forward +
+// hand-written adjoint (risk).
+//
+// Reference semantics followed: sobol.h:55-151, gaussians.h:47-87,
+// interp.h:26-63, mcMdlDupire.h:238-280, mcMdlBS.h:321-350,
+// mcPrd.h:113-125, :235-288, :374-399; the adjoint reproduces the derivative
+// conventions of the reference tape (AADExpr.h:115-253, :364-598).
+//
+// Block organisation
+//   * a persistent grid; wave w of the grid takes Sobol blocks q = w, w + W, ...
+//     of 64 consecutive Sobol indices n = 64q + lane (path p = n - 1);
+//   * Gray code splits as gray(64q + r) = gray6(r) ^ (q & 1) << 5 ^ gray(q) << 6,
+//     so the point of lane r in dimension d is  LO[d][r] ^ HI_q[d]:  LO is a
+//     per-block LDS table built once, HI_q a per-wave LDS row built per block
+//     of paths;
+//   * local-vol table, log-spot knots, timeline sample map in LDS;
+//   * payoffs reduced per wave with cross-lane shuffles into per-wave LDS
+//     accumulators, then per block, then across blocks by a separate kernel
+//     in fixed order;
+//   * adjoint: path states checkpointed in a per-wave HBM slot (reused by the
+//     wave for each block of paths), reverse sweep over the Euler steps,
+//     local-vol adjoints accumulated with LDS atomics on a per-block
+//     (step x spot) grid.
+// This header holds the device helpers and the path kernel template; each
+// cf_kern_inst.hip build instantiates one (precision, adjoint) family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cf_internal.h"
+
+namespace cfk {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// a / b in FP64 from the hardware reciprocal refined by two Newton steps and a
+// residual correction (within an ulp of the IEEE quotient)
+__device__ __forceinline__ double fast_div(double a, double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    r = fma(fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+}
+
+// Natural logarithm of a positive normal double: x = 2^e m with m in
+// [1/sqrt(2), sqrt(2)), ln(m) = 2 atanh(s) with s = (m - 1) / (m + 1),
+// |s| <= 0.1716, summed to s^21 (truncation below 1e-17 relative), and
+// e ln(2) added in two parts.  About two ulps; used where the argument is
+// known to be positive and finite (the inverse-normal tail).
+__device__ __forceinline__ double log_pos(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);   // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);       // x = m 2^e
+    const bool lo = m < 0.70710678118654752440;
+    m = lo ? m + m : m;
+    e = lo ? e - 1 : e;
+    const double s = fast_div(m - 1.0, m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 21.0;
+    p = fma(p, s2, 1.0 / 19.0);
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    const double ed = double(e);
+    const double ln2_hi = 6.93147180369123816490e-01;   // 0x3FE62E42FEE00000, exact in ed * ln2_hi
+    const double ln2_lo = 1.90821492927058770002e-10;
+    // ed * ln2_hi is exact (32-bit constant times an 11-bit integer)
+    return fma(ed, ln2_hi, fma(2.0 * s * s2, p, fma(ed, ln2_lo, 2.0 * s)));
+}
+
+// Beasley-Springer-Moro inverse normal of u = ONEOVER2POW32 * xi
+// (gaussians.h:47-87, sobol.h:26, :110-116), FP64, same branches and
+// constants as the reference.
+__device__ __forceinline__ double inv_normal(uint32_t xi, double)
+{
+    const double p = 2.3283064365387E-10 * double(xi);
+    const bool sup = p > 0.5;
+    const double up = sup ? 1.0 - p : p;
+    const double x = up - 0.5;
+    if (fabs(x) < 0.42) {
+        const double r = x * x;
+        const double z = x * (((-25.44106049637 * r + 41.39119773534) * r + -18.61500062529) * r + 2.50662823884) /
+                         ((((3.13082909833 * r + -21.06224101826) * r + 23.08336743743) * r + -8.47351093090) * r + 1.0);
+        return sup ? -z : z;
+    }
+    double r = log_pos(-log_pos(up));
+    r = 0.3374754822726147 +
+        r * (0.9761690190917186 +
+        r * (0.1607979714918209 +
+        r * (0.0276438810333863 +
+        r * (0.0038405729373609 +
+        r * (0.0003951896511919 +
+        r * (0.0000321767881768 +
+        r * (0.0000002888167364 + r * 0.0000003960315187)))))));
+    return sup ? r : -r;
+}
+
+// FP32 form: Giles' single-precision erfinv approximation (M. Giles,
+// "Approximating the erfinv function", GPU Computing Gems Jade Edition,
+// 2011), z = sqrt(2) erfinv(2u - 1).  Its central branch covers 99.7% of the
+// draws (Moro's 84%), so a wave rarely runs the tail branch.  The tail mass
+// min(u, 1 - u) is formed exactly in the integer domain (2^32 - xi) before
+// scaling, so no precision is lost near u = 1.  Differences to the FP64
+// Moro values are of the order of FP32 rounding.
+__device__ __forceinline__ float inv_normal(uint32_t xi, float)
+{
+    const bool sup = xi >= 0x80000000u;              // u > 1/2 for the reference scaling
+    const uint32_t xs = sup ? (0u - xi) : xi;         // 2^32 - xi, exact
+    const float up = float(xs) * 2.3283064e-10f;      // min(u, 1 - u)
+    const float ax = 1.0f - 2.0f * up;                // |2u - 1|
+    const float ln2 = 0.693147180559945f;
+    // w = -ln((1 - x)(1 + x)) = -ln(4 up (1 - up)); the argument is a normal number
+    float w = -__builtin_amdgcn_logf(4.0f * up * (1.0f - up)) * ln2;
+    float p;
+    if (w < 5.0f) {
+        w = w - 2.5f;
+        p = 2.81022636e-08f;
+        p = 3.43273939e-07f + p * w;
+        p = -3.5233877e-06f + p * w;
+        p = -4.39150654e-06f + p * w;
+        p = 0.00021858087f + p * w;
+        p = -0.00125372503f + p * w;
+        p = -0.00417768164f + p * w;
+        p = 0.246640727f + p * w;
+        p = 1.50140941f + p * w;
+    } else {
+        w = __builtin_amdgcn_sqrtf(w) - 3.0f;
+        p = -0.000200214257f;
+        p = 0.000100950558f + p * w;
+        p = 0.00134934322f + p * w;
+        p = -0.00367342844f + p * w;
+        p = 0.00573950773f + p * w;
+        p = -0.0076224613f + p * w;
+        p = 0.00943887047f + p * w;
+        p = 1.00167406f + p * w;
+        p = 2.83297682f + p * w;
+    }
+    const float z = 1.41421356237f * p * ax;
+    return sup ? z : -z;
+}
+
+// Spot from the path state: FP64 state is log(S) (reference), FP32 state is
+// log(S / S0), which keeps more significant bits in single precision.
+__device__ __forceinline__ double state_spot(double x, double) { return exp(x); }
+__device__ __forceinline__ float state_spot(float x, double s0) { return float(s0) * __expf(x); }
+
+__device__ __forceinline__ double rexp(double x) { return exp(x); }
+__device__ __forceinline__ float rexp(float x) { return __expf(x); }
+
+// Local vol at state x on one timeline row (interp.h:26-63): upper_bound on
+// the knots, linear in between, flat outside.  Returns the two grid nodes
+// touched and their weights for the adjoint.
+template <typename R>
+struct Bracket {
+    R vol;
+    R slope;      // d vol / d x
+    int j;        // left node
+    R wa, wb;     // adjoint weights of nodes j and j + 1
+};
+
+// upper_bound on the knots (interp.h:40) through a uniform bucket table:
+// bucket b = min(floor((x - x0) * invh), nb - 1) holds the number of knots
+// in earlier buckets, a lower bound of the answer; at most `fix` knots fall in
+// one bucket and are resolved by comparisons.  The host builds the table with
+// the kernel's own arithmetic, so the answer is exactly upper_bound's.
+template <typename R>
+struct KnotSearch {
+    const R* knots;
+    const uint16_t* table;
+    int m, nb, fix;
+    R x0, invh;
+};
+
+template <typename R>
+__device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
+{
+    // states below the first knot map to bucket 0, whose count is 0 and whose
+    // comparisons all fail; NaN maps there too
+    R bf = (x - s.x0) * s.invh;
+    bf = bf > R(0) ? bf : R(0);
+    bf = bf < R(s.nb - 1) ? bf : R(s.nb - 1);
+    int lo = s.table[int(bf)];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int c = lo < s.m ? lo : s.m - 1;
+        lo += (lo < s.m && s.knots[c] <= x) ? 1 : 0;
+    }
+    for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
+        const int c = lo < s.m ? lo : s.m - 1;
+        lo += (lo < s.m && s.knots[c] <= x) ? 1 : 0;
+    }
+    return lo;
+}
+
+// Branch-free: every load is in range (nodes clamped), the extrapolation
+// cases are selected afterwards.
+template <typename R>
+__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x)
+{
+    const int m = ks.m;
+    const int lo = count_le(ks, x);   // number of knots <= x
+    int j = lo - 1;
+    j = j > m - 2 ? m - 2 : j;
+    j = j < 0 ? 0 : j;
+    const int j2 = j + 1 < m ? j + 1 : j;
+    const R y1 = row[j], y2 = row[j2];
+    const R x1 = ks.knots[j], ih = inv_dx[j];
+    const bool right = lo >= m && m > 1;
+    const bool left = !right && (lo == 0 || m == 1);
+    const R t = (x - x1) * ih;
+    const R dy = y2 - y1;
+    Bracket<R> b;
+    b.j = j;
+    b.vol = left ? y1 : (right ? y2 : y1 + dy * t);
+    b.slope = (left || right) ? R(0) : dy * ih;
+    b.wa = left ? R(1) : (right ? R(0) : R(1) - t);
+    b.wb = left ? R(0) : (right ? R(1) : t);
+    return b;
+}
+
+__device__ __forceinline__ double recip_mul(double a, double b) { return fast_div(a, b); }
+__device__ __forceinline__ float recip_mul(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
+// Log-Euler step of the Dupire state (mcMdlDupire.h:271); one definition for
+// the forward sweep and the recompute variant's re-simulation, so both give
+// the same states.
+template <typename R>
+__device__ __forceinline__ R euler_step(R x, R vol, R g)
+{
+    return x + vol * (R(-0.5) * vol + g);
+}
+
+// ------------------------------------------------------------------ products
+
+// Per-lane product state carried along the path.
+template <typename R>
+struct ProductState {
+    R alive;        // barrier: product of smoothing factors (before a breach)
+    R pnz;          // barrier: product of the non-zero smoothing factors
+    int nzero;      // barrier: number of exactly-zero factors
+    bool dead;      // barrier: breached (alive forced to 0, no derivative)
+    R euro;         // barrier: vanilla payoff at maturity
+    R fin_f;        // barrier: final forward
+    R fin_num;      // barrier: final numeraire
+    R aggp;         // running aggregate of payoffs emitted at samples
+};
+
+// ------------------------------------------------------------------ kernel
+
+template <typename R, int MODEL, int PRD, int ADJ, bool LEAN>
+__global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kernel(const LaunchParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool DUP = MODEL == MODEL_DUPIRE;
+    constexpr bool AD = ADJ != ADJ_NONE;
+    constexpr bool RECOMP = ADJ == ADJ_RECOMPUTE;
+    constexpr int kSeg = sizeof(R) == 4 ? kSegF32 : kSegF64;   // recompute segment
+    constexpr bool EUS = PRD == PRD_EUROPEANS;
+    const int waves = blockDim.x >> 6;
+    const int n = P.n_steps, m = P.n_spots, ns = P.n_samples, npay = P.n_pay;
+    const int stride = out_stride(npay);
+    constexpr bool BOOK = PRD == PRD_BOOK;
+    const SmemLayout L = smem_layout(n, m, ns, npay, waves, int(sizeof(R)), DUP, EUS, AD, LEAN, P.n_buckets,
+                                     BOOK ? P.bk.bytes : 0);
+
+    uint32_t* lo_tab = reinterpret_cast<uint32_t*>(smem + L.lo);
+    uint32_t* hi_all = reinterpret_cast<uint32_t*>(smem + L.hi);
+    int* step_sample = reinterpret_cast<int*>(smem + L.step_sample);
+    R* knots = reinterpret_cast<R*>(smem + L.knots);
+    R* inv_dx = reinterpret_cast<R*>(smem + L.inv_dx);
+    uint16_t* bucket = reinterpret_cast<uint16_t*>(smem + L.bucket);
+    R* iv_lds = reinterpret_cast<R*>(smem + L.iv);
+    R* drift = reinterpret_cast<R*>(smem + L.drift);
+    R* stdv = reinterpret_cast<R*>(smem + L.stdv);
+    double* s_ff = reinterpret_cast<double*>(smem + L.samples);
+    double* s_disc = s_ff + ns;
+    double* s_num = s_disc + ns;
+    int* pay_first = reinterpret_cast<int*>(smem + L.pay_first);
+    double* pay_strike = reinterpret_cast<double*>(smem + L.pay_strike);
+    double* acc = reinterpret_cast<double*>(smem + L.acc);
+    double* ivbar = reinterpret_cast<double*>(smem + L.ivbar);
+
+    const int tid = threadIdx.x, nthr = blockDim.x;
+
+    // ---- stage the block's tables in LDS
+    if constexpr (LEAN) {
+        for (int e = tid; e < n * 8; e += nthr) {
+            const int d = e >> 3, k = e & 7;
+            lo_tab[e] = k < 6 ? P.dirs[k * kSobolDims + d] : 0u;
+        }
+    } else {
+        for (int e = tid; e < n * kWave; e += nthr) {
+            const int d = e >> 6, r = e & 63;
+            const uint32_t g = uint32_t(r ^ (r >> 1));
+            uint32_t x = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if ((g >> k) & 1u) x ^= P.dirs[k * kSobolDims + d];
+            lo_tab[e] = x;
+        }
+    }
+    for (int e = tid; e <= n; e += nthr) step_sample[e] = P.step_sample[e];
+    if constexpr (DUP) {
+        const R* gk = static_cast<const R*>(P.knots);
+        const R* gi = static_cast<const R*>(P.inv_dx);
+        const R* gv = static_cast<const R*>(P.iv);
+        for (int e = tid; e < m; e += nthr) { knots[e] = gk[e]; inv_dx[e] = gi[e]; }
+        for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = P.bucket[e];
+        if constexpr (!LEAN)
+            for (int e = tid; e < n * m; e += nthr) iv_lds[e] = gv[e];
+    } else {
+        const R* gd = static_cast<const R*>(P.drift);
+        const R* gs = static_cast<const R*>(P.stdv);
+        for (int e = tid; e < n; e += nthr) { drift[e] = gd[e]; stdv[e] = gs[e]; }
+        for (int e = tid; e < ns; e += nthr) { s_ff[e] = P.ff[e]; s_disc[e] = P.disc[e]; s_num[e] = P.num[e]; }
+    }
+    if constexpr (EUS) {
+        for (int e = tid; e <= ns; e += nthr) pay_first[e] = P.pay_first[e];
+        for (int e = tid; e < npay; e += nthr) pay_strike[e] = P.pay_strike[e];
+    }
+    if constexpr (BOOK) {
+        const uint32_t* src = static_cast<const uint32_t*>(P.book);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(smem + L.book);
+        for (int e = tid; e < P.bk.bytes / 4; e += nthr) dst[e] = src[e];
+    }
+    for (int e = tid; e < waves * stride; e += nthr) acc[e] = 0.0;
+    if constexpr (AD && DUP)
+        for (int e = tid; e < n * m; e += nthr) ivbar[e] = 0.0;
+    __syncthreads();
+
+    const int wave = tid >> 6, lane = tid & 63;
+    // local-vol table: LDS (FULL) or global memory (LEAN)
+    const R* iv;
+    if constexpr (LEAN) iv = static_cast<const R*>(P.iv); else iv = iv_lds;
+    // LEAN: masks of the bits of gray6(lane), selecting direction numbers 0..5
+    uint32_t lmask[6];
+    {
+        const uint32_t g6 = uint32_t(lane ^ (lane >> 1));
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lmask[k] = ((g6 >> k) & 1u) ? 0xffffffffu : 0u;
+    }
+    // low Sobol word of dimension i for this lane
+    auto low_word = [&](int i) -> uint32_t {
+        if constexpr (LEAN) {
+            const uint32_t* v = lo_tab + i * 8;
+            return (lmask[0] & v[0]) ^ (lmask[1] & v[1]) ^ (lmask[2] & v[2]) ^ (lmask[3] & v[3]) ^
+                   (lmask[4] & v[4]) ^ (lmask[5] & v[5]);
+        } else {
+            return lo_tab[i * kWave + lane];
+        }
+    };
+    uint32_t* hi = hi_all + wave * n;
+    double* accw = acc + wave * stride;
+    const int64_t nlo = P.first_path + 1, nhi = P.first_path + P.n_path;   // Sobol indices
+    const int64_t q0 = nlo >> 6, q1 = nhi >> 6;
+    const int slot = blockIdx.x * waves + wave;
+    // checkpoint slot of the wave: [n][64] states (Dupire) or normals
+    // (Black-Scholes); recompute variant: [ceil(n / kSeg)][64] Dupire states
+    const int ck_rows = ckpt_rows(n, DUP, RECOMP, int(sizeof(R)));
+    R* ck = AD ? static_cast<R*>(P.ckpt) + size_t(slot) * ck_rows * kWave : nullptr;
+    KnotSearch<R> ks;
+    ks.knots = knots;
+    ks.table = bucket;
+    ks.m = m;
+    ks.nb = P.n_buckets;
+    ks.fix = P.bucket_fix;
+    ks.x0 = R(P.bucket_x0);
+    ks.invh = R(P.bucket_invh);
+    const double s0 = P.spot;
+    const int last = ns - 1;
+
+    // per-lane parameter adjoints, summed over the lane's paths
+    double padj[kMaxParamAdj] = {0.0, 0.0, 0.0, 0.0};
+
+    // book tables in LDS
+    const unsigned char* bkb = smem + L.book;
+    const int* bk_van_first = reinterpret_cast<const int*>(bkb + P.bk.van_first);
+    const double* bk_van_strike = reinterpret_cast<const double*>(bkb + P.bk.van_strike);
+    const int* bk_van_col = reinterpret_cast<const int*>(bkb + P.bk.van_col);
+    const uint32_t* bk_bar_mask = reinterpret_cast<const uint32_t*>(bkb + P.bk.bar_mask);
+    const uint32_t* bk_bar_last = reinterpret_cast<const uint32_t*>(bkb + P.bk.bar_last);
+    const double* bk_bar_par = reinterpret_cast<const double*>(bkb + P.bk.bar_par);
+    const int* bk_bar_col = reinterpret_cast<const int*>(bkb + P.bk.bar_col);
+    const int* bk_bar_put = reinterpret_cast<const int*>(bkb + P.bk.bar_put);
+    const uint32_t* bk_grp_mask = reinterpret_cast<const uint32_t*>(bkb + P.bk.grp_mask);
+    const uint32_t* bk_grp_last = reinterpret_cast<const uint32_t*>(bkb + P.bk.grp_last);
+    const double* bk_grp_inv = reinterpret_cast<const double*>(bkb + P.bk.grp_inv);
+    const int* bk_grp_first = reinterpret_cast<const int*>(bkb + P.bk.grp_first);
+    const double* bk_as_strike = reinterpret_cast<const double*>(bkb + P.bk.as_strike);
+    const int* bk_as_col = reinterpret_cast<const int*>(bkb + P.bk.as_col);
+    const int* bk_as_put = reinterpret_cast<const int*>(bkb + P.bk.as_put);
+    constexpr int NBAR = BOOK ? kMaxBookBarriers : 1, NGRP = BOOK ? kMaxBookGroups : 1;
+    // barrier flags: number of exactly-zero smoothing factors (saturating at
+    // 2) and the breach bit
+    constexpr int kNzOne = 1, kNzMask = 3, kDead = 4;
+
+    for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= q1; q += int64_t(gridDim.x) * waves) {
+        // ---- high Sobol words of this block of 64 indices
+        {
+            const uint32_t gq = uint32_t(q ^ (q >> 1));
+            for (int d = lane; d < n; d += kWave) {
+                uint32_t x = (q & 1) ? P.dirs[5 * kSobolDims + d] : 0u;
+                uint32_t b = gq;
+                while (b) {
+                    const int j = __builtin_ctz(b);
+                    x ^= P.dirs[(j + 6) * kSobolDims + d];
+                    b &= b - 1u;
+                }
+                hi[d] = x;
+            }
+            wave_sync_lds();
+        }
+        const int64_t sidx = q * kWave + lane;
+        const bool active = sidx >= nlo && sidx <= nhi;
+        const int64_t pidx = sidx - nlo;
+
+        ProductState<R> ps;
+        ps.alive = R(1); ps.pnz = R(1); ps.nzero = 0; ps.dead = false;
+        ps.euro = R(0); ps.fin_f = R(0); ps.fin_num = R(1); ps.aggp = R(0);
+        // book: per barrier the product of the non-zero smoothing factors, the
+        // flags and (after its maturity) weight x vanilla, the adjoint of its
+        // survival factor; per fixing schedule the running sum, replaced at
+        // its last fixing by the adjoint of each fixing, and the weighted
+        // intrinsic value (numeraire adjoint)
+        R bpnz[NBAR], babar[NBAR], gsum[NGRP], gq[NGRP];
+        int bflag[NBAR];
+#pragma unroll
+        for (int b = 0; b < NBAR; ++b) { bpnz[b] = R(1); babar[b] = R(0); bflag[b] = 0; }
+#pragma unroll
+        for (int g = 0; g < NGRP; ++g) { gsum[g] = R(0); gq[g] = R(0); }
+
+        // payoff `col` of this lane's path: wave sums, per-path output
+        auto emit = [&](int col, R pay) {
+            const double v = active ? double(pay) : 0.0;
+            const double vs = wave_sum(v), vq = wave_sum(v * v);
+            if (lane == 0) { accw[col] += vs; accw[npay + col] += vq; }
+            if (P.per_path && active) P.per_path[pidx * npay + col] = v;
+            if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[col]) * pay;
+        };
+
+        // observation of product sample s with spot S (and for Black-Scholes
+        // the sample's forward factor / numeraire / discount)
+        auto observe = [&](int s, R S) {
+            R F = S, numv = R(1), discv = R(1);
+            if constexpr (!DUP) {
+                F = S * R(s_ff[s]);
+                numv = P.spot_measure ? R(s_num[s]) * S : R(s_num[s]);
+                discv = R(s_disc[s]);
+            }
+            if constexpr (PRD == PRD_BARRIER) {
+                // mcPrd.h:256-287
+                if (!ps.dead) {
+                    if (F > R(P.bar_smooth)) {
+                        ps.dead = true;
+                    } else if (F > R(P.minus_smooth)) {
+                        const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
+                        ps.alive = ps.alive * f;
+                        if (f == R(0)) ps.nzero++; else ps.pnz = ps.pnz * f;
+                    }
+                }
+                if (s == last) {
+                    const R k = R(P.strike);
+                    const R intrinsic = P.put ? (k - F > R(0) ? k - F : R(0)) : (F - k > R(0) ? F - k : R(0));
+                    ps.euro = intrinsic / numv;
+                    ps.fin_f = F;
+                    ps.fin_num = numv;
+                }
+            } else if constexpr (PRD == PRD_EUROPEAN) {
+                // mcPrd.h:113-125
+                const R k = R(P.strike);
+                const R pay = (F - k > R(0) ? F - k : R(0)) * discv / numv;
+                const double v = active ? double(pay) : 0.0;
+                const double vs = wave_sum(v), vq = wave_sum(v * v);
+                if (lane == 0) { accw[0] += vs; accw[npay] += vq; }
+                if (P.per_path && active) P.per_path[pidx * npay] = v;
+                if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[0]) * pay;
+            } else if constexpr (PRD == PRD_BOOK) {
+                // vanilla calls at this sample (European / Europeans parts)
+                for (int k = bk_van_first[s]; k < bk_van_first[s + 1]; ++k) {
+                    const R kk = R(bk_van_strike[k]);
+                    emit(bk_van_col[k], (F - kk > R(0) ? F - kk : R(0)) / numv);
+                }
+                // up-and-out barriers (mcPrd.h:256-287, one state per barrier)
+                const uint32_t bm = bk_bar_mask[s], bl = bk_bar_last[s];
+                if (bm | bl) {
+#pragma unroll
+                    for (int b = 0; b < NBAR; ++b) {
+                        if ((bm >> b) & 1u) {
+                            const double* bp = bk_bar_par + 4 * b;
+                            if (!(bflag[b] & kDead)) {
+                                if (F > R(bp[1])) {
+                                    bflag[b] |= kDead;
+                                } else if (F > R(bp[2])) {
+                                    const R f = (R(bp[1]) - F) * R(bp[3]);
+                                    if (f == R(0)) bflag[b] += (bflag[b] & kNzMask) < 2 ? kNzOne : 0;
+                                    else bpnz[b] = bpnz[b] * f;
+                                }
+                            }
+                        }
+                        if ((bl >> b) & 1u) {
+                            const R k = R(bk_bar_par[4 * b]);
+                            const R intrinsic = bk_bar_put[b] ? (k - F > R(0) ? k - F : R(0))
+                                                              : (F - k > R(0) ? F - k : R(0));
+                            const R euro = intrinsic / numv;
+                            const R alive = (bflag[b] & (kDead | kNzMask)) ? R(0) : bpnz[b];
+                            const int col = bk_bar_col[b];
+                            emit(col, alive * euro);
+                            emit(col + 1, euro);
+                            if constexpr (AD) babar[b] = R(P.weights[col]) * euro;
+                        }
+                    }
+                }
+                // arithmetic averages: running sums per fixing schedule
+                const uint32_t gm = bk_grp_mask[s], gl = bk_grp_last[s];
+                if (gm) {
+#pragma unroll
+                    for (int g = 0; g < NGRP; ++g) {
+                        if (!((gm >> g) & 1u)) continue;
+                        gsum[g] = gsum[g] + F;
+                        if ((gl >> g) & 1u) {
+                            const R inv_n = R(bk_grp_inv[g]);
+                            const R avg = gsum[g] * inv_n;
+                            R dsum = R(0), qsum = R(0);
+                            for (int a = bk_grp_first[g]; a < bk_grp_first[g + 1]; ++a) {
+                                const R k = R(bk_as_strike[a]);
+                                const bool put = bk_as_put[a] != 0;
+                                const R intrinsic = put ? (k - avg > R(0) ? k - avg : R(0))
+                                                        : (avg - k > R(0) ? avg - k : R(0));
+                                const int col = bk_as_col[a];
+                                emit(col, intrinsic / numv);
+                                if constexpr (AD) {
+                                    const R w = R(P.weights[col]);
+                                    const R d = put ? (k > avg ? R(-1) : R(0)) : (avg > k ? R(1) : R(0));
+                                    dsum = dsum + w * d;
+                                    qsum = qsum + w * intrinsic;
+                                }
+                            }
+                            if constexpr (AD) {
+                                gsum[g] = dsum * inv_n / numv;   // adjoint of each fixing
+                                gq[g] = qsum;
+                            }
+                        }
+                    }
+                }
+            } else {
+                // mcPrd.h:374-399
+                for (int k = pay_first[s]; k < pay_first[s + 1]; ++k) {
+                    const R kk = R(pay_strike[k]);
+                    const R pay = (F - kk > R(0) ? F - kk : R(0)) / numv;
+                    const double v = active ? double(pay) : 0.0;
+                    const double vs = wave_sum(v), vq = wave_sum(v * v);
+                    if (lane == 0) { accw[k] += vs; accw[npay + k] += vq; }
+                    if (P.per_path && active) P.per_path[pidx * npay + k] = v;
+                    if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[k]) * pay;
+                }
+            }
+        };
+
+        // ---- forward sweep
+        R x;
+        if constexpr (DUP) x = R(P.x0); else x = R(s0);   // Dupire: log-spot state; BS: spot
+        {
+            const int s = step_sample[0];
+            if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
+        }
+        for (int i = 0; i < n; ++i) {
+            const uint32_t u = hi[i] ^ low_word(i);
+            const R g = inv_normal(u, R());
+            if constexpr (DUP) {
+                if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = x;
+                if constexpr (RECOMP)
+                    if ((i & (kSeg - 1)) == 0) ck[(i / kSeg) * kWave + lane] = x;
+                const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, x);
+                x = euler_step(x, b.vol, g);
+            } else {
+                if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
+                x = x * rexp(drift[i] + stdv[i] * g);      // mcMdlBS.h:344-345
+            }
+            const int s = step_sample[i + 1];
+            if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
+        }
+
+        // ---- payoffs of the barrier (known at the end of the path)
+        if constexpr (PRD == PRD_BARRIER) {
+            const R p0 = ps.dead ? R(0) : ps.alive * ps.euro;
+            const R p1 = ps.euro;
+            const double v0 = active ? double(p0) : 0.0, v1 = active ? double(p1) : 0.0;
+            const double a0 = wave_sum(v0), a1 = wave_sum(v1), b0 = wave_sum(v0 * v0), b1 = wave_sum(v1 * v1);
+            if (lane == 0) { accw[0] += a0; accw[1] += a1; accw[npay] += b0; accw[npay + 1] += b1; }
+            if (P.per_path && active) { P.per_path[pidx * 2] = v0; P.per_path[pidx * 2 + 1] = v1; }
+            if constexpr (AD) ps.aggp = R(P.weights[0]) * p0 + R(P.weights[1]) * p1;
+        }
+
+        if constexpr (AD) {
+            const double agg = active ? double(ps.aggp) : 0.0;
+            const double as = wave_sum(agg), aq = wave_sum(agg * agg);
+            if (lane == 0) { accw[2 * npay] += as; accw[2 * npay + 1] += aq; }
+            if (P.per_path_agg && active) P.per_path_agg[pidx] = agg;
+
+            // adjoints of the forward, numeraire and discount at sample s
+            const R w0 = R(P.weights[0]);
+            const R w1 = npay > 1 ? R(P.weights[1]) : R(0);
+            const R alive_eff = ps.dead ? R(0) : ps.alive;
+            const R euro_bar = w0 * alive_eff + w1;       // barrier: adjoint of the vanilla
+            const R alive_bar = w0 * ps.euro;             // barrier: adjoint of alive
+            auto sample_adj = [&](int s, R F, R numv, R discv, R& Fb, R& nb, R& db) {
+                Fb = R(0); nb = R(0); db = R(0);
+                if constexpr (PRD == PRD_BARRIER) {
+                    if (!ps.dead && F > R(P.minus_smooth)) {
+                        const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
+                        R others;
+                        if (ps.nzero == 0) others = recip_mul(ps.alive, f);
+                        else others = (ps.nzero == 1 && f == R(0)) ? ps.pnz : R(0);
+                        Fb -= alive_bar * others * R(P.inv_two_smooth);
+                    }
+                    if (s == last) {
+                        const R k = R(P.strike);
+                        const R dmax = P.put ? (k > F ? R(-1) : R(0)) : (F > k ? R(1) : R(0));
+                        Fb += euro_bar * dmax / numv;
+                        nb += -euro_bar * ps.euro / numv;
+                    }
+                } else if constexpr (PRD == PRD_EUROPEAN) {
+                    const R k = R(P.strike);
+                    const R ind = F > k ? R(1) : R(0);
+                    const R intrinsic = F > k ? F - k : R(0);
+                    Fb = w0 * ind * discv / numv;
+                    db = w0 * intrinsic / numv;
+                    nb = -w0 * intrinsic * discv / (numv * numv);
+                } else if constexpr (PRD == PRD_BOOK) {
+                    for (int k = bk_van_first[s]; k < bk_van_first[s + 1]; ++k) {
+                        const R kk = R(bk_van_strike[k]);
+                        const R wk = R(P.weights[bk_van_col[k]]);
+                        if (F > kk) {
+                            Fb += wk / numv;
+                            nb += -wk * (F - kk) / (numv * numv);
+                        }
+                    }
+                    const uint32_t bm = bk_bar_mask[s], bl = bk_bar_last[s];
+                    if (bm | bl) {
+#pragma unroll
+                        for (int b = 0; b < NBAR; ++b) {
+                            const double* bp = bk_bar_par + 4 * b;
+                            if ((bl >> b) & 1u) {
+                                const int col = bk_bar_col[b];
+                                const R k = R(bp[0]);
+                                const bool put = bk_bar_put[b] != 0;
+                                const R intrinsic = put ? (k - F > R(0) ? k - F : R(0))
+                                                        : (F - k > R(0) ? F - k : R(0));
+                                const R dmax = put ? (k > F ? R(-1) : R(0)) : (F > k ? R(1) : R(0));
+                                const R alive = (bflag[b] & (kDead | kNzMask)) ? R(0) : bpnz[b];
+                                const R euro_bar = R(P.weights[col]) * alive + R(P.weights[col + 1]);
+                                Fb += euro_bar * dmax / numv;
+                                nb += -euro_bar * (intrinsic / numv) / numv;
+                            }
+                            if (((bm >> b) & 1u) && !(bflag[b] & kDead) && F > R(bp[2])) {
+                                const R f = (R(bp[1]) - F) * R(bp[3]);
+                                const int nz = bflag[b] & kNzMask;
+                                R others;
+                                if (nz == 0) others = recip_mul(bpnz[b], f);
+                                else others = (nz == 1 && f == R(0)) ? bpnz[b] : R(0);
+                                Fb -= babar[b] * others * R(bp[3]);
+                            }
+                        }
+                    }
+                    const uint32_t gm = bk_grp_mask[s], gl = bk_grp_last[s];
+                    if (gm) {
+#pragma unroll
+                        for (int g = 0; g < NGRP; ++g) {
+                            if (!((gm >> g) & 1u)) continue;
+                            Fb += gsum[g];
+                            if ((gl >> g) & 1u) nb += -gq[g] / (numv * numv);
+                        }
+                    }
+                } else {
+                    for (int k = pay_first[s]; k < pay_first[s + 1]; ++k) {
+                        const R kk = R(pay_strike[k]);
+                        const R wk = R(P.weights[k]);
+                        if (F > kk) {
+                            Fb += wk / numv;
+                            nb += -wk * (F - kk) / (numv * numv);
+                        }
+                    }
+                }
+            };
+
+            if constexpr (DUP && RECOMP) {
+                // ---- reverse sweep, recompute variant: segments of kSeg steps
+                // from the last one back; each segment is first re-simulated
+                // from its stored state (Sobol point, inverse normal, local
+                // vol), keeping in registers the step's local derivatives
+                //   abar_i = abar_{i+1} (1 + (g - vol) dvol/dx) + sample adjoint
+                //   vol adjoint = abar_{i+1} (g - vol), split on the two nodes
+                R abar = R(0);
+                {
+                    const int s = step_sample[n];
+                    if (s >= 0) {
+                        const R S = state_spot(x, s0);
+                        R Fb, nb, db;
+                        sample_adj(s, S, R(1), R(1), Fb, nb, db);
+                        abar = S * Fb;
+                    }
+                }
+                const int nseg = (n + kSeg - 1) / kSeg;
+                for (int sg = nseg - 1; sg >= 0; --sg) {
+                    const int i0 = sg * kSeg;
+                    R xi = ck[sg * kWave + lane];
+                    R ca[kSeg], cb[kSeg], cw[kSeg], cv[kSeg];
+                    int cj[kSeg];
+#pragma unroll
+                    for (int k = 0; k < kSeg; ++k) {
+                        const int i = i0 + k;
+                        ca[k] = R(1);
+                        cb[k] = R(0);
+                        cw[k] = R(0);
+                        cv[k] = R(0);
+                        cj[k] = 0;
+                        if (i >= n) continue;
+                        const uint32_t u = hi[i] ^ low_word(i);
+                        const R g = inv_normal(u, R());
+                        const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
+                        const R ws = g - b.vol;    // d x_{i+1} / d vol
+                        ca[k] = R(1) + ws * b.slope;
+                        cw[k] = ws * b.wa;
+                        cv[k] = ws * b.wb;
+                        cj[k] = b.j;
+                        const int smp = step_sample[i];
+                        if (smp >= 0) {
+                            const R S = state_spot(xi, s0);
+                            R Fb, nb, db;
+                            sample_adj(smp, S, R(1), R(1), Fb, nb, db);
+                            cb[k] = S * Fb;
+                        }
+                        xi = euler_step(xi, b.vol, g);
+                    }
+#pragma unroll
+                    for (int k = kSeg - 1; k >= 0; --k) {
+                        const int i = i0 + k;
+                        if (i >= n) continue;
+                        if (active) {
+                            double* row = ivbar + i * m;
+                            atomicAdd(row + cj[k], double(abar * cw[k]));
+                            if (m > 1) atomicAdd(row + cj[k] + 1, double(abar * cv[k]));
+                        }
+                        abar = abar * ca[k] + cb[k];
+                    }
+                }
+                if (active) padj[0] += double(abar) / s0;      // d log(S0) / d S0
+            } else if constexpr (DUP) {
+                // ---- reverse sweep over the Euler steps (mcMdlDupire.h:259-279)
+                R abar = R(0);   // adjoint of the log-spot state at the current point
+                {
+                    const int s = step_sample[n];
+                    if (s >= 0) {
+                        const R S = state_spot(x, s0);
+                        R Fb, nb, db;
+                        sample_adj(s, S, R(1), R(1), Fb, nb, db);
+                        abar = S * Fb;
+                    }
+                }
+                R xnext = x;     // state at point i + 1
+                // checkpoint loads run kPrefetch steps ahead of their use
+                constexpr int kPrefetch = sizeof(R) == 8 ? 1 : 4;   // FP64 holds twice the registers
+                R buf[kPrefetch];
+#pragma unroll
+                for (int k = 0; k < kPrefetch; ++k) buf[k] = n - 1 - k >= 0 ? ck[(n - 1 - k) * kWave + lane] : R(0);
+                for (int i0 = n - 1; i0 >= 0; i0 -= kPrefetch) {
+                    // phase 1, independent of the adjoint chain: brackets, the
+                    // derivative of each step to its vol, the sample adjoints
+                    R ws[kPrefetch], sl[kPrefetch], wa[kPrefetch], wb[kPrefetch], sf[kPrefetch];
+                    int js[kPrefetch];
+#pragma unroll
+                    for (int k = 0; k < kPrefetch; ++k) {
+                        const int i = i0 - k;
+                        ws[k] = R(0);
+                        sl[k] = R(0);
+                        wa[k] = R(0);
+                        wb[k] = R(0);
+                        sf[k] = R(0);
+                        js[k] = 0;
+                        if (i < 0) continue;
+                        const R xi = buf[k];
+                        buf[k] = i - kPrefetch >= 0 ? ck[(i - kPrefetch) * kWave + lane] : R(0);
+                        const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
+                        // d x_{i+1} / d vol = g - vol, recovered from the two states:
+                        // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
+                        if (b.vol != R(0)) {
+                            ws[k] = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
+                        } else {
+                            const uint32_t u = hi[i] ^ low_word(i);
+                            ws[k] = inv_normal(u, R());
+                        }
+                        sl[k] = b.slope;
+                        wa[k] = b.wa;
+                        wb[k] = b.wb;
+                        js[k] = b.j;
+                        const int smp = step_sample[i];
+                        if (smp >= 0) {
+                            const R S = state_spot(xi, s0);
+                            R Fb, nb, db;
+                            sample_adj(smp, S, R(1), R(1), Fb, nb, db);
+                            sf[k] = S * Fb;
+                        }
+                        xnext = xi;
+                    }
+                    // phase 2: the adjoint of the state, step by step backwards
+                    R vb[kPrefetch];
+#pragma unroll
+                    for (int k = 0; k < kPrefetch; ++k) {
+                        vb[k] = abar * ws[k];
+                        abar = abar + vb[k] * sl[k];
+                        abar = abar + sf[k];
+                    }
+                    // phase 3: scatter onto the two bracketing local-vol nodes
+                    if (active) {
+#pragma unroll
+                        for (int k = 0; k < kPrefetch; ++k) {
+                            const int i = i0 - k;
+                            if (i < 0) break;
+                            double* row = ivbar + i * m;
+                            atomicAdd(row + js[k], double(vb[k] * wa[k]));
+                            if (m > 1) atomicAdd(row + js[k] + 1, double(vb[k] * wb[k]));
+                        }
+                    }
+                }
+                if (active) padj[0] += double(abar) / s0;      // d log(S0) / d S0
+            } else {
+                // ---- Black-Scholes: forward tangents of the spot with respect
+                // to (spot, vol, rate, div), replaying the checkpointed normals
+                // (mcMdlBS.h:203-225, :240-275, :288-349)
+                const double* tl = P.bs_aux;     // sample times
+                const double* tf = tl + ns;      // forward maturity - sample time
+                const double* td = tf + ns;      // discount maturity - sample time
+                const double* dt = td + ns;      // step lengths
+                const double vol = P.bs_vol;
+                R spot = R(s0);
+                R tau[4] = {R(1.0 / s0), R(0), R(0), R(0)};   // d spot / spot
+                auto accumulate = [&](int s, R S) {
+                    const R ff = R(s_ff[s]);
+                    const R F = S * ff;
+                    const R numv = P.spot_measure ? R(s_num[s]) * S : R(s_num[s]);
+                    const R discv = R(s_disc[s]);
+                    R Fb, nb, db;
+                    sample_adj(s, F, numv, discv, Fb, nb, db);
+                    // dF = F (tau + dff/ff), ff = exp(mu (Tf - t))
+                    const R tfs = R(tf[s]), tds = R(td[s]), ts = R(tl[s]);
+                    R dF[4] = {F * tau[0], F * tau[1], F * (tau[2] + tfs), F * (tau[3] - tfs)};
+                    R dD[4] = {R(0), R(0), -tds * discv, R(0)};
+                    R dN[4];
+                    if (P.spot_measure) {
+                        // num = exp(div t) / S0 * spot
+                        dN[0] = numv * (tau[0] - R(1.0 / s0));
+                        dN[1] = numv * tau[1];
+                        dN[2] = numv * tau[2];
+                        dN[3] = numv * (tau[3] + ts);
+                    } else {
+                        dN[0] = R(0); dN[1] = R(0); dN[2] = numv * ts; dN[3] = R(0);
+                    }
+                    if (active)
+                        for (int c = 0; c < 4; ++c) padj[c] += double(Fb * dF[c] + db * dD[c] + nb * dN[c]);
+                };
+                {
+                    const int s = step_sample[0];
+                    if (s >= 0) accumulate(s, spot);
+                }
+                for (int i = 0; i < n; ++i) {
+                    R g;
+                    if constexpr (RECOMP) g = inv_normal(hi[i] ^ low_word(i), R());   // regenerated
+                    else g = ck[i * kWave + lane];
+                    spot = spot * rexp(drift[i] + stdv[i] * g);
+                    const R dti = R(dt[i]);
+                    const R sq = R(sqrt(dt[i]));
+                    tau[1] = tau[1] + (P.spot_measure ? R(vol) * dti : -R(vol) * dti) + sq * g;
+                    tau[2] = tau[2] + dti;
+                    tau[3] = tau[3] - dti;
+                    const int s = step_sample[i + 1];
+                    if (s >= 0) accumulate(s, spot);
+                }
+            }
+        }
+    }
+
+    // ---- parameter adjoints: wave, then per-wave accumulators
+    if constexpr (AD) {
+        for (int c = 0; c < kMaxParamAdj; ++c) {
+            const double v = wave_sum(padj[c]);
+            if (lane == 0) accw[2 * npay + 2 + c] += v;
+        }
+    }
+    __syncthreads();
+    for (int c = tid; c < stride; c += nthr) {
+        double s = 0.0;
+        for (int w = 0; w < waves; ++w) s += acc[w * stride + c];
+        P.block_out[size_t(blockIdx.x) * stride + c] = s;
+    }
+    if constexpr (AD && DUP) {
+        double* dst = P.ivbar_blocks + size_t(blockIdx.x) * n * m;
+        for (int e = tid; e < n * m; e += nthr) dst[e] = ivbar[e];
+    }
+}
+
+}  // namespace cfk

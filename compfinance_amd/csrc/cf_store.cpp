@@ -126,6 +126,25 @@ int cf_put_europeans(const double* maturities, const double* strikes, int n, con
     });
 }
 
+int cf_put_asian(double strike, double maturity, double fixing_freq, double snap_freq, int call_put, const char* id)
+{
+    return store_call([&] {
+        cfamd::putAsian(strike, maturity, fixing_freq, snap_freq, call_put != 0, id_of(id));
+        return CF_OK;
+    });
+}
+
+int cf_put_book(const char* const* part_ids, int n, const char* id)
+{
+    return store_call([&] {
+        if (n < 1 || !part_ids) throw std::runtime_error("empty book");
+        std::vector<std::string> ids;
+        for (int i = 0; i < n; ++i) ids.push_back(id_of(part_ids[i]));
+        cfamd::putBook(ids, id_of(id));
+        return CF_OK;
+    });
+}
+
 int cf_value(const char* model_id, const char* product_id, const cf_num* num, double* values, int cap)
 {
     return store_call([&] {

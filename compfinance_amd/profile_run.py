@@ -4,7 +4,8 @@
     rocprofv3 --kernel-trace --stats -f csv -d OUT -- python3 compfinance_amd/profile_run.py --workload all
 
 Workloads: c3_fp32_aad, c3_fp64_aad (Dupire barrier, 30x36 grid, 2^20 paths,
-156 steps, price + 1,081 risks), c2_fp32, c2_fp64 (pricing, 2^19 paths),
+156 steps, price + 1,081 risks, checkpoint adjoint), c3_fp32_rec, c3_fp64_rec
+(the same with the recompute adjoint), c2_fp32, c2_fp64 (pricing, 2^19 paths),
 copy (a 1 GiB device-to-device streaming copy, the byte-counter calibration),
 all (every one of them).
 """
@@ -27,8 +28,9 @@ def run(name, calls):
             cf._check(cf.lib().cf_diag_copy(COPY_BYTES, 1))
         elif name.startswith("c3"):
             prec = cf.FP32 if "fp32" in name else cf.FP64
-            cf.mcParallelSimulAAD(prd, mdl, weights=[1.0, 0.0], num=cf.NumericalParam(numPath=W.C3_PATHS,
-                                                                                       precision=prec))
+            mode = cf.AAD_RECOMPUTE if name.endswith("_rec") else cf.AAD_CHECKPOINT
+            cf.mcParallelSimulAAD(prd, mdl, weights=[1.0, 0.0],
+                                  num=cf.NumericalParam(numPath=W.C3_PATHS, precision=prec, aadMode=mode))
         elif name.startswith("c2"):
             prec = cf.FP32 if "fp32" in name else cf.FP64
             cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=W.C2_PATHS, precision=prec), per_path=False)
@@ -43,7 +45,8 @@ def main():
     ap.add_argument("--workload", default="all")
     ap.add_argument("--calls", type=int, default=3)
     a = ap.parse_args()
-    names = ["c3_fp32_aad", "c3_fp64_aad", "c2_fp32", "c2_fp64", "copy"] if a.workload == "all" else [a.workload]
+    names = (["c3_fp32_aad", "c3_fp64_aad", "c3_fp32_rec", "c3_fp64_rec", "c2_fp32", "c2_fp64", "copy"]
+             if a.workload == "all" else [a.workload])
     for n in names:
         run(n, a.calls)
 

@@ -4,8 +4,13 @@ C1  Black-Scholes European call, S0=100, sigma=0.2, r=q=0, K=100, T=1, 2^16 path
 C2  Dupire local vol, up-and-out call K=120, B=150, 5% smoothing, 3y weekly
     (156 steps), 2^19 paths
 C3  as C2 with price, delta and every local-vol sensitivity by AAD, 2^20 paths
-C4  book of Europeans, barriers and Asians on shared paths (next)
-C5  serving mix (next)
+C4  book on shared paths, Dupire 30x36 grid, 3y weekly (156 steps), 2^20 paths:
+    30 European calls K = 70, 72.5, .., 142.5; 10 up-and-out calls K evenly
+    100..130 with barriers 140, 145, .., 185 (5% smoothing, weekly); 5 Asian
+    calls K = 90, 95, .., 110 on monthly fixings moved to the nearest weekly
+    date (keeps 156 steps); notionals uniform [0.5, 2] (numpy PCG64, seed
+    20261203); aggregated risk to 1,081 parameters
+C5  serving mix (see serving.py)
 
 Local-vol grid: spots_i = 50 + 150 i / (nS - 1), times_j = (j + 1) / 12,
 j = 0..35, sigma(S, t) = 0.15 + 0.10 ln(S/100)^2 + 0.05 exp(-t).  The headline
@@ -43,9 +48,25 @@ def barrier_args(strike=120.0, barrier=150.0, maturity=3.0, freq=WEEK, smooth=0.
     return dict(strike=strike, barrier=barrier, maturity=maturity, monitorFreq=freq, smooth=smooth, callPut=put)
 
 
+def c4_parts():
+    """C4 book parts as (kind, keyword arguments), kind in european(s),
+    barrier, asian; the keywords fit both the engine classes and the oracle."""
+    parts = [("europeans", dict(maturities=[3.0] * 30, strikes=[70.0 + 2.5 * i for i in range(30)]))]
+    for i in range(10):
+        parts.append(("barrier", barrier_args(strike=100.0 + 30.0 * i / 9, barrier=140.0 + 5.0 * i)))
+    for k in (90.0, 95.0, 100.0, 105.0, 110.0):
+        parts.append(("asian", dict(strike=k, maturity=3.0, fixingFreq=1.0 / 12, snapFreq=WEEK, callPut=False)))
+    return parts
+
+
+def c4_notionals(n_pay, seed=20261203):
+    return np.random.default_rng(seed).uniform(0.5, 2.0, n_pay)
+
+
 C1_PATHS = 1 << 16
 C2_PATHS = 1 << 19
 C3_PATHS = 1 << 20
+C4_PATHS = 1 << 20
 
 # Black-Scholes closed form of C1 (analytics.h:8-20 with r = q = 0)
 C1_CLOSED_FORM = 7.965567455405804

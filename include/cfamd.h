@@ -66,7 +66,11 @@ typedef struct cf_model {
 enum cf_product_kind {
     CF_PRODUCT_EUROPEAN = 0,   /* European<T>  mcPrd.h:29-126  */
     CF_PRODUCT_BARRIER = 1,    /* UOC<T>       mcPrd.h:128-289 */
-    CF_PRODUCT_EUROPEANS = 2   /* Europeans<T> mcPrd.h:291-400 */
+    CF_PRODUCT_EUROPEANS = 2,  /* Europeans<T> mcPrd.h:291-400 */
+    CF_PRODUCT_ASIAN = 3,      /* arithmetic-average option; no reference class,
+                                  written to the Product<T> interface mcBase.h:122-152 */
+    CF_PRODUCT_BOOK = 4        /* several products on the same paths, payoffs
+                                  concatenated; no reference class (Product<T>) */
 };
 
 /* Product parameters as the reference store functions take them
@@ -84,6 +88,16 @@ typedef struct cf_product {
     int n_options;           /* Europeans: (maturity, strike) pairs */
     const double* maturities;
     const double* strikes;
+    /* Asian: strike, maturity, call_put and fixings at fixing_freq, 2 fixing_freq, ...
+     * up to the maturity (accumulated as the barrier's monitoring dates, the last
+     * one at maturity), each moved to the nearest date of the grid 0, snap_freq,
+     * 2 snap_freq, ..., maturity when snap_freq > 0 (ties to the earlier date). */
+    double fixing_freq;
+    double snap_freq;
+    /* Book: the parts (European with settlement = exercise, barrier, Europeans,
+     * Asian); at most 16 barriers and 4 distinct fixing schedules. */
+    int n_parts;
+    const struct cf_product* parts;
 } cf_product;
 
 enum cf_precision { CF_FP64 = 0, CF_FP32 = 1 };
@@ -157,6 +171,12 @@ int cf_put_barrier(double strike, double barrier, double maturity, double monito
 /* store.h:184-205 */
 int cf_put_europeans(const double* maturities, const double* strikes, int n, const char* id);
 
+/* Arithmetic-average option (CF_PRODUCT_ASIAN; no reference counterpart). */
+int cf_put_asian(double strike, double maturity, double fixing_freq, double snap_freq, int call_put, const char* id);
+/* Book of stored products on shared paths (CF_PRODUCT_BOOK; no reference
+ * counterpart); part_ids name products already in the store. */
+int cf_put_book(const char* const* part_ids, int n, const char* id);
+
 /* main.h:80-96: payoff values in label order; returns n_pay, writes at most cap. */
 int cf_value(const char* model_id, const char* product_id, const cf_num* num, double* values, int cap);
 
@@ -180,6 +200,119 @@ int cf_dupire_aad_risk(const char* model_id, const char* product_id, const char*
 
 /* Remove every stored model and product. */
 void cf_store_clear(void);
+
+/* ------------------------------------------------------------ local-vol calibration (FP64, host) */
+
+/* Merton jump-diffusion implied-vol surface (MertonIVS, ivs.h:150-179;
+ * merton, blackScholesIvol analytics.h:23-91). */
+typedef struct cf_merton_ivs {
+    double spot;
+    double vol;
+    double jmp_intens;
+    double jmp_average;
+    double jmp_std;
+} cf_merton_ivs;
+
+/* dupireCalib (main.h:414-438 -> mcMdlDupire.h:337-388): spots = fillData(incl
+ * spots, max_ds, 0.01), times = fillData(incl times, max_dt, ONE_HOUR, {max_dt}),
+ * local vols by Dupire's formula on the surface's call prices (ivs.h:119-138),
+ * flat outside 2.5 ATM standard deviations.  Writes *n_spots, *n_times and,
+ * when n_spots x n_times <= cap, spots, times and lvols (spot major). */
+int cf_dupire_calib(const cf_merton_ivs* ivs, const double* incl_spots, int n_incl_spots, double max_ds,
+                    const double* incl_times, int n_incl_times, double max_dt, double* spots, double* times,
+                    double* lvols, int* n_spots, int* n_times, int cap);
+
+/* The chain step of dupireSuperbucket (main.h:536-566): risks to the risk
+ * view of implied-vol spreads on strikes x mats (vega, strike major) from the
+ * local-vol risks `microbucket` (n_spots x n_times of the calibration above,
+ * spot major), through the calibration's derivative. */
+int cf_superbucket_vega(const cf_merton_ivs* ivs, const double* incl_spots, int n_incl_spots, double max_ds,
+                        const double* incl_times, int n_incl_times, double max_dt_vol, const double* strikes,
+                        int n_strikes, const double* mats, int n_mats, const double* microbucket, int n_micro,
+                        double* vega);
+
+/* dupireSuperbucket (main.h:453-569): calibrates, stores the model as
+ * "superbucket", runs dupireAADRisk on the GPU for the stored product and
+ * notionals, and chains the local-vol risks through the calibration to the
+ * risk view of implied-vol spreads on strikes x mats.  Writes value, delta and
+ * vega[n_strikes * n_mats] (strike major). */
+int cf_dupire_superbucket(const cf_merton_ivs* ivs, double max_dt, const char* product_id,
+                          const char* const* labels, const double* notionals, int n,
+                          const double* incl_spots, int n_incl_spots, double max_ds,
+                          const double* incl_times, int n_incl_times, double max_dt_vol,
+                          const double* strikes, int n_strikes, const double* mats, int n_mats,
+                          const cf_num* num, double* value, double* delta, double* vega);
+
+/* ------------------------------------------------------------ request server */
+
+/* Continuous batching of pricing and risk requests on one GPU; no counterpart
+ * in the reference (its entry points are synchronous, AAD.cpp:22-26).  Each
+ * request is one mcParallelSimul (mcBase.h:314-400) or, with aad,
+ * mcParallelSimulAAD (mcBase.h:564-753) call with a linear aggregator.
+ * Requests are admitted in arrival order while one of `slots` HIP streams is
+ * free and the request's device block (tables, checkpoints, partial sums)
+ * fits in the server's device arena of hbm_budget bytes, allocated once at
+ * creation; a checkpoint AAD request that does not fit runs the recompute
+ * variant when that fits and auto_recompute is set.  Requests arriving while
+ * others run start at once and their kernels overlap on the device. */
+typedef struct cf_server cf_server;
+
+typedef struct cf_server_opts {
+    int slots;               /* requests on the device at once, 1..16 */
+    int64_t hbm_budget;      /* device bytes of the arena */
+    int auto_recompute;      /* 1: switch AAD to the recompute variant to fit */
+} cf_server_opts;
+
+typedef struct cf_request {
+    cf_model model;          /* copied at submission, arrays included */
+    cf_product product;
+    cf_num num;
+    int aad;                 /* 0: prices; 1: prices, aggregate and risks */
+    const double* weights;   /* aad: aggregation weights [n_pay]; NULL: first payoff */
+} cf_request;
+
+typedef struct cf_result {
+    /* in: caller buffers and their capacities (nullable) */
+    double* means;
+    double* std_errs;
+    int cap_pay;
+    double* risks;
+    int cap_risk;
+    /* out */
+    int status;              /* CF_OK or the request's error code */
+    char error[256];
+    int n_pay, n_param;
+    double agg_mean;
+    double submit_ms;        /* submission time since the server's creation */
+    double queue_ms;         /* submission to launch */
+    double latency_ms;       /* submission to results on the host */
+    double kernel_ms;        /* path kernel (HIP events) */
+    int64_t device_bytes;    /* arena bytes held while on the device */
+    int recomputed;          /* 1 if admitted as the recompute variant */
+} cf_result;
+
+typedef struct cf_server_stats {
+    int64_t submitted, completed, failed;
+    int64_t queued, running;
+    int64_t peak_running;         /* most requests on the device at once */
+    int64_t admitted_busy;        /* requests started while others ran */
+    int64_t memory_waits;         /* admission passes that found no room */
+    int64_t recompute_switches;   /* AAD requests switched to recompute */
+    int64_t arena_bytes, peak_arena_bytes;
+} cf_server_stats;
+
+/* opts NULL: 4 slots, 16 GiB, auto_recompute on.  Uses the current device. */
+int cf_server_create(const cf_server_opts* opts, cf_server** out);
+/* Validates and plans the request on the calling thread, then queues it. */
+int cf_server_submit(cf_server* srv, const cf_request* req, int64_t* id);
+/* 1 when the request's results are ready, 0 when not, negative on error. */
+int cf_server_poll(cf_server* srv, int64_t id);
+/* Blocks until the request is done, writes its results and releases it;
+ * returns its status. */
+int cf_server_wait(cf_server* srv, int64_t id, cf_result* out);
+int cf_server_get_stats(cf_server* srv, cf_server_stats* out);
+/* Finishes every queued request, then releases the server. */
+int cf_server_destroy(cf_server* srv);
 
 /* ------------------------------------------------------------ runtime */
 

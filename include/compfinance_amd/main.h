@@ -193,4 +193,64 @@ inline DupireRisk dupireAADRisk(const string& modelId, const string& productId,
     return results;
 }
 
+// main.h:414-438: local vols on a Merton implied-vol surface (FP64, host)
+struct CalibResults {
+    vector<double> spots;
+    vector<Time> times;
+    matrix<double> lVols;
+};
+
+inline CalibResults dupireCalib(const vector<double>& inclSpots, const double maxDs, const vector<Time>& inclTimes,
+                                const double maxDt, const double spot, const double vol, const double jmpIntens = 0.0,
+                                const double jmpAverage = 0.0, const double jmpStd = 0.0)
+{
+    const cf_merton_ivs ivs{spot, vol, jmpIntens, jmpAverage, jmpStd};
+    int nS = 0, nT = 0;
+    cfCheck(cf_dupire_calib(&ivs, inclSpots.data(), int(inclSpots.size()), maxDs, inclTimes.data(),
+                            int(inclTimes.size()), maxDt, nullptr, nullptr, nullptr, &nS, &nT, 0));
+    CalibResults r;
+    r.spots.resize(nS);
+    r.times.resize(nT);
+    r.lVols.resize(nS, nT);
+    cfCheck(cf_dupire_calib(&ivs, inclSpots.data(), int(inclSpots.size()), maxDs, inclTimes.data(),
+                            int(inclTimes.size()), maxDt, r.spots.data(), r.times.data(), &*r.lVols.begin(), &nS,
+                            &nT, nS * nT));
+    return r;
+}
+
+struct SuperbucketResults {
+    double value = 0.0;
+    double delta = 0.0;
+    vector<double> strikes;
+    vector<Time> mats;
+    matrix<double> vega;
+};
+
+// main.h:453-569: calibration, local-vol risks on the GPU (dupireAADRisk),
+// then the chain to the implied-vol risk view
+inline SuperbucketResults dupireSuperbucket(const double spot, const double maxDt, const string& productId,
+                                            const std::map<string, double>& notionals,
+                                            const vector<double>& inclSpots, const double maxDs,
+                                            const vector<Time>& inclTimes, const double maxDtVol,
+                                            const vector<double>& strikes, const vector<Time>& mats,
+                                            const double vol, const double jmpIntens, const double jmpAverage,
+                                            const double jmpStd, const NumericalParam& num)
+{
+    auto cal = dupireCalib(inclSpots, maxDs, inclTimes, maxDtVol, spot, vol, jmpIntens, jmpAverage, jmpStd);
+    putDupire(spot, cal.spots, cal.times, cal.lVols, maxDt, "superbucket");
+    auto micro = dupireAADRisk("superbucket", productId, notionals, num);
+    SuperbucketResults r;
+    r.value = micro.value;
+    r.delta = micro.delta;
+    r.strikes = strikes;
+    r.mats = mats;
+    r.vega.resize(strikes.size(), mats.size());
+    const cf_merton_ivs ivs{spot, vol, jmpIntens, jmpAverage, jmpStd};
+    cfCheck(cf_superbucket_vega(&ivs, inclSpots.data(), int(inclSpots.size()), maxDs, inclTimes.data(),
+                                int(inclTimes.size()), maxDtVol, strikes.data(), int(strikes.size()), mats.data(),
+                                int(mats.size()), &*micro.vega.begin(), int(micro.vega.rows() * micro.vega.cols()),
+                                &*r.vega.begin()));
+    return r;
+}
+
 }  // namespace cfamd

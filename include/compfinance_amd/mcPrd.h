@@ -2,7 +2,9 @@
 // (mcPrd.h:29-126), UOC<T> (mcPrd.h:128-289) and Europeans<T>
 // (mcPrd.h:291-400).  Constructors, timeline(), defline() and payoffLabels()
 // keep the reference's signatures and results; the payoffs are evaluated on
-// the device, fused into the path kernel.
+// the device, fused into the path kernel.  Two products without a reference
+// class, written to the same Product<T> interface (mcBase.h:122-152): the
+// arithmetic-average Asian<T> and Book<T>, several products on shared paths.
 #pragma once
 
 #include <map>
@@ -146,6 +148,79 @@ public:
         d.n_options = int(myFlatMats.size());
         d.maturities = myFlatMats.data();
         d.strikes = myFlatStrikes.data();
+        return d;
+    }
+};
+
+// Arithmetic-average call or put on the spot: fixings at fixingFreq,
+// 2 fixingFreq, ... up to the maturity, each moved to the nearest date of the
+// grid 0, snapFreq, ..., maturity when snapFreq > 0; paid at maturity.
+template <class T>
+class Asian : public PrdBase<T> {
+    double myStrike;
+    Time myMaturity, myFixingFreq, mySnapFreq;
+    bool myCallPut;
+
+public:
+    Asian(const double strike, const Time maturity, const Time fixingFreq, const Time snapFreq = 0.0,
+          const bool callPut = false)
+        : myStrike(strike), myMaturity(maturity), myFixingFreq(fixingFreq), mySnapFreq(snapFreq), myCallPut(callPut)
+    {
+        this->loadTimeline();
+        for (size_t i = 0; i < this->myTimeline.size(); ++i) {
+            SampleDef s;
+            s.numeraire = i + 1 == this->myTimeline.size();
+            s.forwardMats.push_back({this->myTimeline[i]});
+            this->myDefline.push_back(s);
+        }
+    }
+    unique_ptr<Product<T>> clone() const override { return std::make_unique<Asian<T>>(*this); }
+    cf_product descriptor() const override
+    {
+        cf_product d{};
+        d.kind = CF_PRODUCT_ASIAN;
+        d.strike = myStrike;
+        d.maturity = myMaturity;
+        d.fixing_freq = myFixingFreq;
+        d.snap_freq = mySnapFreq;
+        d.call_put = myCallPut ? 1 : 0;
+        return d;
+    }
+};
+
+// Several products simulated on the same paths; payoffs concatenated in part
+// order and labelled "<part index>:<part label>".  Parts: European settled at
+// exercise, UOC, Europeans, Asian; at most 16 barriers and 4 fixing schedules.
+template <class T>
+class Book : public PrdBase<T> {
+    vector<unique_ptr<Product<T>>> myParts;
+    mutable vector<cf_product> myPartDesc;
+
+public:
+    explicit Book(const vector<const Product<T>*>& parts)
+    {
+        for (const Product<T>* p : parts) myParts.push_back(p->clone());
+        this->loadTimeline();
+        for (size_t i = 0; i < this->myTimeline.size(); ++i) {
+            SampleDef s;
+            s.numeraire = true;
+            s.forwardMats.push_back({this->myTimeline[i]});
+            this->myDefline.push_back(s);
+        }
+    }
+    Book(const Book& o) : PrdBase<T>(o)
+    {
+        for (const auto& p : o.myParts) myParts.push_back(p->clone());
+    }
+    unique_ptr<Product<T>> clone() const override { return std::make_unique<Book<T>>(*this); }
+    cf_product descriptor() const override
+    {
+        myPartDesc.clear();
+        for (const auto& p : myParts) myPartDesc.push_back(p->descriptor());
+        cf_product d{};
+        d.kind = CF_PRODUCT_BOOK;
+        d.n_parts = int(myPartDesc.size());
+        d.parts = myPartDesc.data();
         return d;
     }
 };

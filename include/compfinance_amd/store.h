@@ -71,6 +71,34 @@ inline void putEuropeans(const vector<Time>& maturities, const vector<double>& s
     productStore[store] = std::make_pair(std::move(prd), std::move(riskPrd));
 }
 
+// Arithmetic-average option (no reference counterpart; see Asian<T>)
+inline void putAsian(const double strike, const Time maturity, const Time fixingFreq, const Time snapFreq,
+                     const bool callPut, const string& store)
+{
+    auto prd = std::make_unique<Asian<double>>(strike, maturity, fixingFreq, snapFreq, callPut);
+    auto riskPrd = std::make_unique<Asian<Number>>(strike, maturity, fixingFreq, snapFreq, callPut);
+    std::lock_guard<std::recursive_mutex> lk(storeMutex);
+    productStore[store] = std::make_pair(std::move(prd), std::move(riskPrd));
+}
+
+// Book of stored products on shared paths (no reference counterpart; see
+// Book<T>); throws if a part is not in the store.
+inline void putBook(const vector<string>& parts, const string& store)
+{
+    std::lock_guard<std::recursive_mutex> lk(storeMutex);
+    vector<const Product<double>*> pd;
+    vector<const Product<Number>*> pn;
+    for (const string& id : parts) {
+        auto it = productStore.find(id);
+        if (it == productStore.end()) throw std::runtime_error("Could not retrieve product " + id);
+        pd.push_back(it->second.first.get());
+        pn.push_back(it->second.second.get());
+    }
+    auto prd = std::make_unique<Book<double>>(pd);
+    auto riskPrd = std::make_unique<Book<Number>>(pn);
+    productStore[store] = std::make_pair(std::move(prd), std::move(riskPrd));
+}
+
 template <class T>
 const Model<T>* getModel(const string& store);
 

@@ -297,14 +297,17 @@ typedef struct {
     double max_dt;
 } or_model;
 
-typedef struct {
-    int kind;                 // 0 = European, 1 = up-and-out barrier, 2 = Europeans
+typedef struct or_product {
+    int kind;                 // 0 = European, 1 = up-and-out barrier, 2 = Europeans, 3 = Asian, 4 = book
     double strike, exercise, settlement;
     double barrier, maturity, monitor_freq, smooth;
-    int call_put;             // 0 = call, 1 = put (barrier only)
+    int call_put;             // 0 = call, 1 = put (barrier, Asian)
     int n_options;
     const double* maturities;
     const double* strikes;
+    double fixing_freq, snap_freq;   // Asian
+    int n_parts;                     // book
+    const struct or_product* parts;
 } or_product;
 
 }  // extern "C"
@@ -324,7 +327,49 @@ struct ProductDef {
     // Europeans (maturity major)
     vector<vector<double>> strikes;
     int nPay = 0;
+    // Asian: the timeline holds the fixing dates
+    // Book: the parts and, for each, the positions of its dates on the book's
+    // timeline
+    vector<ProductDef> parts;
+    vector<vector<size_t>> at;
 };
+
+// Dates freq, 2 freq, ... accumulated as the barrier's monitoring dates
+// (mcPrd.h:162-193), the maturity closing the list.
+static vector<double> periodicDates(double maturity, double freq)
+{
+    vector<double> out;
+    double t = 0.0 + freq;
+    while (maturity - t > kOneHour) { out.push_back(t); t += freq; }
+    out.push_back(maturity);
+    return out;
+}
+
+// Asian fixing dates: periodic, each moved to the nearest date of the grid
+// {0} + periodicDates(maturity, snap) when snap > 0, ties to the earlier one;
+// from the last fixing back, one that would not fall before the next fixing
+// takes the grid date preceding that fixing.
+static vector<double> asianFixings(double maturity, double freq, double snap)
+{
+    vector<double> fix = periodicDates(maturity, freq);
+    if (snap > 0.0) {
+        vector<double> grid = periodicDates(maturity, snap);
+        grid.insert(grid.begin(), 0.0);
+        for (size_t i = fix.size(); i-- > 0;) {
+            size_t best = 0;
+            for (size_t k = 1; k < grid.size(); ++k)
+                if (std::fabs(grid[k] - fix[i]) < std::fabs(grid[best] - fix[i])) best = k;
+            if (i + 1 < fix.size() && !(grid[best] < fix[i + 1])) {
+                best = 0;
+                for (size_t k = 1; k < grid.size(); ++k)
+                    if (grid[k] < fix[i + 1]) best = k;
+            }
+            fix[i] = grid[best];
+            if (fix[i] <= 0.0) throw std::runtime_error("Asian fixings closer than the snapping grid");
+        }
+    }
+    return fix;
+}
 
 static ProductDef makeProduct(const or_product& p)
 {
@@ -371,6 +416,41 @@ static ProductDef makeProduct(const or_product& p)
             d.defline.push_back(s);
             d.nPay += int(kv.second.size());
         }
+    } else if (p.kind == 3) {
+        d.strike = p.strike;
+        d.put = p.call_put != 0;
+        d.timeline = asianFixings(p.maturity, p.fixing_freq, p.snap_freq);
+        for (size_t i = 0; i < d.timeline.size(); ++i) {
+            SampleDef s;
+            s.numeraire = i + 1 == d.timeline.size();
+            s.forwardMats = {d.timeline[i]};
+            d.defline.push_back(s);
+        }
+        d.nPay = 1;
+    } else if (p.kind == 4) {
+        vector<double> all;
+        for (int i = 0; i < p.n_parts; ++i) {
+            if (p.parts[i].kind == 4) throw std::runtime_error("nested book");
+            d.parts.push_back(makeProduct(p.parts[i]));
+            if (p.parts[i].kind == 0 && p.parts[i].settlement > 0 && p.parts[i].settlement != p.parts[i].exercise)
+                throw std::runtime_error("book European settled after exercise");
+            all.insert(all.end(), d.parts.back().timeline.begin(), d.parts.back().timeline.end());
+            d.nPay += d.parts.back().nPay;
+        }
+        std::sort(all.begin(), all.end());
+        all.erase(std::unique(all.begin(), all.end()), all.end());
+        d.timeline = all;
+        for (double t : all) {
+            SampleDef s;
+            s.numeraire = true;
+            s.forwardMats = {t};
+            d.defline.push_back(s);
+        }
+        for (const auto& q : d.parts) {
+            vector<size_t> pos;
+            for (double t : q.timeline) pos.push_back(size_t(std::lower_bound(all.begin(), all.end(), t) - all.begin()));
+            d.at.push_back(pos);
+        }
     } else {
         throw std::runtime_error("unknown product kind");
     }
@@ -399,11 +479,34 @@ static void payoffs(const ProductDef& d, const vector<Sample<T>>& path, vector<T
                        : max(fin - d.strike, 0.0) / path.back().numeraire;
         out[0] = alive * euro;
         out[1] = euro;
-    } else {
+    } else if (d.kind == 2) {
         // mcPrd.h:374-399
         size_t k = 0;
         for (size_t i = 0; i < d.timeline.size(); ++i)
             for (double K : d.strikes[i]) out[k++] = max(path[i].forwards[0] - K, 0.0) / path[i].numeraire;
+    } else if (d.kind == 3) {
+        T sum(0.0);
+        for (const auto& s : path) sum = sum + s.forwards[0];
+        const T avg = sum / double(path.size());
+        out[0] = (d.put ? max(d.strike - avg, 0.0) : max(avg - d.strike, 0.0)) / path.back().numeraire;
+    } else {
+        // each part on its own dates of the shared path
+        size_t col = 0;
+        for (size_t q = 0; q < d.parts.size(); ++q) {
+            const ProductDef& part = d.parts[q];
+            vector<Sample<T>> sub;
+            for (size_t j = 0; j < d.at[q].size(); ++j) {
+                const Sample<T>& src = path[d.at[q][j]];
+                Sample<T> smp;
+                smp.numeraire = part.defline[j].numeraire ? src.numeraire : T(1.0);
+                smp.forwards = {src.forwards[0]};
+                smp.discounts.assign(part.defline[j].discountMats.size(), T(1.0));
+                sub.push_back(smp);
+            }
+            vector<T> po(part.nPay);
+            payoffs(part, sub, po);
+            for (int k = 0; k < part.nPay; ++k) out[col++] = po[k];
+        }
     }
 }
 

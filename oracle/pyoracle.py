@@ -29,12 +29,17 @@ class or_model(ctypes.Structure):
 
 
 class or_product(ctypes.Structure):
-    _fields_ = [
-        ("kind", ctypes.c_int), ("strike", ctypes.c_double), ("exercise", ctypes.c_double),
-        ("settlement", ctypes.c_double), ("barrier", ctypes.c_double), ("maturity", ctypes.c_double),
-        ("monitor_freq", ctypes.c_double), ("smooth", ctypes.c_double), ("call_put", ctypes.c_int),
-        ("n_options", ctypes.c_int), ("maturities", _dp), ("strikes", _dp),
-    ]
+    pass
+
+
+or_product._fields_ = [
+    ("kind", ctypes.c_int), ("strike", ctypes.c_double), ("exercise", ctypes.c_double),
+    ("settlement", ctypes.c_double), ("barrier", ctypes.c_double), ("maturity", ctypes.c_double),
+    ("monitor_freq", ctypes.c_double), ("smooth", ctypes.c_double), ("call_put", ctypes.c_int),
+    ("n_options", ctypes.c_int), ("maturities", _dp), ("strikes", _dp),
+    ("fixing_freq", ctypes.c_double), ("snap_freq", ctypes.c_double),
+    ("n_parts", ctypes.c_int), ("parts", ctypes.POINTER(or_product)),
+]
 
 
 _LIB = None
@@ -103,12 +108,24 @@ class Model:
 class Product:
     """European: Product(strike=, exerciseDate=, settlementDate=);
     barrier: Product(strike=, barrier=, maturity=, monitorFreq=, smooth=, callPut=);
-    Europeans: Product(maturities=, strikes=)."""
+    Europeans: Product(maturities=, strikes=);
+    Asian: Product(strike=, maturity=, fixingFreq=, snapFreq=, callPut=);
+    book: Product(parts=[dict, ...])."""
 
     def __init__(self, **kw):
         self.kw = kw
         d = or_product()
-        if "barrier" in kw:
+        if "parts" in kw:
+            d.kind = 4
+            self._parts = [Product(**q) for q in kw["parts"]]
+            self._arr = (or_product * len(self._parts))(*[q.d for q in self._parts])
+            d.n_parts, d.parts = len(self._parts), ctypes.cast(self._arr, ctypes.POINTER(or_product))
+        elif "fixingFreq" in kw:
+            d.kind = 3
+            d.strike, d.maturity = kw["strike"], kw["maturity"]
+            d.fixing_freq, d.snap_freq = kw["fixingFreq"], kw.get("snapFreq", 0.0)
+            d.call_put = int(bool(kw.get("callPut", False)))
+        elif "barrier" in kw:
             d.kind = 1
             d.strike, d.barrier, d.maturity = kw["strike"], kw["barrier"], kw["maturity"]
             d.monitor_freq, d.smooth = kw["monitorFreq"], kw["smooth"]

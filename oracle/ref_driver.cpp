@@ -11,6 +11,12 @@
  *   filldata <maxDx> <minDx> <n> <x...> <nAdd> <a...>   (args on stdin)
  *   value <nPath> <nOut> <threads> <repeats>            spec on stdin
  *   aad <nPath> <riskPayoff|-1> <threads> <repeats> <nOut>   spec on stdin
+ *   calib                            dupireCalib (main.h:414-438), on stdin:
+ *        <spot> <vol> <jmpIntens> <jmpAverage> <jmpStd> <nS> <spots> <maxDs> <nT> <times> <maxDt>
+ *   superbucket <nPath>              dupireSuperbucket (main.h:453-569), on stdin:
+ *        <spot> <maxDt> <nS> <spots> <maxDs> <nT> <times> <maxDtVol>
+ *        <nK> <strikes> <nM> <mats> <vol> <jmpIntens> <jmpAverage> <jmpStd>
+ *        then a product spec and notionals as for aad
  * The spec is whitespace separated:
  *   model bs <spot> <vol> <qspot> <rate> <div>
  *   model dupire <spot> <maxDt> <nS> <spots> <nT> <times> <vols, spot major>
@@ -255,5 +261,60 @@ int main(int argc, char** argv)
         if (threads > 1) ThreadPool::getInstance()->stop();
         return 0;
     }
+    if (cmd == "calib") {
+        double spot, vol, ji, ja, js, maxDs, maxDt;
+        size_t nS, nT;
+        std::cin >> spot >> vol >> ji >> ja >> js >> nS;
+        auto spots = readVec(std::cin, nS);
+        std::cin >> maxDs >> nT;
+        auto times = readVec(std::cin, nT);
+        std::cin >> maxDt;
+        auto res = dupireCalib(spots, maxDs, times, maxDt, spot, vol, ji, ja, js);
+        std::vector<double> lv;
+        for (size_t i = 0; i < res.lVols.rows(); ++i)
+            for (size_t j = 0; j < res.lVols.cols(); ++j) lv.push_back(res.lVols[i][j]);
+        std::printf("{");
+        printVec("spots", res.spots);
+        printVec("times", res.times);
+        printVec("lvols", lv, false);
+        std::printf("}\n");
+        return 0;
+    }
+
+    if (cmd == "superbucket") {
+        const int nPath = std::atoi(argv[2]);
+        double spot, maxDt, maxDs, maxDtVol, vol, ji, ja, js;
+        size_t nS, nT, nK, nM;
+        std::cin >> spot >> maxDt >> nS;
+        auto spots = readVec(std::cin, nS);
+        std::cin >> maxDs >> nT;
+        auto times = readVec(std::cin, nT);
+        std::cin >> maxDtVol >> nK;
+        auto strikes = readVec(std::cin, nK);
+        std::cin >> nM;
+        auto mats = readVec(std::cin, nM);
+        std::cin >> vol >> ji >> ja >> js;
+        Spec spec = readSpec();
+        NumericalParam num;
+        num.parallel = false;
+        num.useSobol = true;
+        num.numPath = nPath;
+        std::map<std::string, double> nots(spec.notionals.begin(), spec.notionals.end());
+        auto res = dupireSuperbucket(spot, maxDt, "p", nots, spots, maxDs, times, maxDtVol, strikes, mats, vol, ji,
+                                     ja, js, num);
+        std::vector<double> vega;
+        for (size_t i = 0; i < res.vega.rows(); ++i)
+            for (size_t j = 0; j < res.vega.cols(); ++j) vega.push_back(res.vega[i][j]);
+        // the microbucket of the calibrated model, stored by the superbucket
+        // as "superbucket" (main.h:519-534), recomputed on the same paths
+        auto micro = dupireAADRisk("superbucket", "p", nots, num);
+        std::vector<double> mb(micro.vega.begin(), micro.vega.end());
+        std::printf("{\"value\": %.17g, \"delta\": %.17g,\n", res.value, res.delta);
+        printVec("microbucket", mb);
+        printVec("vega", vega, false);
+        std::printf("}\n");
+        return 0;
+    }
+
     fail("unknown command " + cmd);
 }

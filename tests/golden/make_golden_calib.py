@@ -1,0 +1,77 @@
+"""Writes tests/golden/calib.json from the reference build: local-vol
+calibrations (dupireCalib, main.h:414-438) on Merton surfaces and
+superbuckets (dupireSuperbucket, main.h:453-569) of two products.
+
+    python tests/golden/make_golden_calib.py
+"""
+import json
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, ROOT)
+
+import pyoracle as po  # noqa: E402
+from compfinance_amd import workloads as W  # noqa: E402
+
+
+def fl(xs):
+    return " ".join(repr(float(x)) for x in xs)
+
+
+CALIBS = [
+    dict(name="black_scholes_flat", ivs=[100.0, 0.2, 0.0, 0.0, 0.0], incl_spots=[50.0, 100.0, 150.0], max_ds=5.0,
+         incl_times=[0.5, 1.0], max_dt=0.25),
+    dict(name="merton_skew", ivs=[100.0, 0.15, 0.05, -0.15, 0.1], incl_spots=[50.0, 100.0, 200.0], max_ds=5.0,
+         incl_times=[0.25, 1.0, 3.0], max_dt=0.1),
+    dict(name="merton_wide", ivs=[100.0, 0.25, 0.2, -0.1, 0.15], incl_spots=[20.0, 100.0, 300.0], max_ds=10.0,
+         incl_times=[0.1, 2.0, 5.0], max_dt=0.5),
+]
+
+SUPERBUCKETS = [
+    dict(name="barrier", product=W.barrier_args(), notionals={0: 1.0}, n_path=4096),
+    dict(name="europeans", product=dict(maturities=[1.0, 2.0, 3.0, 3.0], strikes=[95.0, 100.0, 110.0, 120.0]),
+         notionals={0: 1.0, 2: -0.5, 3: 2.0}, n_path=4096),
+]
+SB_GRID = dict(ivs=[100.0, 0.15, 0.05, -0.15, 0.1], max_dt=1 / 52, incl_spots=[50.0, 100.0, 200.0], max_ds=10.0,
+               incl_times=[0.25, 1.0, 3.0], max_dt_vol=0.25, strikes=[80.0, 100.0, 120.0, 140.0],
+               mats=[0.5, 1.0, 2.0, 3.0])
+
+
+def main():
+    drv = po.REF_DRIVER
+    if not os.path.exists(drv):
+        raise SystemExit("reference build missing: run oracle/build_ref.sh")
+    out = dict(calibs=[], superbuckets=[], grid=SB_GRID)
+    for c in CALIBS:
+        s, v, ji, ja, js = c["ivs"]
+        stdin = "%r %r %r %r %r %d %s %r %d %s %r\n" % (s, v, ji, ja, js, len(c["incl_spots"]), fl(c["incl_spots"]),
+                                                      c["max_ds"], len(c["incl_times"]), fl(c["incl_times"]),
+                                                      c["max_dt"])
+        res = json.loads(subprocess.run([drv, "calib"], input=stdin, capture_output=True, text=True,
+                                        check=True).stdout)
+        out["calibs"].append(dict(c, **res))
+    g = SB_GRID
+    for sb in SUPERBUCKETS:
+        prd = po.Product(**sb["product"])
+        s, v, ji, ja, js = g["ivs"]
+        stdin = "%r %r %d %s %r %d %s %r %d %s %d %s %r %r %r %r\n" % (
+            s, g["max_dt"], len(g["incl_spots"]), fl(g["incl_spots"]), g["max_ds"], len(g["incl_times"]),
+            fl(g["incl_times"]),
+            g["max_dt_vol"], len(g["strikes"]), fl(g["strikes"]), len(g["mats"]), fl(g["mats"]), v, ji, ja, js)
+        stdin += prd.spec() + "\n"
+        stdin += "notionals %d %s\n" % (len(sb["notionals"]),
+                                        " ".join("%d %r" % (i, w) for i, w in sb["notionals"].items()))
+        res = json.loads(subprocess.run([drv, "superbucket", str(sb["n_path"])], input=stdin, capture_output=True,
+                                        text=True, check=True).stdout)
+        out["superbuckets"].append(dict(sb, notionals={str(k): w for k, w in sb["notionals"].items()}, **res))
+    with open(os.path.join(HERE, "calib.json"), "w") as f:
+        json.dump(out, f)
+    print("written", os.path.join(HERE, "calib.json"))
+
+
+if __name__ == "__main__":
+    main()

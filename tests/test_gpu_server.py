@@ -1,0 +1,93 @@
+"""The request server: results in the mix equal to the same requests run
+alone, overlap of requests on the device, admission by device footprint
+(waits, switch to the recompute variant), errors at submission."""
+import numpy as np
+import pytest
+
+import compfinance_amd as cf
+from compfinance_amd import serving
+from compfinance_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(r):
+    if r["aad"]:
+        a = cf.mcParallelSimulAAD(r["product"], r["model"], num=r["num"])
+        return a.payoff_means, a.risks, a.agg_mean
+    a = cf.mcParallelSimul(r["product"], r["model"], num=r["num"], per_path=False)
+    return a.means, None, None
+
+
+def test_results_in_the_mix_equal_results_alone(engine):
+    """24 uneven requests submitted at once to 4 slots: prices bit for bit,
+    aggregates and risks to the rounding of the LDS-atomic sums (1e-12)."""
+    reqs = serving.c5_requests(24, rate=1e9, seed=7)
+    with cf.Server(slots=4, hbmBudget=8 << 30) as srv:
+        ids = [srv.submit(r["product"], r["model"], r["num"], aad=r["aad"]) for r in reqs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["completed"] == 24 and st["failed"] == 0
+    assert st["peak_running"] >= 2 and st["admitted_busy"] >= 1     # requests overlapped
+    assert st["recompute_switches"] == 0
+    for r, got in zip(reqs, res):
+        means, risks, agg = _alone(r)
+        np.testing.assert_array_equal(got["means"], means)
+        if r["aad"]:
+            assert abs(got["agg_mean"] - agg) <= 1e-12 * max(abs(agg), 1e-300)
+            np.testing.assert_allclose(got["risks"], risks, rtol=1e-12, atol=1e-12 * np.abs(risks).max())
+
+
+def test_admission_waits_for_memory_and_switches_to_recompute(engine):
+    """A budget of one checkpoint block plus one recompute block: the second
+    AAD request runs the recompute variant, the next ones wait for memory;
+    each gives the results of its variant alone."""
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    num = cf.NumericalParam(numPath=1 << 20, precision=cf.FP32)
+    ck = cf.aad_footprint(prd, mdl, num)
+    rc = cf.aad_footprint(prd, mdl, cf.NumericalParam(numPath=1 << 20, precision=cf.FP32,
+                                                      aadMode=cf.AAD_RECOMPUTE))
+    budget = ck + rc + (1 << 20)
+    with cf.Server(slots=4, hbmBudget=budget) as srv:
+        ids = [srv.submit(prd, mdl, num, aad=True) for _ in range(6)]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["completed"] == 6 and st["peak_arena_bytes"] <= budget
+    assert st["memory_waits"] > 0 and st["recompute_switches"] > 0
+    ref_ck = cf.mcParallelSimulAAD(prd, mdl, num=num)
+    ref_rc = cf.mcParallelSimulAAD(prd, mdl, num=cf.NumericalParam(numPath=1 << 20, precision=cf.FP32,
+                                                                   aadMode=cf.AAD_RECOMPUTE))
+    for g in res:
+        ref = ref_rc if g["recomputed"] else ref_ck
+        np.testing.assert_array_equal(g["means"], ref.payoff_means)
+        np.testing.assert_allclose(g["risks"], ref.risks, rtol=1e-12, atol=1e-12 * np.abs(ref.risks).max())
+
+
+def test_book_request_and_errors(engine):
+    mdl = cf.Dupire(**W.dupire_args(30))
+    book = cf.Book([cf.Europeans([3.0] * 3, [90.0, 100.0, 110.0]), cf.UOC(**W.barrier_args()),
+                    cf.Asian(100.0, 3.0, 1 / 12, 1 / 52)])
+    num = cf.NumericalParam(numPath=1 << 16, precision=cf.FP32)
+    w = np.array([1.0, 0.5, 2.0, 1.5, 0.0, 0.7])
+    with cf.Server(slots=2, hbmBudget=64 << 20) as srv:
+        rid = srv.submit(book, mdl, num, aad=True, weights=w)
+        with pytest.raises(cf.CfError):
+            srv.submit(cf.European(100.0, -1.0), mdl, num)            # invalid product
+        with pytest.raises(cf.CfError):                               # beyond the arena
+            srv.submit(cf.UOC(**W.barrier_args(maturity=5.0)), mdl, cf.NumericalParam(numPath=1 << 20),
+                       aad=True)
+        got = srv.wait(rid)
+    ref = cf.mcParallelSimulAAD(book, mdl, weights=w, num=num)
+    np.testing.assert_array_equal(got["means"], ref.payoff_means)
+    np.testing.assert_allclose(got["risks"], ref.risks, rtol=1e-12, atol=1e-12 * np.abs(ref.risks).max())
+
+
+def test_open_loop_replay_small(engine):
+    """A short C5-style replay: every request completes, latencies positive."""
+    reqs = serving.c5_requests(40, rate=400.0, seed=11)
+    with cf.Server(slots=4, hbmBudget=4 << 30) as srv:
+        res, wall = serving.replay(reqs, srv)
+        st = srv.stats()
+    s = serving.summarize(reqs, res, wall, st)
+    assert s["requests"] == 40 and st["failed"] == 0
+    assert 0 < s["p50_ms"] <= s["p99_ms"]
