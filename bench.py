@@ -205,7 +205,7 @@ def main():
                        "delta_fp32": float(r32.risks[0]), "delta_fp64": float(r64.risks[0])},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[cf.FP32], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS[cf.FP32], "traffic": traffic,
-                     "kernel": "path_kernel<float, Dupire, barrier, adjoint>", "kernel_ms": kavg,
+                     "kernel": "path_kernel<float, Dupire, barrier, checkpoint adjoint>", "kernel_ms": kavg,
                      "flop_per_path_step": FLOP_PER_PATH_STEP[("aad", cf.FP32)], "path_steps": path_steps},
     }
     if not args.no_serving:

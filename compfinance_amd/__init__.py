@@ -24,7 +24,8 @@ __all__ = [
     "BlackScholes", "Dupire", "European", "UOC", "Europeans", "Asian", "Book", "NumericalParam",
     "mcParallelSimul", "mcParallelSimulAAD", "SimulResults", "AADSimulResults",
     "putBlackScholes", "putDupire", "putEuropean", "putBarrier", "putEuropeans", "putAsian", "putBook",
-    "value", "AADriskOne", "AADriskAggregate", "dupireAADRisk", "clearStore",
+    "value", "AADriskOne", "AADriskAggregate", "AADriskMulti", "dupireAADRisk", "clearStore",
+    "mcParallelSimulAADMulti",
     "CfError", "lib", "last_kernel_stats", "LIB_PATH", "Server", "dupireCalib", "dupireSuperbucket",
 ]
 
@@ -783,3 +784,36 @@ def dupireSuperbucket(spot, maxDt, productId, notionals, inclSpots, maxDs, inclT
                                    _dptr(vega)))
     return OrderedDict(value=float(val[0]), delta=float(delta[0]), strikes=k, mats=m,
                        vega=vega.reshape(len(k), len(m)))
+
+
+# ---------------------------------------------------------------- itemized risk
+
+
+def mcParallelSimulAADMulti(prd, mdl, nPath=None, num=None, precision=None):
+    """mcParallelSimulAADMulti (mcBase.h:858-989): (payoff means, risks
+    [nParam][nPay]) with the risk of each payoff separately, same paths."""
+    L = lib()
+    L.cf_mc_simul_aad_multi.argtypes = [ctypes.POINTER(cf_model), ctypes.POINTER(cf_product), ctypes.POINTER(cf_num),
+                                        _c_double_p, _c_double_p]
+    dm, dp = mdl.descriptor(), prd.descriptor()
+    n = _num(num, nPath, precision)
+    npay = _check(L.cf_num_payoffs(ctypes.byref(dp)))
+    npar = _check(L.cf_num_params(ctypes.byref(dm)))
+    means, risks = np.zeros(npay), np.zeros((npar, npay))
+    _check(L.cf_mc_simul_aad_multi(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n), _dptr(means), _dptr(risks)))
+    return means, risks
+
+
+def AADriskMulti(modelId, productId, num):
+    """main.h:269-312: {payoffs, params, values, risks [nParam][nPay]}."""
+    L = lib()
+    L.cf_aad_risk_multi.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(cf_num), _c_double_p,
+                                    ctypes.c_int, _c_double_p, ctypes.c_int]
+    npay, npar = _caps(productId, modelId)
+    vals, risks = np.zeros(npay), np.zeros(npay * npar)
+    n = num.descriptor()
+    got = _check(L.cf_aad_risk_multi(_id(modelId), _id(productId), ctypes.byref(n), _dptr(vals), npay, _dptr(risks),
+                                      len(risks)))
+    m = _model_store.get(modelId)
+    return OrderedDict(payoffs=_labels(productId), params=m.parameterLabels() if m is not None else None,
+                       values=vals, risks=risks[:got * npay].reshape(got, npay))

@@ -1256,6 +1256,33 @@ int cf_mc_simul_aad(const cf_model* mdl, const cf_product* prd, const cf_num* nu
     });
 }
 
+int cf_mc_simul_aad_multi(const cf_model* mdl, const cf_product* prd, const cf_num* num, double* payoff_means,
+                          double* risks)
+{
+    return guarded([&] {
+        if (!risks) fail(CF_ERR_ARG, "null risks");
+        const int npay = plan_product(prd).n_pay;
+        validate_model(mdl);
+        const int npar = mdl->kind == CF_MODEL_BLACK_SCHOLES ? 4 : 1 + mdl->n_spots * mdl->n_times;
+        // one adjoint sweep per payoff over the same paths (one-hot weights)
+        vector<double> w(size_t(npay), 0.0), col(static_cast<size_t>(npar), 0.0), agg(1, 0.0);
+        for (int k = 0; k < npay; ++k) {
+            std::fill(w.begin(), w.end(), 0.0);
+            w[k] = 1.0;
+            Call c;
+            c.adjoint = true;
+            c.num = num;
+            c.weights = w.data();
+            c.means = k == 0 ? payoff_means : nullptr;
+            c.agg_mean = agg.data();
+            c.risks = col.data();
+            run(mdl, prd, c);
+            for (int i = 0; i < npar; ++i) risks[size_t(i) * npay + k] = col[i];
+        }
+        return npar;
+    });
+}
+
 int64_t cf_aad_footprint(const cf_model* mdl, const cf_product* prd, const cf_num* num)
 {
     int64_t bytes = 0;

@@ -241,6 +241,7 @@ struct Call {
 
 void validate_num(const cf_num* num);
 void check_finite(double x, const char* what);
+void validate_model(const cf_model* m);
 // utility.h:11-79
 vector<double> fill_data(const vector<double>& original, double maxDx, double minDx, const vector<double>& add);
 ProductPlan plan_product(const cf_product* p);

@@ -166,6 +166,17 @@ int cf_aad_risk_one(const char* model_id, const char* product_id, const cf_num* 
     });
 }
 
+int cf_aad_risk_multi(const char* model_id, const char* product_id, const cf_num* num, double* payoff_values,
+                      int cap_pay, double* risks, int cap_risk)
+{
+    return store_call([&] {
+        auto r = cfamd::AADriskMulti(id_of(model_id), id_of(product_id), to_num(num));
+        copy_out(r.values, payoff_values, cap_pay);
+        copy_out(std::vector<double>(r.risks.begin(), r.risks.end()), risks, cap_risk);
+        return int(r.params.size());
+    });
+}
+
 int cf_aad_risk_aggregate(const char* model_id, const char* product_id, const char* const* labels,
                           const double* notionals, int n, const cf_num* num, double* payoff_values, int cap_pay,
                           double* risk_value, double* risks, int cap_risk)

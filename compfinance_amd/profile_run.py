@@ -5,7 +5,9 @@
 
 Workloads: c3_fp32_aad, c3_fp64_aad (Dupire barrier, 30x36 grid, 2^20 paths,
 156 steps, price + 1,081 risks, checkpoint adjoint), c3_fp32_rec, c3_fp64_rec
-(the same with the recompute adjoint), c2_fp32, c2_fp64 (pricing, 2^19 paths),
+(the same with the recompute adjoint), c4_fp32, c4_fp64 (the book of 30
+Europeans, 10 barriers and 5 Asians, aggregated risk, 2^20 paths), c2_fp32,
+c2_fp64 (pricing, 2^19 paths),
 copy (a 1 GiB device-to-device streaming copy, the byte-counter calibration),
 all (every one of them).
 """
@@ -31,6 +33,12 @@ def run(name, calls):
             mode = cf.AAD_RECOMPUTE if name.endswith("_rec") else cf.AAD_CHECKPOINT
             cf.mcParallelSimulAAD(prd, mdl, weights=[1.0, 0.0],
                                   num=cf.NumericalParam(numPath=W.C3_PATHS, precision=prec, aadMode=mode))
+        elif name.startswith("c4"):
+            prec = cf.FP32 if "fp32" in name else cf.FP64
+            cls = {"europeans": cf.Europeans, "barrier": cf.UOC, "asian": cf.Asian}
+            book = cf.Book([cls[k](**kw) for k, kw in W.c4_parts()])
+            w = W.c4_notionals(len(book.payoffLabels()))
+            cf.mcParallelSimulAAD(book, mdl, weights=w, num=cf.NumericalParam(numPath=W.C4_PATHS, precision=prec))
         elif name.startswith("c2"):
             prec = cf.FP32 if "fp32" in name else cf.FP64
             cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=W.C2_PATHS, precision=prec), per_path=False)
@@ -45,7 +53,8 @@ def main():
     ap.add_argument("--workload", default="all")
     ap.add_argument("--calls", type=int, default=3)
     a = ap.parse_args()
-    names = (["c3_fp32_aad", "c3_fp64_aad", "c3_fp32_rec", "c3_fp64_rec", "c2_fp32", "c2_fp64", "copy"]
+    names = (["c3_fp32_aad", "c3_fp64_aad", "c3_fp32_rec", "c3_fp64_rec", "c4_fp32", "c4_fp64", "c2_fp32", "c2_fp64",
+              "copy"]
              if a.workload == "all" else [a.workload])
     for n in names:
         run(n, a.calls)

@@ -152,6 +152,12 @@ int cf_mc_simul_aad(const cf_model* mdl, const cf_product* prd, const cf_num* nu
                     const double* weights, double* payoff_means, double* agg_mean,
                     double* risks, double* per_path_payoffs, double* per_path_agg);
 
+/* mcParallelSimulAADMulti (mcBase.h:858-989): the risk of every payoff
+ * separately, risks[n_param][n_pay] = d E[payoff_k] / d parameter_i, on the
+ * same paths; one adjoint sweep per payoff.  Returns n_param. */
+int cf_mc_simul_aad_multi(const cf_model* mdl, const cf_product* prd, const cf_num* num,
+                          double* payoff_means, double* risks);
+
 /* Device memory one AAD call on this product would take (checkpoint buffers
  * dominate); used for admission.  Bytes, or negative on error. */
 int64_t cf_aad_footprint(const cf_model* mdl, const cf_product* prd, const cf_num* num);
@@ -192,6 +198,11 @@ int cf_aad_risk_aggregate(const char* model_id, const char* product_id, const ch
                           const double* notionals, int n, const cf_num* num,
                           double* payoff_values, int cap_pay, double* risk_value,
                           double* risks, int cap_risk);
+
+/* main.h:269-312 (AADriskMulti): payoff values and risks[n_param][n_pay],
+ * writes at most cap_pay values and cap_risk risks.  Returns n_param. */
+int cf_aad_risk_multi(const char* model_id, const char* product_id, const cf_num* num,
+                      double* payoff_values, int cap_pay, double* risks, int cap_risk);
 
 /* main.h:364-411: value, delta and the n_spots x n_times vega matrix. */
 int cf_dupire_aad_risk(const char* model_id, const char* product_id, const char* const* labels,

@@ -170,6 +170,27 @@ inline RiskReports bumpRisk(const string& modelId, const string& productId, cons
     return results;
 }
 
+// main.h:269-312: itemized AAD risk, payoffs in columns, parameters in rows
+inline RiskReports AADriskMulti(const string& modelId, const string& productId, const NumericalParam& num)
+{
+    const Model<Number>* model = getModel<Number>(modelId);
+    const Product<Number>* product = getProduct<Number>(productId);
+    if (!model || !product) throw std::runtime_error("AADrisk() : Could not retrieve model and product");
+    if (!checkCompatiblity(*product, *model)) throw std::runtime_error("Model and product are not compatible");
+    OptionsScope scope(num);
+    auto rng = makeRng(num);
+    const cf_model m = model->descriptor();
+    const cf_product p = product->descriptor();
+    const cf_num n = makeNum(*rng, size_t(num.numPath));
+    RiskReports results;
+    results.payoffs = product->payoffLabels();
+    results.params = model->parameterLabels();
+    results.values.resize(results.payoffs.size());
+    results.risks.resize(results.params.size(), results.payoffs.size());
+    cfCheck(cf_mc_simul_aad_multi(&m, &p, &n, results.values.data(), &*results.risks.begin()));
+    return results;
+}
+
 struct DupireRisk {
     double value = 0.0;
     double delta = 0.0;

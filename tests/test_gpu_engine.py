@@ -131,3 +131,34 @@ def test_repeat_calls_are_deterministic(engine):
     # local-vol adjoints are summed with LDS atomics inside a block: equal to
     # rounding, not bit for bit
     np.testing.assert_allclose(a.risks, b.risks, rtol=1e-12, atol=1e-18)
+
+
+@pytest.mark.parametrize("mname,pk", [("lv30", W.barrier_args()),
+                                      ("bs", dict(maturities=[1.0, 2.0, 2.0], strikes=[95.0, 100.0, 110.0]))])
+def test_itemized_risks_match_oracle_per_payoff(engine, mname, pk):
+    """mcParallelSimulAADMulti / AADriskMulti: column k is the risk of payoff k,
+    equal to the oracle tape's risk of that payoff alone."""
+    mk = W.dupire_args(30) if mname == "lv30" else dict(spot=100.0, vol=0.25, rate=0.03, div=0.01)
+    mdl = cf.Dupire(**mk) if "spots" in mk else cf.BlackScholes(**mk)
+    prd = cf.UOC(**pk) if "barrier" in pk else cf.Europeans(**pk)
+    om, op = po.Model(**mk), po.Product(**pk)
+    n = 1000
+    means, risks = cf.mcParallelSimulAADMulti(prd, mdl, n)
+    assert risks.shape == (mdl.numParams(), len(prd.payoffLabels()))
+    for k in range(risks.shape[1]):
+        _, _, ref = po.aad(om, op, n, k)
+        assert np.abs(risks[:, k] - ref).max() <= 1e-10 * max(np.abs(ref).max(), 1e-300)
+    np.testing.assert_allclose(means, po.value_paths(om, op, n).mean(0), rtol=1e-12, atol=1e-14)
+    cf.clearStore()
+    if "spots" in mk:
+        cf.putDupire(**mk, store="m")
+    else:
+        cf.putBlackScholes(mk["spot"], mk["vol"], False, mk["rate"], mk["div"], "m")
+    if "barrier" in pk:
+        cf.putBarrier(**pk, store="p")
+    else:
+        cf.putEuropeans(pk["maturities"], pk["strikes"], "p")
+    rep = cf.AADriskMulti("m", "p", cf.NumericalParam(numPath=n))
+    np.testing.assert_allclose(rep["risks"], risks, rtol=1e-12, atol=1e-12 * np.abs(risks).max())
+    assert rep["payoffs"] == prd.payoffLabels()
+    cf.clearStore()
