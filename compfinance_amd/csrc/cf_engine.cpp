@@ -53,6 +53,16 @@ int block_threads()
     return v;
 }
 
+// CF_IVBAR_COPIES=1 keeps one local-vol adjoint grid per block (tuning).
+bool ivbar_copies_allowed()
+{
+    static const bool v = [] {
+        const char* e = std::getenv("CF_IVBAR_COPIES");
+        return !(e && std::string(e) == "1");
+    }();
+    return v;
+}
+
 int forced_layout()   // -1 automatic, 0 FULL, 1 LEAN
 {
     static const int v = [] {
@@ -462,11 +472,15 @@ BookTables build_book(const ProductPlan& d, double width0)
     L.bar_mask = put(bar_mask.data(), bar_mask.size() * 4);
     L.bar_last = put(bar_last.data(), bar_last.size() * 4);
     L.bar_par = put(bar_par.data(), bar_par.size() * 8);
+    const vector<float> bar_par_f(bar_par.begin(), bar_par.end());
+    L.bar_par_f = put(bar_par_f.data(), bar_par_f.size() * 4);
     L.bar_col = put(bar_col.data(), bar_col.size() * 4);
     L.bar_put = put(bar_put.data(), bar_put.size() * 4);
     L.grp_mask = put(grp_mask.data(), grp_mask.size() * 4);
     L.grp_last = put(grp_last.data(), grp_last.size() * 4);
     L.grp_inv = put(grp_inv.data(), grp_inv.size() * 8);
+    const vector<float> grp_inv_f(grp_inv.begin(), grp_inv.end());
+    L.grp_inv_f = put(grp_inv_f.data(), grp_inv_f.size() * 4);
     L.grp_first = put(grp_first.data(), grp_first.size() * 4);
     L.as_strike = put(as_strike.data(), as_strike.size() * 8);
     L.as_col = put(as_col.data(), as_col.size() * 4);
@@ -696,7 +710,6 @@ Engine& engine()
 void validate_num(const cf_num* num)
 {
     if (!num) fail(CF_ERR_ARG, "null numerical parameters");
-    if (!num->use_sobol) fail(CF_ERR_UNSUPPORTED, "only the Sobol generator is implemented on the GPU engine");
     if (num->num_path < 1) fail(CF_ERR_ARG, "numPath must be positive");
     if (num->first_path < 0) fail(CF_ERR_ARG, "first path must be non-negative");
     if (int64_t(num->first_path) + num->num_path >= (int64_t(1) << 32) - 1)
@@ -756,6 +769,41 @@ string plan_key(const cf_model* m, const cf_product* p, int precision, int adj)
     return k;
 }
 
+// mrg32k3a jump matrices (mrg32k3a.h:264-381): for j = 0..31 the two
+// component matrices raised to dim * 2^j, modulo m1 and m2, [j][component][9].
+vector<uint64_t> mrg_jumps(int dim)
+{
+    using u128 = unsigned __int128;
+    const uint64_t m[2] = {4294967087ull, 4294944443ull};
+    const uint64_t base[2][9] = {{0, 1403580ull, 4294967087ull - 810728ull, 1, 0, 0, 0, 1, 0},
+                                 {527612ull, 0, 4294944443ull - 1370589ull, 1, 0, 0, 0, 1, 0}};
+    auto mul = [](const uint64_t* a, const uint64_t* b, uint64_t mod, uint64_t* r) {
+        uint64_t t[9];
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) {
+                u128 acc = 0;
+                for (int l = 0; l < 3; ++l) acc += u128(a[3 * i + l]) * b[3 * l + k];
+                t[3 * i + k] = uint64_t(acc % mod);
+            }
+        std::memcpy(r, t, sizeof t);
+    };
+    vector<uint64_t> out(32 * 2 * 9);
+    for (int c = 0; c < 2; ++c) {
+        // A^dim by squaring
+        uint64_t p[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, a[9];
+        std::memcpy(a, base[c], sizeof a);
+        for (unsigned e = unsigned(std::max(dim, 0)); e; e >>= 1) {
+            if (e & 1) mul(p, a, m[c], p);
+            mul(a, a, m[c], a);
+        }
+        for (int j = 0; j < 32; ++j) {
+            std::memcpy(&out[size_t(2 * j + c) * 9], p, sizeof p);
+            mul(p, p, m[c], p);
+        }
+    }
+    return out;
+}
+
 // Plans, tables and layout of a model and product; FULL layout when it fits,
 // LEAN otherwise.
 std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj)
@@ -783,24 +831,40 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
     const bool book = kernel_product(pd) == cfk::PRD_BOOK;
     if (book) P->book = build_book(pd, md.s_t0);
     P->cfg = cfk::KernelConfig{fp64 ? 0 : 1, md.kind, kernel_product(pd), adjoint, false, adj == cfk::ADJ_RECOMPUTE};
-    P->block = book ? std::min(block_threads(), cfk::kBookBlock) : block_threads();
+    P->block = std::min(block_threads(), cfk::book_block_limit(P->cfg.product, fp64 ? 8 : 4, adj));
     const int waves = P->block / cfk::kWave;
     const int forced = forced_layout();
+    auto lds_bytes = [&](int copies) {
+        return cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves, fp64 ? 8 : 4, dup,
+                                pd.kind == CF_PRODUCT_EUROPEANS, adjoint, P->cfg.lean, dup ? P->bt.nb : 0,
+                                book ? P->book.L.bytes : 0, copies)
+            .total;
+    };
+    P->ivbar_copies = 1;
     for (int lean = forced == 1 ? 1 : 0; lean < 2; ++lean) {
         P->cfg.lean = lean != 0;
-        const cfk::SmemLayout L =
-            cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves, fp64 ? 8 : 4, dup,
-                             pd.kind == CF_PRODUCT_EUROPEANS, adjoint, P->cfg.lean, dup ? P->bt.nb : 0,
-                             book ? P->book.L.bytes : 0);
-        P->smem = L.total;
+        P->smem = lds_bytes(1);
         if (P->smem <= size_t(E.max_lds)) break;
     }
     if (P->smem > size_t(E.max_lds))
         fail(CF_ERR_CAPACITY, "tables need " + std::to_string(P->smem) + " bytes of LDS per block, device allows " +
                                   std::to_string(E.max_lds));
-    const int rc = cfk::path_kernel_occupancy(P->cfg, P->block, P->smem, &P->per_cu);
+    int rc = cfk::path_kernel_occupancy(P->cfg, P->block, P->smem, &P->per_cu);
     if (rc != 0) hip_check(hipError_t(rc), "occupancy");
     if (P->per_cu < 1) fail(CF_ERR_CAPACITY, "path kernel does not fit on a compute unit");
+    // a second local-vol adjoint grid (odd lanes) when it costs no occupancy
+    if (adjoint && dup && ivbar_copies_allowed()) {
+        const size_t smem2 = lds_bytes(2);
+        int per_cu2 = 0;
+        if (smem2 <= size_t(E.max_lds)) {
+            rc = cfk::path_kernel_occupancy(P->cfg, P->block, smem2, &per_cu2);
+            if (rc != 0) hip_check(hipError_t(rc), "occupancy");
+            if (per_cu2 >= P->per_cu) {
+                P->smem = smem2;
+                P->ivbar_copies = 2;
+            }
+        }
+    }
 
     Staging st;
     P->o_ss = st.add(md.step_sample);
@@ -835,6 +899,10 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
         P->o_pk = st.add(pd.pay_strike);
     }
     if (book) P->o_book = st.add(P->book.blob);
+    {
+        const vector<uint64_t> jump = mrg_jumps(md.n_steps);
+        P->o_mrg = st.add(jump);
+    }
     P->staged = std::move(st.bytes);
 
     if (dup && adjoint) {
@@ -897,6 +965,12 @@ CallShape shape_call(const Engine& E, const Prepared& PR, const cf_num* num, boo
     const bool dup = PR.md.kind == CF_MODEL_DUPIRE;
     S.ivn = adjoint && dup ? size_t(PR.md.n_steps) * PR.md.n_spots : 0;
     S.ck_rows = adjoint ? cfk::ckpt_rows(PR.md.n_steps, dup, PR.cfg.recompute, S.fp64 ? 8 : 4) : 0;
+    S.n_steps = PR.md.n_steps;
+    S.mrg = !num->use_sobol && PR.md.n_steps > 0;
+    // the reference converts the int seeds to unsigned (main.h:55-57); states
+    // are equal modulo m, so the seeds are reduced
+    S.seed1 = uint64_t(unsigned(num->seed1)) % 4294967087ull;
+    S.seed2 = uint64_t(unsigned(num->seed2)) % 4294944443ull;
     return S;
 }
 
@@ -904,7 +978,8 @@ static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
 
 size_t CallBytes::total() const
 {
-    return al256(block_out) + al256(reduced) + al256(ivbar_blocks) + al256(ivbar) + al256(weights) + al256(ckpt);
+    return al256(block_out) + al256(reduced) + al256(ivbar_blocks) + al256(ivbar) + al256(weights) + al256(ckpt) +
+           al256(normals);
 }
 
 CallBytes call_bytes(const CallShape& S)
@@ -916,6 +991,7 @@ CallBytes call_bytes(const CallShape& S)
     b.ivbar = S.ivn * 8;
     b.weights = S.adjoint ? size_t(S.npay) * 8 : 0;
     b.ckpt = size_t(S.slots) * S.ck_rows * cfk::kWave * (S.fp64 ? 8 : 4);
+    b.normals = S.mrg ? size_t(S.n_path) * size_t(S.n_steps) * (S.fp64 ? 8 : 4) : 0;
     return b;
 }
 
@@ -966,20 +1042,37 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     P.pay_strike = reinterpret_cast<const double*>(dt + PR.o_pk);
     P.book = dt + PR.o_book;
     P.bk = PR.book.L;
+
     P.block_out = B.block_out;
     P.per_path = B.per_path;
     P.weights = B.weights;
     P.ckpt = B.ckpt;
     P.ivbar_blocks = B.ivbar_blocks;
     P.per_path_agg = B.per_path_agg;
+    P.normals = S.mrg ? B.normals : nullptr;
+    P.ivbar_copies = PR.ivbar_copies;
     return P;
 }
 
 void enqueue_call(const Prepared& PR, const CallShape& S, const CallBuffers& B, const cfk::LaunchParams& P,
                   hipStream_t stream, hipEvent_t ev_kernel0, hipEvent_t ev_kernel1)
 {
+    if (S.mrg) {
+        const uint64_t* jump = reinterpret_cast<const uint64_t*>(B.tables + PR.o_mrg);
+        const int rc = cfk::launch_mrg_normals(jump, S.seed1, S.seed2, PR.md.n_steps, S.first, S.n_path,
+                                               S.fp64 ? 8 : 4, B.normals, stream);
+        if (rc != 0) hip_check(hipError_t(rc), "normals launch");
+    }
+    cfk::KernelConfig cfg = PR.cfg;
+    cfg.normals = S.mrg;
+    if (S.mrg) {
+        // the normals-buffer kernel's LDS limit is set once, like the plan's kernel
+        int per_cu = 0;
+        const int orc = cfk::path_kernel_occupancy(cfg, PR.block, PR.smem, &per_cu);
+        if (orc != 0) hip_check(hipError_t(orc), "occupancy");
+    }
     if (ev_kernel0) hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
-    int rc = cfk::launch_path_kernel(PR.cfg, P, S.grid, PR.block, PR.smem, stream);
+    int rc = cfk::launch_path_kernel(cfg, P, S.grid, PR.block, PR.smem, stream);
     if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
     if (ev_kernel1) hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
     rc = cfk::launch_reduce_rows(B.block_out, S.grid, S.stride, B.reduced, stream);
@@ -1061,6 +1154,7 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
         B.weights = static_cast<double*>(E.weights.ensure(nb.weights));
         B.ckpt = E.ckpt.ensure(nb.ckpt);
     }
+    if (S.mrg) B.normals = E.normals.ensure(nb.normals);
     if (c.per_path) B.per_path = static_cast<double*>(E.per_path.ensure(size_t(S.n_path) * S.npay * 8));
     if (c.per_path_agg) B.per_path_agg = static_cast<double*>(E.per_path_agg.ensure(size_t(S.n_path) * 8));
     double* h_in = static_cast<double*>(E.pin_in.ensure(size_t(S.npay) * 8));

@@ -191,11 +191,13 @@ struct Prepared {
     size_t smem = 0;
     int per_cu = 0;
     int block = 0;
+    int ivbar_copies = 1;   // LDS local-vol adjoint grids per block
     BookTables book;
     vector<unsigned char> staged;   // the tables as uploaded
     DevBuf tables;                  // device copy for the synchronous calls
     size_t o_ss = 0, o_knots = 0, o_invdx = 0, o_iv = 0, o_drift = 0, o_std = 0, o_bucket = 0;
     size_t o_ff = 0, o_disc = 0, o_num = 0, o_aux = 0, o_pf = 0, o_pk = 0, o_book = 0;
+    size_t o_mrg = 0;   // mrg32k3a jump matrices [32][2][9] for dim = n_steps
     // adjoint of the local-vol pre-interpolation: sqrt(dt_i) and the time
     // interpolation weights of each step
     vector<double> sqrtdt;
@@ -215,7 +217,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     uint32_t* d_dirs = nullptr;
-    DevBuf block_out, ivbar_blocks, reduced, ivbar, per_path, per_path_agg, ckpt, weights;
+    DevBuf block_out, ivbar_blocks, reduced, ivbar, per_path, per_path_agg, ckpt, weights, normals;
     PinnedBuf pin_in, pin_out;
     std::list<std::pair<string, std::shared_ptr<Prepared>>> plans;   // most recent first
     std::mutex plan_mu;   // the plan cache
@@ -257,7 +259,9 @@ std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_produc
 // blocks of the device, enough waves for the 64-path Sobol blocks.
 struct CallShape {
     int64_t first = 0, n_path = 0;
-    int grid = 0, waves = 0, slots = 0, stride = 0, npay = 0, ck_rows = 0;
+    bool mrg = false;               // mrg32k3a normals pre-generated into a buffer
+    uint64_t seed1 = 0, seed2 = 0;  // reduced modulo m1, m2
+    int grid = 0, waves = 0, slots = 0, stride = 0, npay = 0, ck_rows = 0, n_steps = 0;
     size_t ivn = 0;                 // Dupire adjoint: steps x spots
     bool adjoint = false, fp64 = false;
 };
@@ -274,12 +278,13 @@ struct CallBuffers {
     void* ckpt = nullptr;           // [slots][ck_rows][64] reals
     double* per_path = nullptr;     // [n_path][npay] or null
     double* per_path_agg = nullptr; // [n_path] or null
+    void* normals = nullptr;        // mrg32k3a: [n_steps][n_path] reals
 };
 
 // Bytes of each device buffer (0 when unused) and of all of them, each part
 // aligned to 256 bytes, tables excluded.
 struct CallBytes {
-    size_t block_out, reduced, ivbar_blocks, ivbar, weights, ckpt;
+    size_t block_out, reduced, ivbar_blocks, ivbar, weights, ckpt, normals;
     size_t total() const;
 };
 CallBytes call_bytes(const CallShape& S);

@@ -32,6 +32,13 @@ constexpr int kSegF32 = 8, kSegF64 = 4;
 constexpr int kMaxBookBarriers = 16, kMaxBookGroups = 4;
 constexpr int kBookBlock = 512;
 
+// Threads per block of a path kernel: 1024, except the book kernels whose
+// registers need 512 (FP64, and the FP32 recompute variant).
+constexpr int book_block_limit(int prd, int real_bytes, int adj)
+{
+    return prd == PRD_BOOK && (real_bytes == 8 || adj == ADJ_RECOMPUTE) ? kBookBlock : 1024;
+}
+
 // Byte offsets of the book tables inside one blob (staged whole in LDS).
 struct BookLayout {
     int n_van, n_bar, n_grp, n_asian;
@@ -42,11 +49,13 @@ struct BookLayout {
     int bar_mask;     // [n_samples] uint32: barriers observing at s
     int bar_last;     // [n_samples] uint32: barriers maturing at s
     int bar_par;      // [n_bar][4] double: strike, B + w, B - w, 1 / 2w
+    int bar_par_f;    // [n_bar][4] the same in float
     int bar_col;      // [n_bar] int: column of the barrier payoff (vanilla next)
     int bar_put;      // [n_bar] int
     int grp_mask;     // [n_samples] uint32: schedules fixing at s
     int grp_last;     // [n_samples] uint32: schedules ending at s
     int grp_inv;      // [n_grp] double: 1 / number of fixings
+    int grp_inv_f;    // [n_grp] float
     int grp_first;    // [n_grp + 1] int: Asians of each schedule
     int as_strike;    // [n_asian] double
     int as_col;       // [n_asian] int
@@ -111,6 +120,9 @@ struct LaunchParams {
     const void* book;            // book tables, layout bk
     BookLayout bk;
 
+    // pre-generated normals [n_steps][n_path] (mrg32k3a), or null: Sobol
+    const void* normals;
+
     // outputs
     double* block_out;           // [gridDim][out_stride(n_pay)]
     double* per_path;            // [n_path][n_pay] or null
@@ -118,6 +130,7 @@ struct LaunchParams {
     // adjoint
     const double* weights;       // [n_pay] aggregation weights
     void* ckpt;                  // [slots][ckpt_rows][64] path states or normals
+    int ivbar_copies;            // LDS local-vol adjoint grids per block: even and odd lanes (2) or one
     double* ivbar_blocks;        // [gridDim][n_steps][n_spots] local-vol adjoints
     double* per_path_agg;        // [n_path] or null
 };
@@ -137,7 +150,7 @@ CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 // table from global memory through the caches.
 CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int n_pay, int waves,
                                     int real_bytes, bool dupire, bool europeans, bool adjoint, bool lean,
-                                    int n_buckets, int book_bytes)
+                                    int n_buckets, int book_bytes, int ivbar_copies)
 {
     SmemLayout L{};
     size_t o = 0;
@@ -155,7 +168,7 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
     L.book = o;        o = cf_align16(o + size_t(book_bytes));
     L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
-    L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(n_steps) * n_spots * 8 : 0));
+    L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * n_steps * n_spots * 8 : 0));
     L.total = o;
     return L;
 }
@@ -168,18 +181,24 @@ struct KernelConfig {
     bool adjoint;
     bool lean;       // LEAN shared-memory layout
     bool recompute;  // adjoint by re-simulation instead of per-step checkpoints
+    bool normals;    // normals read from a buffer (mrg32k3a) instead of Sobol
 };
 
-// Path kernel families (precision r: 0 FP64, 1 FP32; adjoint a: AdjointKind),
-// one per build of cf_kern_inst.hip: kernel for a model, product and layout.
-#define CF_PICK_CAT(r, a) pick_r##r##_a##a
-#define CF_PICK_NAME(r, a) CF_PICK_CAT(r, a)
-const void* pick_r0_a0(int model, int prd, bool lean);
-const void* pick_r0_a1(int model, int prd, bool lean);
-const void* pick_r0_a2(int model, int prd, bool lean);
-const void* pick_r1_a0(int model, int prd, bool lean);
-const void* pick_r1_a1(int model, int prd, bool lean);
-const void* pick_r1_a2(int model, int prd, bool lean);
+// Path kernel families (precision r: 0 FP64, 1 FP32; adjoint a: AdjointKind;
+// normals b: 0 Sobol, 1 buffer), one per build of cf_kern_inst.hip: kernel
+// for a model, product and layout.
+#define CF_PICK_CAT(r, a, b) pick_r##r##_a##a##_b##b
+#define CF_PICK_NAME(r, a, b) CF_PICK_CAT(r, a, b)
+#define CF_PICK_DECL(r, a)                                                                                            \
+    const void* pick_r##r##_a##a##_b0(int model, int prd, bool lean);                                                  \
+    const void* pick_r##r##_a##a##_b1(int model, int prd, bool lean);
+CF_PICK_DECL(0, 0)
+CF_PICK_DECL(0, 1)
+CF_PICK_DECL(0, 2)
+CF_PICK_DECL(1, 0)
+CF_PICK_DECL(1, 1)
+CF_PICK_DECL(1, 2)
+#undef CF_PICK_DECL
 
 // Returns 0 on success, else a hipError_t value.
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block,
@@ -188,6 +207,12 @@ int launch_reduce_rows(const double* in, int rows, int cols, double* out, void* 
 int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* blocks_per_cu);
 
 int launch_copy(const void* in, void* out, size_t bytes, void* stream);
+
+// mrg32k3a normals of paths [first_path, first_path + n_path), dim steps,
+// written [step][path] in FP64 (real_bytes 8) or FP32.  jump: [32][2][9] the
+// component matrices raised to dim * 2^j (mod m1, m2); seeds already reduced.
+int launch_mrg_normals(const uint64_t* jump, uint64_t seed1, uint64_t seed2, int dim, int64_t first_path,
+                       int64_t n_path, int real_bytes, void* out, void* stream);
 
 // Sets the thread-local message returned by cf_last_error() (cf_engine.cpp).
 void set_last_error(const char* msg);

@@ -1,13 +1,14 @@
-// One (precision, adjoint) family of path kernels, instantiated for every
-// model, product and LDS layout.  Built once per family with
-// -DCF_INST_REAL=0|1 (FP64, FP32) and -DCF_INST_ADJ=0|1|2 (cfk::AdjointKind),
-// so the families compile in parallel.
+// One (precision, adjoint, normals source) family of path kernels,
+// instantiated for every model, product and LDS layout.  Built once per
+// family with -DCF_INST_REAL=0|1 (FP64, FP32), -DCF_INST_ADJ=0|1|2
+// (cfk::AdjointKind) and -DCF_INST_NBUF=0|1 (Sobol, normals buffer), so the
+// families compile in parallel.
 #include <type_traits>
 
 #include "cf_path_kernel.h"
 
-#if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ)
-#error "CF_INST_REAL and CF_INST_ADJ must be defined"
+#if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ) || !defined(CF_INST_NBUF)
+#error "CF_INST_REAL, CF_INST_ADJ and CF_INST_NBUF must be defined"
 #endif
 
 namespace cfk {
@@ -15,12 +16,13 @@ namespace {
 
 using Real = std::conditional_t<CF_INST_REAL == 0, double, float>;
 constexpr int kAdj = CF_INST_ADJ;
+constexpr bool kNbuf = CF_INST_NBUF != 0;
 
 template <int MODEL, int PRD>
 const void* ptr(bool lean)
 {
-    return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true>)
-                : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false>);
+    return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true, kNbuf>)
+                : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false, kNbuf>);
 }
 
 template <int MODEL>
@@ -37,7 +39,7 @@ const void* by_product(int prd, bool lean)
 
 }  // namespace
 
-const void* CF_PICK_NAME(CF_INST_REAL, CF_INST_ADJ)(int model, int prd, bool lean)
+const void* CF_PICK_NAME(CF_INST_REAL, CF_INST_ADJ, CF_INST_NBUF)(int model, int prd, bool lean)
 {
     return model == MODEL_DUPIRE ? by_product<MODEL_DUPIRE>(prd, lean) : by_product<MODEL_BS>(prd, lean);
 }

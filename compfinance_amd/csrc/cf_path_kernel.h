@@ -90,12 +90,10 @@ __device__ __forceinline__ double log_pos(double x)
     return fma(ed, ln2_hi, fma(2.0 * s * s2, p, fma(ed, ln2_lo, 2.0 * s)));
 }
 
-// Beasley-Springer-Moro inverse normal of u = ONEOVER2POW32 * xi
-// (gaussians.h:47-87, sobol.h:26, :110-116), FP64, same branches and
-// constants as the reference.
-__device__ __forceinline__ double inv_normal(uint32_t xi, double)
+// Beasley-Springer-Moro inverse normal (gaussians.h:47-87), FP64, same
+// branches and constants as the reference.
+__device__ __forceinline__ double inv_normal_p(double p)
 {
-    const double p = 2.3283064365387E-10 * double(xi);
     const bool sup = p > 0.5;
     const double up = sup ? 1.0 - p : p;
     const double x = up - 0.5;
@@ -115,6 +113,12 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, double)
         r * (0.0000321767881768 +
         r * (0.0000002888167364 + r * 0.0000003960315187)))))));
     return sup ? r : -r;
+}
+
+// of the Sobol uniform u = ONEOVER2POW32 * xi (sobol.h:26, :110-116)
+__device__ __forceinline__ double inv_normal(uint32_t xi, double)
+{
+    return inv_normal_p(2.3283064365387E-10 * double(xi));
 }
 
 // FP32 form: Giles' single-precision erfinv approximation (M. Giles,
@@ -269,8 +273,8 @@ struct ProductState {
 
 // ------------------------------------------------------------------ kernel
 
-template <typename R, int MODEL, int PRD, int ADJ, bool LEAN>
-__global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kernel(const LaunchParams P)
+template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF>
+__global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) path_kernel(const LaunchParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool DUP = MODEL == MODEL_DUPIRE;
@@ -283,7 +287,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
     const int stride = out_stride(npay);
     constexpr bool BOOK = PRD == PRD_BOOK;
     const SmemLayout L = smem_layout(n, m, ns, npay, waves, int(sizeof(R)), DUP, EUS, AD, LEAN, P.n_buckets,
-                                     BOOK ? P.bk.bytes : 0);
+                                     BOOK ? P.bk.bytes : 0, P.ivbar_copies);
 
     uint32_t* lo_tab = reinterpret_cast<uint32_t*>(smem + L.lo);
     uint32_t* hi_all = reinterpret_cast<uint32_t*>(smem + L.hi);
@@ -347,7 +351,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
     }
     for (int e = tid; e < waves * stride; e += nthr) acc[e] = 0.0;
     if constexpr (AD && DUP)
-        for (int e = tid; e < n * m; e += nthr) ivbar[e] = 0.0;
+        for (int e = tid; e < P.ivbar_copies * n * m; e += nthr) ivbar[e] = 0.0;
     __syncthreads();
 
     const int wave = tid >> 6, lane = tid & 63;
@@ -391,6 +395,10 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
     const double s0 = P.spot;
     const int last = ns - 1;
 
+    const R* nrm = static_cast<const R*>(P.normals);
+    // this lane's local-vol adjoint grid: odd lanes use the second copy, which
+    // halves the lanes contending for one address in an atomic
+    double* ivbl = ivbar + ((P.ivbar_copies > 1 && (lane & 1)) ? n * m : 0);
     // per-lane parameter adjoints, summed over the lane's paths
     double padj[kMaxParamAdj] = {0.0, 0.0, 0.0, 0.0};
 
@@ -401,24 +409,31 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
     const int* bk_van_col = reinterpret_cast<const int*>(bkb + P.bk.van_col);
     const uint32_t* bk_bar_mask = reinterpret_cast<const uint32_t*>(bkb + P.bk.bar_mask);
     const uint32_t* bk_bar_last = reinterpret_cast<const uint32_t*>(bkb + P.bk.bar_last);
-    const double* bk_bar_par = reinterpret_cast<const double*>(bkb + P.bk.bar_par);
-    const int* bk_bar_col = reinterpret_cast<const int*>(bkb + P.bk.bar_col);
-    const int* bk_bar_put = reinterpret_cast<const int*>(bkb + P.bk.bar_put);
     const uint32_t* bk_grp_mask = reinterpret_cast<const uint32_t*>(bkb + P.bk.grp_mask);
     const uint32_t* bk_grp_last = reinterpret_cast<const uint32_t*>(bkb + P.bk.grp_last);
-    const double* bk_grp_inv = reinterpret_cast<const double*>(bkb + P.bk.grp_inv);
     const int* bk_grp_first = reinterpret_cast<const int*>(bkb + P.bk.grp_first);
     const double* bk_as_strike = reinterpret_cast<const double*>(bkb + P.bk.as_strike);
     const int* bk_as_col = reinterpret_cast<const int*>(bkb + P.bk.as_col);
     const int* bk_as_put = reinterpret_cast<const int*>(bkb + P.bk.as_put);
+    const int* bk_bar_col = reinterpret_cast<const int*>(bkb + P.bk.bar_col);
+    const int* bk_bar_put = reinterpret_cast<const int*>(bkb + P.bk.bar_put);
+    // barrier parameters (strike, B + w, B - w, 1 / 2w) and fixing weights in
+    // the kernel's precision
+    const R* bk_par = reinterpret_cast<const R*>(bkb + (sizeof(R) == 4 ? P.bk.bar_par_f : P.bk.bar_par));
+    const R* bk_ginv = reinterpret_cast<const R*>(bkb + (sizeof(R) == 4 ? P.bk.grp_inv_f : P.bk.grp_inv));
+    auto bar_par = [&](int b, int k) { return bk_par[4 * b + k]; };
     constexpr int NBAR = BOOK ? kMaxBookBarriers : 1, NGRP = BOOK ? kMaxBookGroups : 1;
-    // barrier flags: number of exactly-zero smoothing factors (saturating at
-    // 2) and the breach bit
-    constexpr int kNzOne = 1, kNzMask = 3, kDead = 4;
+    // Book barrier state in one real per barrier:
+    //   v in [0, 1]   no exactly-zero smoothing factor yet, v = product of the
+    //                 factors (the survival factor)
+    //   v in [-1, 0)  one zero factor, -v = product of the others
+    //   v = -2        two or more zero factors
+    //   v = -4        breached (survival 0, no derivative)
+    constexpr float kTwoZeros = -2.0f, kBreached = -4.0f;
 
     for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= q1; q += int64_t(gridDim.x) * waves) {
         // ---- high Sobol words of this block of 64 indices
-        {
+        if constexpr (!NBUF) {
             const uint32_t gq = uint32_t(q ^ (q >> 1));
             for (int d = lane; d < n; d += kWave) {
                 uint32_t x = (q & 1) ? P.dirs[5 * kSobolDims + d] : 0u;
@@ -435,19 +450,24 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
         const int64_t sidx = q * kWave + lane;
         const bool active = sidx >= nlo && sidx <= nhi;
         const int64_t pidx = sidx - nlo;
+        // normal of step i: from the Sobol point, or from the buffer of
+        // pre-generated normals ([step][path] of the call's paths)
+        const size_t ncol = size_t(active ? pidx : 0);
+        auto draw = [&](int i) -> R {
+            if constexpr (NBUF) return nrm[size_t(i) * size_t(P.n_path) + ncol];
+            else return inv_normal(hi[i] ^ low_word(i), R());
+        };
 
         ProductState<R> ps;
         ps.alive = R(1); ps.pnz = R(1); ps.nzero = 0; ps.dead = false;
         ps.euro = R(0); ps.fin_f = R(0); ps.fin_num = R(1); ps.aggp = R(0);
-        // book: per barrier the product of the non-zero smoothing factors, the
-        // flags and (after its maturity) weight x vanilla, the adjoint of its
-        // survival factor; per fixing schedule the running sum, replaced at
-        // its last fixing by the adjoint of each fixing, and the weighted
-        // intrinsic value (numeraire adjoint)
-        R bpnz[NBAR], babar[NBAR], gsum[NGRP], gq[NGRP];
-        int bflag[NBAR];
+        // book: per barrier the state above and (after its maturity) weight x
+        // vanilla, the adjoint of its survival factor; per fixing schedule
+        // the running sum, replaced at its last fixing by the adjoint of each
+        // fixing, and the weighted intrinsic value (numeraire adjoint)
+        R bst[NBAR], babar[NBAR], gsum[NGRP], gq[NGRP];
 #pragma unroll
-        for (int b = 0; b < NBAR; ++b) { bpnz[b] = R(1); babar[b] = R(0); bflag[b] = 0; }
+        for (int b = 0; b < NBAR; ++b) { bst[b] = R(1); babar[b] = R(0); }
 #pragma unroll
         for (int g = 0; g < NGRP; ++g) { gsum[g] = R(0); gq[g] = R(0); }
 
@@ -502,29 +522,30 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                     const R kk = R(bk_van_strike[k]);
                     emit(bk_van_col[k], (F - kk > R(0) ? F - kk : R(0)) / numv);
                 }
-                // up-and-out barriers (mcPrd.h:256-287, one state per barrier)
+                // up-and-out barriers (mcPrd.h:256-287, one state per barrier),
+                // branch free: breach, smoothing zone, exactly-zero factor
                 const uint32_t bm = bk_bar_mask[s], bl = bk_bar_last[s];
                 if (bm | bl) {
 #pragma unroll
                     for (int b = 0; b < NBAR; ++b) {
                         if ((bm >> b) & 1u) {
-                            const double* bp = bk_bar_par + 4 * b;
-                            if (!(bflag[b] & kDead)) {
-                                if (F > R(bp[1])) {
-                                    bflag[b] |= kDead;
-                                } else if (F > R(bp[2])) {
-                                    const R f = (R(bp[1]) - F) * R(bp[3]);
-                                    if (f == R(0)) bflag[b] += (bflag[b] & kNzMask) < 2 ? kNzOne : 0;
-                                    else bpnz[b] = bpnz[b] * f;
-                                }
-                            }
+                            const R bs = bar_par(b, 1), ms = bar_par(b, 2);
+                            const R v = bst[b];
+                            const bool live = v > R(kBreached);
+                            const bool zone = live && F > ms && !(F > bs);
+                            const R f = (bs - F) * bar_par(b, 3);
+                            R nv = v;
+                            nv = zone && f != R(0) && v >= R(-1) ? v * f : nv;          // one more factor
+                            nv = zone && f == R(0) ? (v >= R(0) ? -v : R(kTwoZeros)) : nv;   // a zero factor
+                            nv = live && F > bs ? R(kBreached) : nv;                    // breach
+                            bst[b] = nv;
                         }
                         if ((bl >> b) & 1u) {
-                            const R k = R(bk_bar_par[4 * b]);
+                            const R k = bar_par(b, 0);
                             const R intrinsic = bk_bar_put[b] ? (k - F > R(0) ? k - F : R(0))
-                                                              : (F - k > R(0) ? F - k : R(0));
+                                                                         : (F - k > R(0) ? F - k : R(0));
                             const R euro = intrinsic / numv;
-                            const R alive = (bflag[b] & (kDead | kNzMask)) ? R(0) : bpnz[b];
+                            const R alive = bst[b] > R(0) ? bst[b] : R(0);
                             const int col = bk_bar_col[b];
                             emit(col, alive * euro);
                             emit(col + 1, euro);
@@ -540,7 +561,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                         if (!((gm >> g) & 1u)) continue;
                         gsum[g] = gsum[g] + F;
                         if ((gl >> g) & 1u) {
-                            const R inv_n = R(bk_grp_inv[g]);
+                            const R inv_n = bk_ginv[g];
                             const R avg = gsum[g] * inv_n;
                             R dsum = R(0), qsum = R(0);
                             for (int a = bk_grp_first[g]; a < bk_grp_first[g + 1]; ++a) {
@@ -586,8 +607,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
             if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
         }
         for (int i = 0; i < n; ++i) {
-            const uint32_t u = hi[i] ^ low_word(i);
-            const R g = inv_normal(u, R());
+            const R g = draw(i);
             if constexpr (DUP) {
                 if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = x;
                 if constexpr (RECOMP)
@@ -661,26 +681,32 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                     if (bm | bl) {
 #pragma unroll
                         for (int b = 0; b < NBAR; ++b) {
-                            const double* bp = bk_bar_par + 4 * b;
                             if ((bl >> b) & 1u) {
                                 const int col = bk_bar_col[b];
-                                const R k = R(bp[0]);
+                                const R k = bar_par(b, 0);
                                 const bool put = bk_bar_put[b] != 0;
                                 const R intrinsic = put ? (k - F > R(0) ? k - F : R(0))
                                                         : (F - k > R(0) ? F - k : R(0));
                                 const R dmax = put ? (k > F ? R(-1) : R(0)) : (F > k ? R(1) : R(0));
-                                const R alive = (bflag[b] & (kDead | kNzMask)) ? R(0) : bpnz[b];
+                                const R alive = bst[b] > R(0) ? bst[b] : R(0);
                                 const R euro_bar = R(P.weights[col]) * alive + R(P.weights[col + 1]);
                                 Fb += euro_bar * dmax / numv;
                                 nb += -euro_bar * (intrinsic / numv) / numv;
                             }
-                            if (((bm >> b) & 1u) && !(bflag[b] & kDead) && F > R(bp[2])) {
-                                const R f = (R(bp[1]) - F) * R(bp[3]);
-                                const int nz = bflag[b] & kNzMask;
-                                R others;
-                                if (nz == 0) others = recip_mul(bpnz[b], f);
-                                else others = (nz == 1 && f == R(0)) ? bpnz[b] : R(0);
-                                Fb -= babar[b] * others * R(bp[3]);
+                            if ((bm >> b) & 1u) {
+                                // d alive / dF = -(product of the other factors) / 2w
+                                const R v = bst[b];
+                                if (v > R(kBreached) && F > bar_par(b, 2)) {   // smoothing zone
+                                    const R i2s = bar_par(b, 3);
+                                    const R f = (bar_par(b, 1) - F) * i2s;
+                                    // product of the other factors: v / f without a zero
+                                    // factor, the non-zero ones for the only zero factor
+                                    // (v >= 0 with f == 0 only after the product underflowed
+                                    // to zero: the other factors' product is 0)
+                                    const R others = v >= R(0) ? (f != R(0) ? recip_mul(v, f) : R(0))
+                                                               : ((v >= R(-1) && f == R(0)) ? -v : R(0));
+                                    Fb -= babar[b] * others * i2s;
+                                }
                             }
                         }
                     }
@@ -737,8 +763,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                         cv[k] = R(0);
                         cj[k] = 0;
                         if (i >= n) continue;
-                        const uint32_t u = hi[i] ^ low_word(i);
-                        const R g = inv_normal(u, R());
+                        const R g = draw(i);
                         const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol
                         ca[k] = R(1) + ws * b.slope;
@@ -759,7 +784,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                         const int i = i0 + k;
                         if (i >= n) continue;
                         if (active) {
-                            double* row = ivbar + i * m;
+                            double* row = ivbl + i * m;
                             atomicAdd(row + cj[k], double(abar * cw[k]));
                             if (m > 1) atomicAdd(row + cj[k] + 1, double(abar * cv[k]));
                         }
@@ -781,7 +806,8 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                 }
                 R xnext = x;     // state at point i + 1
                 // checkpoint loads run kPrefetch steps ahead of their use
-                constexpr int kPrefetch = sizeof(R) == 8 ? 1 : 4;   // FP64 holds twice the registers
+                // FP64 and the book hold more registers
+                constexpr int kPrefetch = sizeof(R) == 8 ? 1 : (BOOK ? 1 : 4);
                 R buf[kPrefetch];
 #pragma unroll
                 for (int k = 0; k < kPrefetch; ++k) buf[k] = n - 1 - k >= 0 ? ck[(n - 1 - k) * kWave + lane] : R(0);
@@ -808,8 +834,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                         if (b.vol != R(0)) {
                             ws[k] = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
                         } else {
-                            const uint32_t u = hi[i] ^ low_word(i);
-                            ws[k] = inv_normal(u, R());
+                            ws[k] = draw(i);
                         }
                         sl[k] = b.slope;
                         wa[k] = b.wa;
@@ -838,7 +863,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                         for (int k = 0; k < kPrefetch; ++k) {
                             const int i = i0 - k;
                             if (i < 0) break;
-                            double* row = ivbar + i * m;
+                            double* row = ivbl + i * m;
                             atomicAdd(row + js[k], double(vb[k] * wa[k]));
                             if (m > 1) atomicAdd(row + js[k] + 1, double(vb[k] * wb[k]));
                         }
@@ -886,7 +911,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
                 }
                 for (int i = 0; i < n; ++i) {
                     R g;
-                    if constexpr (RECOMP) g = inv_normal(hi[i] ^ low_word(i), R());   // regenerated
+                    if constexpr (RECOMP) g = draw(i);   // regenerated
                     else g = ck[i * kWave + lane];
                     spot = spot * rexp(drift[i] + stdv[i] * g);
                     const R dti = R(dt[i]);
@@ -916,7 +941,7 @@ __global__ void __launch_bounds__(PRD == PRD_BOOK ? kBookBlock : 1024) path_kern
     }
     if constexpr (AD && DUP) {
         double* dst = P.ivbar_blocks + size_t(blockIdx.x) * n * m;
-        for (int e = tid; e < n * m; e += nthr) dst[e] = ivbar[e];
+        for (int e = tid; e < n * m; e += nthr) dst[e] = P.ivbar_copies > 1 ? ivbar[e] + ivbar[n * m + e] : ivbar[e];
     }
 }
 

@@ -232,6 +232,7 @@ void launch(Server& S, Slot& s, Request& r)
     B.ivbar = reinterpret_cast<double*>(take(nb.ivbar));
     B.weights = reinterpret_cast<double*>(take(nb.weights));
     B.ckpt = take(nb.ckpt);
+    B.normals = take(nb.normals);
     // host staging: tables then weights, one upload each
     const size_t tb = PR.staged.size();
     unsigned char* hin = static_cast<unsigned char*>(s.pin_in.ensure(tb + nb.weights + 8));

@@ -106,9 +106,9 @@ enum cf_aad_mode { CF_AAD_CHECKPOINT = 0, CF_AAD_RECOMPUTE = 1 };
 /* NumericalParam (main.h:33-40) plus the engine's own options. */
 typedef struct cf_num {
     int parallel;            /* accepted for compatibility; the GPU is always parallel */
-    int use_sobol;           /* must be 1: the engine implements Sobol only */
+    int use_sobol;           /* 1: Sobol; 0: mrg32k3a with seed1, seed2 (main.h:51-52) */
     int num_path;
-    int seed1;               /* unused with Sobol (main.h:51-52) */
+    int seed1;               /* mrg32k3a seeds, converted to unsigned as the reference does */
     int seed2;
     int precision;           /* CF_FP64 or CF_FP32 */
     int aad_mode;            /* CF_AAD_CHECKPOINT or CF_AAD_RECOMPUTE */

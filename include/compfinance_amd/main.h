@@ -26,9 +26,10 @@ struct NumericalParam {
     int aadMode = CF_AAD_CHECKPOINT;
 };
 
+// main.h:51-52: Sobol, or mrg32k3a with the seeds
 inline unique_ptr<RNG> makeRng(const NumericalParam& num)
 {
-    if (!num.useSobol) throw std::runtime_error("only the Sobol generator is implemented on the GPU engine");
+    if (!num.useSobol) return std::make_unique<mrg32k3a>(unsigned(num.seed1), unsigned(num.seed2));
     return std::make_unique<Sobol>();
 }
 

@@ -122,8 +122,8 @@ public:
     }
 };
 
-// RNG (mcBase.h:228-246).  The engine draws Sobol points on the device; an
-// RNG here only selects the sequence and the first path (skipTo).
+// RNG (mcBase.h:228-246).  The engine draws the numbers on the device; an
+// RNG here only selects the sequence, its seeds and the first path (skipTo).
 class RNG {
 public:
     virtual unique_ptr<RNG> clone() const = 0;
@@ -132,6 +132,8 @@ public:
     virtual void skipTo(const unsigned b) = 0;
     virtual bool isSobol() const = 0;
     virtual unsigned firstPath() const = 0;
+    virtual unsigned seed1() const { return 0; }
+    virtual unsigned seed2() const { return 0; }
 };
 
 // Sobol (sobol.h:30-152): path p of a simulation uses point p + 1 of the
@@ -149,6 +151,22 @@ public:
     unsigned firstPath() const override { return myFirst; }
 };
 
+// mrg32k3a (mrg32k3a.h:25-395) with seeds: path 2k draws block k of the
+// sequence and path 2k + 1 its negation (antithetic pairs); skipTo(b) with b
+// even starts at path b, as the reference's parallel batches do.
+class mrg32k3a : public RNG {
+    unsigned myA, myB, myFirst = 0;
+
+public:
+    mrg32k3a(const unsigned a = 12345, const unsigned b = 12346) : myA(a), myB(b) {}
+    unique_ptr<RNG> clone() const override { return std::make_unique<mrg32k3a>(*this); }
+    void skipTo(const unsigned b) override { myFirst = b; }
+    bool isSobol() const override { return false; }
+    unsigned firstPath() const override { return myFirst; }
+    unsigned seed1() const override { return myA; }
+    unsigned seed2() const override { return myB; }
+};
+
 // Compatibility of model and product (mcBase.h:254-260)
 template <class T>
 inline bool checkCompatiblity(const Product<T>& prd, const Model<T>& mdl)
@@ -158,14 +176,13 @@ inline bool checkCompatiblity(const Product<T>& prd, const Model<T>& mdl)
 
 inline cf_num makeNum(const RNG& rng, size_t nPath)
 {
-    if (!rng.isSobol()) throw std::runtime_error("only the Sobol generator is implemented on the GPU engine");
     if (nPath == 0 || nPath > size_t(0x7fffffff)) throw std::runtime_error("number of paths out of range");
     cf_num n{};
     n.parallel = 1;
-    n.use_sobol = 1;
+    n.use_sobol = rng.isSobol() ? 1 : 0;
     n.num_path = int(nPath);
-    n.seed1 = 12345;
-    n.seed2 = 1234;
+    n.seed1 = rng.isSobol() ? 12345 : int(rng.seed1());
+    n.seed2 = rng.isSobol() ? 1234 : int(rng.seed2());
     n.precision = engineOptions().precision;
     n.aad_mode = engineOptions().aadMode;
     n.first_path = int(rng.firstPath());

@@ -134,6 +134,107 @@ static double invNormalCdf(double p)
     return sup ? r : -r;
 }
 
+// ---------------------------------------------------------------- mrg32k3a
+// mrg32k3a.h:25-395: L'Ecuyer's combined recursion in doubles, uniforms
+// (x - y) / (m1 + 1) folded into (0, 1), antithetic pairs (a cached block of
+// Gaussians returned negated on the next call), skip-ahead by matrix powers.
+struct Mrg {
+    static constexpr double m1 = 4294967087.0, m2 = 4294944443.0;
+    static constexpr double a12 = 1403580.0, a13 = 810728.0, a21 = 527612.0, a23 = 1370589.0;
+    double A = 12345, B = 12346;
+    size_t dim = 0;
+    double xn, xn1, xn2, yn, yn1, yn2;
+    bool anti = false;
+    vector<double> cached;
+
+    Mrg(unsigned a, unsigned b) : A(a), B(b) { reset(); }
+    void reset()
+    {
+        xn = xn1 = xn2 = A;
+        yn = yn1 = yn2 = B;
+        anti = false;
+    }
+    void init(size_t d)
+    {
+        dim = d;
+        cached.assign(d, 0.0);
+    }
+    double next()
+    {
+        double x = a12 * xn1 - a13 * xn2;
+        x -= long(x / m1) * m1;
+        if (x < 0) x += m1;
+        xn2 = xn1;
+        xn1 = xn;
+        xn = x;
+        double y = a21 * yn - a23 * yn2;
+        y -= long(y / m2) * m2;
+        if (y < 0) y += m2;
+        yn2 = yn1;
+        yn1 = yn;
+        yn = y;
+        return x > y ? (x - y) / 4294967088.0 : (x - y + m1) / 4294967088.0;
+    }
+    void nextG(vector<double>& g)
+    {
+        if (anti) {
+            for (size_t i = 0; i < dim; ++i) g[i] = -cached[i];
+            anti = false;
+        } else {
+            for (size_t i = 0; i < dim; ++i) cached[i] = g[i] = invNormalCdf(next());
+            anti = true;
+        }
+    }
+    static void mprd(const uint64_t l[3][3], const uint64_t r[3][3], uint64_t mod, uint64_t out[3][3])
+    {
+        uint64_t t[3][3];
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                uint64_t acc = 0;
+                for (int q = 0; q < 3; ++q) acc = (acc + (l[j][q] * r[q][k]) % mod) % mod;
+                t[j][k] = acc;
+            }
+        std::memcpy(out, t, sizeof t);
+    }
+    // advance by b numbers (mrg32k3a.h:310-381)
+    void skipNumbers(uint64_t b)
+    {
+        uint64_t Ab[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}, Bb[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        uint64_t Ai[3][3] = {{0, uint64_t(a12), uint64_t(m1 - a13)}, {1, 0, 0}, {0, 1, 0}};
+        uint64_t Bi[3][3] = {{uint64_t(a21), 0, uint64_t(m2 - a23)}, {1, 0, 0}, {0, 1, 0}};
+        for (; b; b >>= 1) {
+            if (b & 1) {
+                mprd(Ab, Ai, uint64_t(m1), Ab);
+                mprd(Bb, Bi, uint64_t(m2), Bb);
+            }
+            mprd(Ai, Ai, uint64_t(m1), Ai);
+            mprd(Bi, Bi, uint64_t(m2), Bi);
+        }
+        const uint64_t X[3] = {uint64_t(xn), uint64_t(xn1), uint64_t(xn2)};
+        const uint64_t Y[3] = {uint64_t(yn), uint64_t(yn1), uint64_t(yn2)};
+        double nx[3], ny[3];
+        for (int j = 0; j < 3; ++j) {
+            uint64_t sx = 0, sy = 0;
+            for (int q = 0; q < 3; ++q) {
+                sx = (sx + (Ab[j][q] * X[q]) % uint64_t(m1)) % uint64_t(m1);
+                sy = (sy + (Bb[j][q] * Y[q]) % uint64_t(m2)) % uint64_t(m2);
+            }
+            nx[j] = double(sx);
+            ny[j] = double(sy);
+        }
+        xn = nx[0]; xn1 = nx[1]; xn2 = nx[2];
+        yn = ny[0]; yn1 = ny[1]; yn2 = ny[2];
+    }
+    // mrg32k3a.h:187-236 for an even path index (the reference's batches
+    // start at multiples of 64): skip b / 2 blocks of dim numbers
+    void skipTo(uint64_t b)
+    {
+        reset();
+        if (b & 1) throw std::runtime_error("mrg32k3a skip to an odd path");
+        skipNumbers(b / 2 * dim);
+    }
+};
+
 // utility.h:11-79
 static vector<double> fillData(const vector<double>& original, double maxDx, double minDx,
                                const double* add, size_t nAdd)
@@ -735,22 +836,47 @@ void or_inv_normal(long n, const double* u, double* out)
 
 // mcSimul (mcBase.h:267-308) over paths [first, first + n): per-path payoffs
 // out[p * nPay + k].
-int or_value(const or_model* m, const or_product* p, long first, long n, double* out)
+// Gaussians of consecutive paths: Sobol (path p uses point p + 1), or
+// mrg32k3a with seeds (use_mrg != 0).
+struct Gauss {
+    orc::Sobol sobol;
+    orc::Mrg mrg;
+    bool use_mrg;
+    Gauss(int dim, long first, int use, unsigned s1, unsigned s2) : mrg(s1, s2), use_mrg(use != 0)
+    {
+        if (use_mrg) {
+            mrg.init(size_t(dim));
+            mrg.skipTo(uint64_t(first));
+        } else {
+            sobol.init(dim);
+            sobol.skipTo(uint32_t(first));
+        }
+    }
+    void next(std::vector<double>& g)
+    {
+        if (use_mrg) {
+            mrg.nextG(g);
+            return;
+        }
+        sobol.next();
+        for (size_t k = 0; k < g.size(); ++k) g[k] = orc::invNormalCdf(orc::kOneOver2Pow32 * sobol.state[k]);
+    }
+};
+
+int or_value_rng(const or_model* m, const or_product* p, long first, long n, int use_mrg, unsigned seed1,
+                 unsigned seed2, double* out)
 {
     try {
         orc::ProductDef d = orc::makeProduct(*p);
         auto md = orc::makeModel<double>(*m);
         md.allocate(d);
         md.init();
-        orc::Sobol rng;
-        rng.init(md.simDim());
-        rng.skipTo(uint32_t(first));
+        Gauss rng(md.simDim(), first, use_mrg, seed1, seed2);
         std::vector<double> g(md.simDim()), pay(d.nPay);
         std::vector<orc::Sample<double>> path;
         orc::allocateAndInit(d.defline, path);
         for (long i = 0; i < n; ++i) {
-            rng.next();
-            for (int k = 0; k < md.simDim(); ++k) g[k] = orc::invNormalCdf(orc::kOneOver2Pow32 * rng.state[k]);
+            rng.next(g);
             md.generatePath(g, path);
             orc::payoffs(d, path, pay);
             std::memcpy(out + (size_t)i * d.nPay, pay.data(), d.nPay * sizeof(double));
@@ -759,13 +885,18 @@ int or_value(const or_model* m, const or_product* p, long first, long n, double*
     } catch (const std::exception& e) { g_err = e.what(); return -1; }
 }
 
+int or_value(const or_model* m, const or_product* p, long first, long n, double* out)
+{
+    return or_value_rng(m, p, first, n, 0, 0, 0, out);
+}
+
 // mcSimulAAD (mcBase.h:427-532): the model initialisation is recorded once
 // before the mark, every path is recorded after it, swept back to the mark
 // and rewound; the initialisation is swept once at the end.  The aggregate is
 // payoff[risk_index] (main.h:135) when weights is NULL, otherwise the inner
 // product with weights (main.h:210-213).  risks are averaged over paths.
-int or_aad(const or_model* m, const or_product* p, long n, int risk_index, const double* weights,
-           double* payoffs_out, double* agg_out, double* risks_out)
+int or_aad_rng(const or_model* m, const or_product* p, long n, int risk_index, const double* weights, int use_mrg,
+               unsigned seed1, unsigned seed2, double* payoffs_out, double* agg_out, double* risks_out)
 {
     using orc::Var;
     try {
@@ -779,14 +910,12 @@ int or_aad(const or_model* m, const or_product* p, long n, int risk_index, const
         std::vector<orc::Sample<Var>> path;
         orc::allocateAndInit(d.defline, path);
         orc::g_tape.mark();
-        orc::Sobol rng;
-        rng.init(md.simDim());
+        Gauss rng(md.simDim(), 0, use_mrg, seed1, seed2);
         std::vector<double> g(md.simDim());
         std::vector<Var> pay(d.nPay);
         for (long i = 0; i < n; ++i) {
             orc::g_tape.rewindToMark();
-            rng.next();
-            for (int k = 0; k < md.simDim(); ++k) g[k] = orc::invNormalCdf(orc::kOneOver2Pow32 * rng.state[k]);
+            rng.next(g);
             md.generatePath(g, path);
             orc::payoffs(d, path, pay);
             Var res;
@@ -808,6 +937,12 @@ int or_aad(const or_model* m, const or_product* p, long n, int risk_index, const
         orc::g_tape.clear();
         return 0;
     } catch (const std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int or_aad(const or_model* m, const or_product* p, long n, int risk_index, const double* weights,
+           double* payoffs_out, double* agg_out, double* risks_out)
+{
+    return or_aad_rng(m, p, n, risk_index, weights, 0, 0, 0, payoffs_out, agg_out, risks_out);
 }
 
 }  // extern "C"

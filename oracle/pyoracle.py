@@ -60,10 +60,10 @@ def lib():
         L.or_sobol_dirs.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint)]
         L.or_sobol_states.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_int, ctypes.POINTER(ctypes.c_uint)]
         L.or_inv_normal.argtypes = [ctypes.c_long, _dp, _dp]
-        L.or_value.argtypes = [ctypes.POINTER(or_model), ctypes.POINTER(or_product), ctypes.c_long, ctypes.c_long,
-                               _dp]
-        L.or_aad.argtypes = [ctypes.POINTER(or_model), ctypes.POINTER(or_product), ctypes.c_long, ctypes.c_int,
-                             _dp, _dp, _dp, _dp]
+        L.or_value_rng.argtypes = [ctypes.POINTER(or_model), ctypes.POINTER(or_product), ctypes.c_long,
+                                   ctypes.c_long, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, _dp]
+        L.or_aad_rng.argtypes = [ctypes.POINTER(or_model), ctypes.POINTER(or_product), ctypes.c_long, ctypes.c_int,
+                                 _dp, ctypes.c_int, ctypes.c_uint, ctypes.c_uint, _dp, _dp, _dp]
         L.or_last_error.restype = ctypes.c_char_p
         _LIB = L
     return _LIB
@@ -196,24 +196,28 @@ def inv_normal(u):
     return out
 
 
-def value_paths(mdl, prd, n, first=0):
-    """Per-path payoffs [n][nPay] of paths first .. first + n - 1 (mcSimul)."""
+def value_paths(mdl, prd, n, first=0, mrg=None):
+    """Per-path payoffs [n][nPay] of paths first .. first + n - 1 (mcSimul);
+    Sobol, or mrg32k3a with mrg = (seed1, seed2) (even first path)."""
     npay = num_payoffs(prd)
     out = np.zeros(n * npay)
-    _check(lib().or_value(ctypes.byref(mdl.d), ctypes.byref(prd.d), first, n, out.ctypes.data_as(_dp)))
+    s1, s2 = mrg if mrg else (0, 0)
+    _check(lib().or_value_rng(ctypes.byref(mdl.d), ctypes.byref(prd.d), first, n, int(mrg is not None), s1, s2,
+                              out.ctypes.data_as(_dp)))
     return out.reshape(n, npay)
 
 
-def aad(mdl, prd, n, risk_index=0, weights=None):
+def aad(mdl, prd, n, risk_index=0, weights=None, mrg=None):
     """mcSimulAAD over n paths: (payoffs [n][nPay], aggregated [n], risks [nParam])."""
     npay, npar = num_payoffs(prd), num_params(mdl)
     pay = np.zeros(n * npay)
     agg = np.zeros(n)
     risks = np.zeros(npar)
     w = _arr(weights) if weights is not None else None
-    _check(lib().or_aad(ctypes.byref(mdl.d), ctypes.byref(prd.d), n, risk_index,
-                        w.ctypes.data_as(_dp) if w is not None else None, pay.ctypes.data_as(_dp),
-                        agg.ctypes.data_as(_dp), risks.ctypes.data_as(_dp)))
+    s1, s2 = mrg if mrg else (0, 0)
+    _check(lib().or_aad_rng(ctypes.byref(mdl.d), ctypes.byref(prd.d), n, risk_index,
+                            w.ctypes.data_as(_dp) if w is not None else None, int(mrg is not None), s1, s2,
+                            pay.ctypes.data_as(_dp), agg.ctypes.data_as(_dp), risks.ctypes.data_as(_dp)))
     return pay.reshape(n, npay), agg, risks
 
 
@@ -230,12 +234,17 @@ def ref_run(args, stdin=""):
     return json.loads(res.stdout)
 
 
-def ref_value(mdl, prd, n, threads=1, repeats=1, n_out=0):
-    return ref_run(["value", n, threads, repeats, n_out], mdl.spec() + "\n" + prd.spec() + "\n")
-
-
-def ref_aad(mdl, prd, n, risk_index=-1, threads=1, repeats=1, notionals=None):
+def ref_value(mdl, prd, n, threads=1, repeats=1, n_out=0, mrg=None):
     spec = mdl.spec() + "\n" + prd.spec() + "\n"
+    if mrg:
+        spec += "rng mrg %d %d\n" % tuple(mrg)
+    return ref_run(["value", n, threads, repeats, n_out], spec)
+
+
+def ref_aad(mdl, prd, n, risk_index=-1, threads=1, repeats=1, notionals=None, mrg=None):
+    spec = mdl.spec() + "\n" + prd.spec() + "\n"
+    if mrg:
+        spec += "rng mrg %d %d\n" % tuple(mrg)
     if notionals:
         spec += "notionals %d %s\n" % (len(notionals), " ".join("%d %r" % (i, w) for i, w in notionals.items()))
     return ref_run(["aad", n, risk_index, threads, repeats, 0], spec)

@@ -57,6 +57,8 @@ static void printVec(const char* name, const std::vector<double>& v, bool comma 
 struct Spec {
     std::vector<std::pair<std::string, double>> notionals;
     bool aggregate = false;
+    bool mrg = false;            // rng mrg <seed1> <seed2>: mrg32k3a instead of Sobol
+    int seed1 = 0, seed2 = 0;
 };
 
 static Spec readSpec()
@@ -101,6 +103,11 @@ static Spec readSpec()
                 auto strikes = readVec(std::cin, n);
                 putEuropeans(mats, strikes, "p");
             } else fail("unknown product " + kind);
+        } else if (tok == "rng") {
+            std::string kind;
+            std::cin >> kind >> spec.seed1 >> spec.seed2;
+            if (kind != "mrg") fail("unknown rng " + kind);
+            spec.mrg = true;
         } else if (tok == "notionals") {
             size_t n;
             std::cin >> n;
@@ -200,7 +207,9 @@ int main(int argc, char** argv)
         Spec spec = readSpec();
         NumericalParam num;
         num.parallel = threads > 1;
-        num.useSobol = true;
+        num.useSobol = !spec.mrg;
+        num.seed1 = spec.seed1;
+        num.seed2 = spec.seed2;
         num.numPath = nPath;
         if (threads > 1) ThreadPool::getInstance()->start(threads - 1);
 
@@ -217,8 +226,10 @@ int main(int argc, char** argv)
             // first nOut per-path payoffs, via the serial simulator on the same
             // Sobol sequence (mcBase.h:267-308; identical paths to the parallel one)
             if (nOut > 0) {
-                Sobol rng;
-                auto mat = mcSimul(*getProduct<double>("p"), *getModel<double>("m"), rng, nOut);
+                std::unique_ptr<RNG> rng;
+                if (spec.mrg) rng = std::make_unique<mrg32k3a>(spec.seed1, spec.seed2);
+                else rng = std::make_unique<Sobol>();
+                auto mat = mcSimul(*getProduct<double>("p"), *getModel<double>("m"), *rng, nOut);
                 std::vector<double> flat;
                 for (auto& row : mat) flat.insert(flat.end(), row.begin(), row.end());
                 printVec("per_path", flat);

@@ -161,16 +161,6 @@ def test_store_errors_follow_reference_messages():
     cf.clearStore()
 
 
-def test_mrg32k3a_is_reported_unsupported():
-    cf.clearStore()
-    cf.putBlackScholes(100.0, 0.2, False, 0.0, 0.0, "bs")
-    cf.putEuropean(100.0, 1.0, 1.0, "eu")
-    with pytest.raises(cf.CfError, match="Sobol") as e:
-        cf.value("bs", "eu", cf.NumericalParam(numPath=64, useSobol=False))
-    assert e.value.code == -3
-    cf.clearStore()
-
-
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")
 def test_simulation_without_gpu_fails_loudly():
     mdl, prd = cf.BlackScholes(100.0, 0.2), cf.European(100.0, 1.0)
