@@ -621,7 +621,8 @@ def dupireAADRisk(modelId, productId, notionals, num):
 
 
 class cf_server_opts(ctypes.Structure):
-    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_int64), ("auto_recompute", ctypes.c_int)]
+    _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_int64), ("auto_recompute", ctypes.c_int),
+                ("graphs", ctypes.c_int)]
 
 
 class cf_request(ctypes.Structure):
@@ -643,7 +644,7 @@ class cf_result(ctypes.Structure):
 class cf_server_stats(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int64) for f in (
         "submitted", "completed", "failed", "queued", "running", "peak_running", "admitted_busy", "memory_waits",
-        "recompute_switches", "arena_bytes", "peak_arena_bytes")]
+        "recompute_switches", "arena_bytes", "peak_arena_bytes", "graph_captures", "graph_replays")]
 
 
 def _server_lib():
@@ -667,11 +668,13 @@ def _server_lib():
 class Server:
     """Request server on the current GPU: `slots` requests on the device at
     once, device blocks from an arena of `hbmBudget` bytes; AAD requests that
-    do not fit switch to the recompute variant when `autoRecompute`."""
+    do not fit switch to the recompute variant when `autoRecompute`; with
+    `graphs`, repeated requests replay a captured HIP graph of their launch
+    sequence."""
 
-    def __init__(self, slots=4, hbmBudget=16 << 30, autoRecompute=True):
+    def __init__(self, slots=4, hbmBudget=16 << 30, autoRecompute=True, graphs=True):
         L = _server_lib()
-        o = cf_server_opts(int(slots), int(hbmBudget), int(bool(autoRecompute)))
+        o = cf_server_opts(int(slots), int(hbmBudget), int(bool(autoRecompute)), int(bool(graphs)))
         h = ctypes.c_void_p()
         _check(L.cf_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h

@@ -1001,7 +1001,8 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     const ModelPlan& md = PR.md;
     const bool dup = md.kind == CF_MODEL_DUPIRE;
     const unsigned char* dt = B.tables;
-    cfk::LaunchParams P{};
+    cfk::LaunchParams P;
+    std::memset(&P, 0, sizeof P);   // padding too: sequences compare byte for byte
     P.first_path = S.first;
     P.n_path = S.n_path;
     P.n_steps = md.n_steps;
@@ -1054,33 +1055,73 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     return P;
 }
 
+void make_sequence(const Prepared& PR, const CallShape& S, const CallBuffers& B, const cfk::LaunchParams& P,
+                   Sequence* q)
+{
+    std::memset(q, 0, sizeof *q);
+    q->cfg = PR.cfg;
+    q->cfg.normals = S.mrg;
+    std::memcpy(&q->P, &P, sizeof P);
+    q->grid = S.grid;
+    q->block = PR.block;
+    q->smem = PR.smem;
+    if (S.mrg) {
+        q->mrg = 1;
+        q->jump = reinterpret_cast<const uint64_t*>(B.tables + PR.o_mrg);
+        q->seed1 = S.seed1;
+        q->seed2 = S.seed2;
+        q->n_steps = PR.md.n_steps;
+        q->real_bytes = S.fp64 ? 8 : 4;
+        q->first = S.first;
+        q->n_path = S.n_path;
+        q->normals = B.normals;
+    }
+    q->block_out = B.block_out;
+    q->stride = S.stride;
+    q->reduced = B.reduced;
+    if (S.ivn) {
+        q->ivn = int64_t(S.ivn);
+        q->ivbar_blocks = B.ivbar_blocks;
+        q->ivbar = B.ivbar;
+    }
+}
+
+void ready_sequence(const Sequence& q)
+{
+    // the plan's kernel was set up at planning; the normals-buffer variant
+    // of it is set up on first use (cached by the launcher)
+    if (!q.cfg.normals) return;
+    int per_cu = 0;
+    const int rc = cfk::path_kernel_occupancy(q.cfg, q.block, q.smem, &per_cu);
+    if (rc != 0) hip_check(hipError_t(rc), "occupancy");
+}
+
+void enqueue_sequence(const Sequence& q, hipStream_t stream, hipEvent_t ev_kernel0, hipEvent_t ev_kernel1)
+{
+    if (q.mrg) {
+        const int rc = cfk::launch_mrg_normals(q.jump, q.seed1, q.seed2, q.n_steps, q.first, q.n_path, q.real_bytes,
+                                               q.normals, stream);
+        if (rc != 0) hip_check(hipError_t(rc), "normals launch");
+    }
+    if (ev_kernel0) hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
+    int rc = cfk::launch_path_kernel(q.cfg, q.P, q.grid, q.block, q.smem, stream);
+    if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
+    if (ev_kernel1) hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
+    rc = cfk::launch_reduce_rows(q.block_out, q.grid, q.stride, q.reduced, stream);
+    if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
+    if (q.ivn) {
+        rc = cfk::launch_reduce_rows(q.ivbar_blocks, q.grid, int(q.ivn), q.ivbar, stream);
+        if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
+    }
+}
+
 void enqueue_call(const Prepared& PR, const CallShape& S, const CallBuffers& B, const cfk::LaunchParams& P,
                   hipStream_t stream, hipEvent_t ev_kernel0, hipEvent_t ev_kernel1)
 {
-    if (S.mrg) {
-        const uint64_t* jump = reinterpret_cast<const uint64_t*>(B.tables + PR.o_mrg);
-        const int rc = cfk::launch_mrg_normals(jump, S.seed1, S.seed2, PR.md.n_steps, S.first, S.n_path,
-                                               S.fp64 ? 8 : 4, B.normals, stream);
-        if (rc != 0) hip_check(hipError_t(rc), "normals launch");
-    }
-    cfk::KernelConfig cfg = PR.cfg;
-    cfg.normals = S.mrg;
-    if (S.mrg) {
-        // the normals-buffer kernel's LDS limit is set once, like the plan's kernel
-        int per_cu = 0;
-        const int orc = cfk::path_kernel_occupancy(cfg, PR.block, PR.smem, &per_cu);
-        if (orc != 0) hip_check(hipError_t(orc), "occupancy");
-    }
-    if (ev_kernel0) hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
-    int rc = cfk::launch_path_kernel(cfg, P, S.grid, PR.block, PR.smem, stream);
-    if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
-    if (ev_kernel1) hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
-    rc = cfk::launch_reduce_rows(B.block_out, S.grid, S.stride, B.reduced, stream);
-    if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
-    if (S.ivn) {
-        rc = cfk::launch_reduce_rows(B.ivbar_blocks, S.grid, int(S.ivn), B.ivbar, stream);
-        if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
-    }
+    Sequence q;
+    make_sequence(PR, S, B, P, &q);
+    ready_sequence(q);
+    enqueue_sequence(q, stream, ev_kernel0, ev_kernel1);
 }
 
 void collect_results(const Prepared& PR, const CallShape& S, const double* red, const double* ivbar, const Call& c)

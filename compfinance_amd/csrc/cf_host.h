@@ -291,8 +291,39 @@ CallBytes call_bytes(const CallShape& S);
 
 cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallShape& S, const CallBuffers& B);
 
-// Path kernel then cross-block reductions on `stream`; ev_kernel brackets the
-// path kernel (either may be null).
+// The device work of one call as plain values: the normals kernel
+// (mrg32k3a), the path kernel and the cross-block reductions.  Enqueueing
+// reads nothing else, so two calls whose sequences are equal byte for byte
+// enqueue the same work (the server replays a captured graph on that
+// equality).  Built zero-filled, padding included.
+struct Sequence {
+    cfk::KernelConfig cfg;
+    cfk::LaunchParams P;
+    int grid, block;
+    size_t smem;
+    int mrg;                       // 1: generate the normals first
+    const uint64_t* jump;          // mrg32k3a jump matrices (device)
+    uint64_t seed1, seed2;
+    int n_steps, real_bytes;
+    int64_t first, n_path;
+    void* normals;
+    double* block_out;             // [grid][stride]
+    int stride;
+    double* reduced;               // [stride]
+    int64_t ivn;
+    double* ivbar_blocks;          // [grid][ivn]
+    double* ivbar;                 // [ivn]
+};
+void make_sequence(const Prepared& PR, const CallShape& S, const CallBuffers& B, const cfk::LaunchParams& P,
+                   Sequence* q);
+// Host-side set-up of the sequence's kernels (LDS limits); not a stream
+// operation, so it runs before a capture.
+void ready_sequence(const Sequence& q);
+// Enqueues the sequence on `stream`; ev_kernel brackets the path kernel
+// (either may be null).
+void enqueue_sequence(const Sequence& q, hipStream_t stream, hipEvent_t ev_kernel0, hipEvent_t ev_kernel1);
+
+// make_sequence + ready_sequence + enqueue_sequence.
 void enqueue_call(const Prepared& PR, const CallShape& S, const CallBuffers& B, const cfk::LaunchParams& P,
                   hipStream_t stream, hipEvent_t ev_kernel0, hipEvent_t ev_kernel1);
 

@@ -15,6 +15,12 @@
 //     variant does not fit runs the recompute variant when that fits;
 //     requests that arrive while others run start at once, and the kernels
 //     of different slots overlap on the device;
+//   * steady state: when a slot sees the same launch sequence a second time
+//     (the exact bytes of every operation's parameters: same plan, shape,
+//     arena block and staging), it captures the uploads, kernels and result
+//     copies in a HIP graph; later requests with that sequence replay the
+//     graph in one launch.  One-off requests launch directly, so a mix of
+//     distinct requests pays no instantiation;
 //   * completion: a host function queued on the slot's stream after the
 //     result copies signals the dispatcher, which forms the results, frees
 //     the block and the slot and admits more work.
@@ -159,6 +165,18 @@ struct Request {
 
 struct Server;
 
+// An instantiated graph of one launch sequence and the bytes that name it.
+struct GraphEntry {
+    vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    uint64_t last_use = 0;
+};
+
+// Instantiated graphs kept per slot; the least recently used goes first.
+constexpr size_t kGraphsPerSlot = 8;
+// Sequences launched directly that a slot remembers, to capture on a repeat.
+constexpr size_t kSeenPerSlot = 32;
+
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -167,6 +185,9 @@ struct Slot {
     Clock::time_point t_end;
     Server* srv = nullptr;
     int index = 0;
+    vector<GraphEntry> graphs;   // used only by the dispatcher while the slot is idle
+    std::deque<vector<unsigned char>> seen;
+    uint64_t tick = 0;
 };
 
 // Requests passed over for lack of room at most this many times; then the
@@ -177,6 +198,7 @@ struct Server {
     Engine* E = nullptr;
     int device = 0;
     bool auto_recompute = true;
+    bool graphs = true;
     Arena arena;
     vector<std::unique_ptr<Slot>> slots;
     std::mutex mu;
@@ -210,6 +232,80 @@ size_t block_bytes(const Prepared& PR, const CallShape& S)
     return ((PR.staged.size() + 255) & ~size_t(255)) + call_bytes(S).total();
 }
 
+// Host-device copies of a request around its sequence.
+struct Copies {
+    void* tables_dst;
+    const void* tables_src;
+    size_t tables_bytes;
+    void* weights_dst;
+    const void* weights_src;
+    size_t weights_bytes;
+    void* out_dst;
+    const void* out_src;
+    size_t out_bytes;
+    void* ivbar_dst;
+    const void* ivbar_src;
+    size_t ivbar_bytes;
+};
+
+void enqueue_request(const Sequence& q, const Copies& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    hip_check(hipMemcpyAsync(c.tables_dst, c.tables_src, c.tables_bytes, hipMemcpyHostToDevice, st), "upload");
+    if (c.weights_bytes)
+        hip_check(hipMemcpyAsync(c.weights_dst, c.weights_src, c.weights_bytes, hipMemcpyHostToDevice, st), "upload");
+    enqueue_sequence(q, st, ev0, ev1);
+    hip_check(hipMemcpyAsync(c.out_dst, c.out_src, c.out_bytes, hipMemcpyDeviceToHost, st), "download");
+    if (c.ivbar_bytes)
+        hip_check(hipMemcpyAsync(c.ivbar_dst, c.ivbar_src, c.ivbar_bytes, hipMemcpyDeviceToHost, st), "download");
+}
+
+// The slot's graph of (q, c): found by its bytes; else, on the second
+// sighting of the sequence, captured from the same enqueue as a direct
+// launch and instantiated (*captured set); else null: launch directly.
+hipGraphExec_t slot_graph(Slot& s, const Sequence& q, const Copies& c, bool* captured)
+{
+    vector<unsigned char> key(sizeof q + sizeof c);
+    std::memcpy(key.data(), &q, sizeof q);
+    std::memcpy(key.data() + sizeof q, &c, sizeof c);
+    ++s.tick;
+    *captured = false;
+    for (GraphEntry& g : s.graphs)
+        if (g.key == key) {
+            g.last_use = s.tick;
+            return g.exec;
+        }
+    auto seen = std::find(s.seen.begin(), s.seen.end(), key);
+    if (seen == s.seen.end()) {
+        s.seen.push_back(std::move(key));
+        if (s.seen.size() > kSeenPerSlot) s.seen.pop_front();
+        return nullptr;
+    }
+    s.seen.erase(seen);
+    hipGraph_t graph = nullptr;
+    hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    try {
+        enqueue_request(q, c, s.stream, nullptr, nullptr);
+    } catch (...) {
+        if (hipStreamEndCapture(s.stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(s.stream, &graph), "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    hip_check(e, "hipGraphInstantiate");
+    if (s.graphs.size() >= kGraphsPerSlot) {
+        // the slot is idle, so none of its graphs is executing
+        auto lru = std::min_element(s.graphs.begin(), s.graphs.end(),
+                                    [](const GraphEntry& a, const GraphEntry& b) { return a.last_use < b.last_use; });
+        (void)hipGraphExecDestroy(lru->exec);
+        s.graphs.erase(lru);
+    }
+    s.graphs.push_back(GraphEntry{std::move(key), exec, s.tick});
+    *captured = true;
+    return exec;
+}
+
 // Launches request r in slot s from its arena block (dispatcher thread, no
 // server lock held).
 void launch(Server& S, Slot& s, Request& r)
@@ -237,17 +333,43 @@ void launch(Server& S, Slot& s, Request& r)
     const size_t tb = PR.staged.size();
     unsigned char* hin = static_cast<unsigned char*>(s.pin_in.ensure(tb + nb.weights + 8));
     std::memcpy(hin, PR.staged.data(), tb);
-    hip_check(hipMemcpyAsync(tables, hin, tb, hipMemcpyHostToDevice, s.stream), "upload");
-    if (r.aad) {
-        std::memcpy(hin + tb, r.weights.data(), nb.weights);
-        hip_check(hipMemcpyAsync(B.weights, hin + tb, nb.weights, hipMemcpyHostToDevice, s.stream), "upload");
-    }
+    if (r.aad) std::memcpy(hin + tb, r.weights.data(), nb.weights);
     double* hout = static_cast<double*>(s.pin_out.ensure((size_t(SH.stride) + SH.ivn) * 8));
+    Copies c;
+    std::memset(&c, 0, sizeof c);
+    c.tables_dst = tables;
+    c.tables_src = hin;
+    c.tables_bytes = tb;
+    if (r.aad) {
+        c.weights_dst = B.weights;
+        c.weights_src = hin + tb;
+        c.weights_bytes = nb.weights;
+    }
+    c.out_dst = hout;
+    c.out_src = B.reduced;
+    c.out_bytes = size_t(SH.stride) * 8;
+    if (SH.ivn) {
+        c.ivbar_dst = hout + SH.stride;
+        c.ivbar_src = B.ivbar;
+        c.ivbar_bytes = SH.ivn * 8;
+    }
     const cfk::LaunchParams P = launch_params(*S.E, PR, SH, B);
-    enqueue_call(PR, SH, B, P, s.stream, s.ev0, s.ev1);
-    hip_check(hipMemcpyAsync(hout, B.reduced, size_t(SH.stride) * 8, hipMemcpyDeviceToHost, s.stream), "download");
-    if (SH.ivn)
-        hip_check(hipMemcpyAsync(hout + SH.stride, B.ivbar, SH.ivn * 8, hipMemcpyDeviceToHost, s.stream), "download");
+    Sequence q;
+    make_sequence(PR, SH, B, P, &q);
+    ready_sequence(q);
+    bool captured = false;
+    hipGraphExec_t exec = S.graphs ? slot_graph(s, q, c, &captured) : nullptr;
+    if (exec) {
+        {
+            std::lock_guard<std::mutex> lk(S.mu);
+            ++(captured ? S.st.graph_captures : S.st.graph_replays);
+        }
+        hip_check(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
+        hip_check(hipGraphLaunch(exec, s.stream), "hipGraphLaunch");
+        hip_check(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
+    } else {
+        enqueue_request(q, c, s.stream, s.ev0, s.ev1);
+    }
     hip_check(hipLaunchHostFunc(s.stream, on_slot_done, &s), "hipLaunchHostFunc");
 }
 
@@ -400,6 +522,7 @@ int cf_server_create(const cf_server_opts* opts, cf_server** out)
         S->E = &engine();
         S->device = S->E->device;
         S->auto_recompute = opts ? opts->auto_recompute != 0 : true;
+        S->graphs = opts ? opts->graphs != 0 : true;
         hip_check(hipSetDevice(S->device), "hipSetDevice");
         S->arena.init(size_t(budget));
         for (int i = 0; i < n_slots; ++i) {
@@ -549,6 +672,8 @@ int cf_server_destroy(cf_server* srv)
         if (S->thread.joinable()) S->thread.join();
         for (auto& s : S->slots) {
             (void)hipStreamSynchronize(s->stream);
+            for (GraphEntry& g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+            s->graphs.clear();
             (void)hipEventDestroy(s->ev0);
             (void)hipEventDestroy(s->ev1);
             (void)hipStreamDestroy(s->stream);

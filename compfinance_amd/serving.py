@@ -114,6 +114,8 @@ def summarize(reqs, results, wall, stats):
         "arena_bytes": int(stats["arena_bytes"]),
         "peak_arena_bytes": int(stats["peak_arena_bytes"]),
         "recomputed_requests": int(sum(r["recomputed"] for r in results)),
+        "graph_captures": int(stats["graph_captures"]),
+        "graph_replays": int(stats["graph_replays"]),
     }
 
 
@@ -126,7 +128,7 @@ def run_c5(n=512, rate=200.0, slots=4, budget=4 << 30, seed=20261204, warmup=16)
         base = srv.stats()
         results, wall = replay(reqs, srv)
         st = srv.stats()
-        for k in ("admitted_busy", "memory_waits", "recompute_switches"):
+        for k in ("admitted_busy", "memory_waits", "recompute_switches", "graph_captures", "graph_replays"):
             st[k] -= base[k]
     return summarize(reqs, results, wall, st)
 

@@ -272,6 +272,11 @@ typedef struct cf_server_opts {
     int slots;               /* requests on the device at once, 1..16 */
     int64_t hbm_budget;      /* device bytes of the arena */
     int auto_recompute;      /* 1: switch AAD to the recompute variant to fit */
+    int graphs;              /* 1: a slot that launches the same sequence a
+                              * second time (same kernels, parameters,
+                              * addresses and sizes) captures it in a HIP graph
+                              * and replays the graph for later repeats;
+                              * 0: direct launches only */
 } cf_server_opts;
 
 typedef struct cf_request {
@@ -297,7 +302,8 @@ typedef struct cf_result {
     double submit_ms;        /* submission time since the server's creation */
     double queue_ms;         /* submission to launch */
     double latency_ms;       /* submission to results on the host */
-    double kernel_ms;        /* path kernel (HIP events) */
+    double kernel_ms;        /* path kernel (HIP events); with graphs, the
+                              * request's whole device sequence */
     int64_t device_bytes;    /* arena bytes held while on the device */
     int recomputed;          /* 1 if admitted as the recompute variant */
 } cf_result;
@@ -310,9 +316,13 @@ typedef struct cf_server_stats {
     int64_t memory_waits;         /* admission passes that found no room */
     int64_t recompute_switches;   /* AAD requests switched to recompute */
     int64_t arena_bytes, peak_arena_bytes;
+    int64_t graph_captures;       /* launch sequences captured and instantiated
+                                   * (and launched) */
+    int64_t graph_replays;        /* launches that replayed a captured graph */
 } cf_server_stats;
 
-/* opts NULL: 4 slots, 16 GiB, auto_recompute on.  Uses the current device. */
+/* opts NULL: 4 slots, 16 GiB, auto_recompute on, graphs on.  Uses the
+ * current device. */
 int cf_server_create(const cf_server_opts* opts, cf_server** out);
 /* Validates and plans the request on the calling thread, then queues it. */
 int cf_server_submit(cf_server* srv, const cf_request* req, int64_t* id);

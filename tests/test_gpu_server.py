@@ -91,3 +91,60 @@ def test_open_loop_replay_small(engine):
     s = serving.summarize(reqs, res, wall, st)
     assert s["requests"] == 40 and st["failed"] == 0
     assert 0 < s["p50_ms"] <= s["p99_ms"]
+
+
+def test_repeated_requests_replay_a_captured_graph(engine):
+    """One slot, the same two requests four times each, one at a time: each
+    kind launches directly once, is captured at its repeat and replayed
+    after; results equal the direct launches of a server without graphs and
+    the synchronous entry points."""
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    num = cf.NumericalParam(numPath=1 << 16, precision=cf.FP32)
+    bs, eu = cf.BlackScholes(100.0, 0.2), cf.European(100.0, 1.0)
+    kinds = [(prd, mdl, num, True), (eu, bs, cf.NumericalParam(numPath=1 << 14), False)]
+    out = {}
+    for graphs in (True, False):
+        with cf.Server(slots=1, hbmBudget=1 << 30, graphs=graphs) as srv:
+            out[graphs] = [srv.wait(srv.submit(p, m, n, aad=a)) for _ in range(4) for p, m, n, a in kinds]
+            st = srv.stats()
+        if graphs:
+            assert st["graph_captures"] == 2 and st["graph_replays"] == 4, st
+        else:
+            assert st["graph_captures"] == 0 and st["graph_replays"] == 0, st
+    ref_aad = cf.mcParallelSimulAAD(prd, mdl, num=num)
+    ref_val = cf.mcParallelSimul(eu, bs, num=kinds[1][2], per_path=False)
+    for k, (g, d) in enumerate(zip(out[True], out[False])):
+        np.testing.assert_array_equal(g["means"], d["means"])
+        if k % 2 == 0:
+            np.testing.assert_array_equal(g["means"], ref_aad.payoff_means)
+            np.testing.assert_allclose(g["risks"], ref_aad.risks, rtol=1e-12,
+                                       atol=1e-12 * np.abs(ref_aad.risks).max())
+            np.testing.assert_allclose(g["risks"], d["risks"], rtol=1e-12, atol=1e-12 * np.abs(d["risks"]).max())
+        else:
+            np.testing.assert_array_equal(g["means"], ref_val.means)
+
+
+def test_graphs_with_changing_seeds_and_weights(engine):
+    """An mrg32k3a request with other seeds is another sequence; AAD
+    requests that differ only in their weights share one graph and follow
+    the weights."""
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    with cf.Server(slots=1, hbmBudget=1 << 30) as srv:
+        seeds = [(12345, 1234), (12345, 1234), (777, 888), (12345, 1234)]
+        mrg = [srv.wait(srv.submit(prd, mdl, cf.NumericalParam(numPath=1 << 14, useSobol=False, seed1=a, seed2=b)))
+               for a, b in seeds]
+        st0 = srv.stats()
+        num = cf.NumericalParam(numPath=1 << 14, precision=cf.FP32)
+        ws = [np.array([1.0, 0.0]), np.array([0.0, 1.0]), np.array([0.5, 2.0]), np.array([1.0, 0.0])]
+        risk = [srv.wait(srv.submit(prd, mdl, num, aad=True, weights=w)) for w in ws]
+        st1 = srv.stats()
+    assert st0["graph_captures"] == 1 and st0["graph_replays"] == 1, st0
+    assert st1["graph_captures"] == 2 and st1["graph_replays"] == 3, st1
+    for (a, b), g in zip(seeds, mrg):
+        ref = cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=1 << 14, useSobol=False, seed1=a, seed2=b),
+                                 per_path=False)
+        np.testing.assert_array_equal(g["means"], ref.means)
+    for w, g in zip(ws, risk):
+        ref = cf.mcParallelSimulAAD(prd, mdl, weights=w, num=num)
+        assert abs(g["agg_mean"] - ref.agg_mean) <= 1e-12 * abs(ref.agg_mean)
+        np.testing.assert_allclose(g["risks"], ref.risks, rtol=1e-12, atol=1e-12 * np.abs(ref.risks).max())
