@@ -14,9 +14,12 @@ latency).  Inputs are synthetic (the local-vol formula of the
 configuration); all tables are resident on the device before timing.
 
 Prints one JSON line.  `value` is paths*steps per second of the FP32 AAD
-call; `roofline` is for the AAD path kernel (vector ALU bound); the CPU
-baseline times the reference engine (oracle/_ref) on the host cores, or the
-FP64 CPU restatement when the reference build is absent.
+call; `roofline` is the AAD path kernel's HBM rate: its algorithmic bytes
+(checkpointed states and knot brackets written and read back, per-block
+local-vol adjoint grids) over its HIP-event duration, against 8 TB/s; the
+same kernel's census FLOP rate against the FP32 vector peak is reported
+beside it.  The CPU baseline times the reference engine (oracle/_ref) on the
+host cores, or the FP64 CPU restatement when the reference build is absent.
 """
 import argparse
 import json
@@ -41,6 +44,11 @@ _BOOK_CLS = {"europeans": cf.Europeans, "barrier": cf.UOC, "asian": cf.Asian}
 FLOP_PER_PATH_STEP = {("aad", cf.FP32): 67.0, ("aad", cf.FP64): 115.0,
                       ("price", cf.FP32): 41.0, ("price", cf.FP64): 69.0}
 PEAK_TFLOPS = {cf.FP32: 157.3, cf.FP64: 78.6}   # MI355X vector peaks
+# Algorithmic HBM bytes of the checkpoint adjoint per path-step: the state
+# (4 B FP32 / 8 B FP64) and its knot-bracket byte, each written in the
+# forward sweep and read back in the reverse sweep.
+CKPT_BYTES_PER_PATH_STEP = {cf.FP32: 2 * (4 + 1), cf.FP64: 2 * (8 + 1)}
+HBM_PEAK_GBS = 8000.0                            # MI355X HBM3E
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_aad_dupire_barrier_fp32.json")
 
 
@@ -155,7 +163,11 @@ def main():
     path_steps = n3 * n_steps
     kavg = float(np.mean(k32))
     flop = FLOP_PER_PATH_STEP[("aad", cf.FP32)] * path_steps
-    achieved = flop / (kavg * 1e-3) / 1e12
+    tflops = flop / (kavg * 1e-3) / 1e12
+    # one FP64 local-vol adjoint grid (steps x spots) written per block
+    grid_bytes = st32["grid_blocks"] * n_steps * 30 * 8
+    alg_bytes = CKPT_BYTES_PER_PATH_STEP[cf.FP32] * path_steps + grid_bytes
+    gbs = alg_bytes / (kavg * 1e-3) / 1e9
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
@@ -203,10 +215,15 @@ def main():
                     "aggregate_fp32": float(b32.agg_mean), "aggregate_fp64": float(b64.agg_mean)},
         "aad_values": {"fp32": float(r32.agg_mean), "fp64": float(r64.agg_mean),
                        "delta_fp32": float(r32.risks[0]), "delta_fp64": float(r64.risks[0])},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[cf.FP32], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS[cf.FP32], "traffic": traffic,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "path_kernel<float, Dupire, barrier, checkpoint adjoint>", "kernel_ms": kavg,
-                     "flop_per_path_step": FLOP_PER_PATH_STEP[("aad", cf.FP32)], "path_steps": path_steps},
+                     "algorithmic_bytes": alg_bytes,
+                     "bytes_per_path_step": CKPT_BYTES_PER_PATH_STEP[cf.FP32],
+                     "adjoint_grid_bytes": grid_bytes, "path_steps": path_steps},
+        "valu_census": {"achieved": tflops, "peak": PEAK_TFLOPS[cf.FP32], "unit": "TFLOP/s",
+                        "frac": tflops / PEAK_TFLOPS[cf.FP32],
+                        "flop_per_path_step": FLOP_PER_PATH_STEP[("aad", cf.FP32)]},
     }
     if not args.no_serving:
         from compfinance_amd import serving

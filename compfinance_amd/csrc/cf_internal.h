@@ -62,10 +62,13 @@ struct BookLayout {
     int as_put;       // [n_asian] int
 };
 
-// Rows of 64 reals in a wave's checkpoint slot.
+// Rows of 64 reals in a wave's checkpoint slot.  The Dupire checkpoint
+// variant also keeps, per step and lane, one byte: the knot count of the
+// state's bracket (grids of up to kMaxStoredKnots knots).
+constexpr int kMaxStoredKnots = 255;
 CF_HD inline int ckpt_rows(int n_steps, bool dupire, bool recompute, int real_bytes)
 {
-    if (!recompute) return n_steps;
+    if (!recompute) return dupire ? n_steps + (n_steps + real_bytes - 1) / real_bytes : n_steps;
     if (!dupire) return 0;
     const int seg = real_bytes == 4 ? kSegF32 : kSegF64;
     return (n_steps + seg - 1) / seg;
@@ -129,7 +132,8 @@ struct LaunchParams {
 
     // adjoint
     const double* weights;       // [n_pay] aggregation weights
-    void* ckpt;                  // [slots][ckpt_rows][64] path states or normals
+    void* ckpt;                  // [slots][ckpt_rows][64] path states or normals,
+                                 // then (Dupire) [n_steps][64] bracket bytes
     int ivbar_copies;            // LDS local-vol adjoint grids per block: even and odd lanes (2) or one
     double* ivbar_blocks;        // [gridDim][n_steps][n_spots] local-vol adjoints
     double* per_path_agg;        // [n_path] or null

@@ -19,7 +19,8 @@
 //     accumulators, then per block, then across blocks by a separate kernel
 //     in fixed order;
 //   * adjoint: path states checkpointed in a per-wave HBM slot (reused by the
-//     wave for each block of paths), reverse sweep over the Euler steps,
+//     wave for each block of paths) with the knot bracket of each state in a
+//     byte, reverse sweep over the Euler steps,
 //     local-vol adjoints accumulated with LDS atomics on a per-block
 //     (step x spot) grid.
 // This header holds the device helpers and the path kernel template; each
@@ -205,26 +206,27 @@ __device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
     R bf = (x - s.x0) * s.invh;
     bf = bf > R(0) ? bf : R(0);
     bf = bf < R(s.nb - 1) ? bf : R(s.nb - 1);
-    int lo = s.table[int(bf)];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int c = lo < s.m ? lo : s.m - 1;
-        lo += (lo < s.m && s.knots[c] <= x) ? 1 : 0;
-    }
+    const int lo = s.table[int(bf)];
+    // the bucket's first two candidate knots, loaded together from clamped
+    // (always valid) indices; the knots increase, so each one <= x adds one
+    const int top = s.m - 1;
+    const R k0 = s.knots[lo < top ? lo : top];
+    const R k1 = s.knots[lo + 1 < top ? lo + 1 : top];
+    int c = lo + int((lo <= top) & (k0 <= x)) + int((lo + 1 <= top) & (k1 <= x));
     for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
-        const int c = lo < s.m ? lo : s.m - 1;
-        lo += (lo < s.m && s.knots[c] <= x) ? 1 : 0;
+        const R kc = s.knots[c < top ? c : top];
+        c += int((c <= top) & (kc <= x));
     }
-    return lo;
+    return c;
 }
 
 // Branch-free: every load is in range (nodes clamped), the extrapolation
-// cases are selected afterwards.
+// cases are selected afterwards.  lo = count_le(ks, x), the number of knots
+// <= x, found by the caller (the reverse sweep reloads it from the forward's).
 template <typename R>
-__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x)
+__device__ __forceinline__ Bracket<R> local_vol_at(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x, int lo)
 {
     const int m = ks.m;
-    const int lo = count_le(ks, x);   // number of knots <= x
     int j = lo - 1;
     j = j > m - 2 ? m - 2 : j;
     j = j < 0 ? 0 : j;
@@ -242,6 +244,12 @@ __device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R
     b.wa = left ? R(1) : (right ? R(0) : R(1) - t);
     b.wb = left ? R(0) : (right ? R(1) : t);
     return b;
+}
+
+template <typename R>
+__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x)
+{
+    return local_vol_at(ks, inv_dx, row, x, count_le(ks, x));
 }
 
 __device__ __forceinline__ double recip_mul(double a, double b) { return fast_div(a, b); }
@@ -384,6 +392,10 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     // (Black-Scholes); recompute variant: [ceil(n / kSeg)][64] Dupire states
     const int ck_rows = ckpt_rows(n, DUP, RECOMP, int(sizeof(R)));
     R* ck = AD ? static_cast<R*>(P.ckpt) + size_t(slot) * ck_rows * kWave : nullptr;
+    // bracket bytes after the states: the knot count of x_i at each step
+    constexpr bool CKB = DUP && ADJ == ADJ_CHECKPOINT;
+    const bool ck_lo = CKB && m <= kMaxStoredKnots;
+    uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave) : nullptr;
     KnotSearch<R> ks;
     ks.knots = knots;
     ks.table = bucket;
@@ -612,7 +624,10 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = x;
                 if constexpr (RECOMP)
                     if ((i & (kSeg - 1)) == 0) ck[(i / kSeg) * kWave + lane] = x;
-                const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, x);
+                const int lo = count_le(ks, x);
+                if constexpr (CKB)
+                    if (ck_lo) ckl[i * kWave + lane] = uint8_t(lo);
+                const Bracket<R> b = local_vol_at(ks, inv_dx, iv + i * m, x, lo);
                 x = euler_step(x, b.vol, g);
             } else {
                 if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
@@ -809,8 +824,13 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 // FP64 and the book hold more registers
                 constexpr int kPrefetch = sizeof(R) == 8 ? 1 : (BOOK ? 1 : 4);
                 R buf[kPrefetch];
+                int lbuf[kPrefetch];
 #pragma unroll
-                for (int k = 0; k < kPrefetch; ++k) buf[k] = n - 1 - k >= 0 ? ck[(n - 1 - k) * kWave + lane] : R(0);
+                for (int k = 0; k < kPrefetch; ++k) {
+                    const bool in = n - 1 - k >= 0;
+                    buf[k] = in ? ck[(n - 1 - k) * kWave + lane] : R(0);
+                    lbuf[k] = in && ck_lo ? int(ckl[(n - 1 - k) * kWave + lane]) : 0;
+                }
                 for (int i0 = n - 1; i0 >= 0; i0 -= kPrefetch) {
                     // phase 1, independent of the adjoint chain: brackets, the
                     // derivative of each step to its vol, the sample adjoints
@@ -827,8 +847,11 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         js[k] = 0;
                         if (i < 0) continue;
                         const R xi = buf[k];
-                        buf[k] = i - kPrefetch >= 0 ? ck[(i - kPrefetch) * kWave + lane] : R(0);
-                        const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
+                        const int lo = ck_lo ? lbuf[k] : count_le(ks, xi);
+                        const bool in = i - kPrefetch >= 0;
+                        buf[k] = in ? ck[(i - kPrefetch) * kWave + lane] : R(0);
+                        lbuf[k] = in && ck_lo ? int(ckl[(i - kPrefetch) * kWave + lane]) : 0;
+                        const Bracket<R> b = local_vol_at(ks, inv_dx, iv + i * m, xi, lo);
                         // d x_{i+1} / d vol = g - vol, recovered from the two states:
                         // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
                         if (b.vol != R(0)) {

@@ -102,3 +102,26 @@ def test_store_entry_points_take_the_mode(engine):
     b = cf.AADriskOne("lv", "uoc", cf.NumericalParam(numPath=1 << 14, aadMode=REC))
     np.testing.assert_allclose(a["risks"], b["risks"], rtol=1e-10, atol=1e-16)
     cf.clearStore()
+
+
+@pytest.mark.parametrize("n_spots", [255, 256])
+@pytest.mark.parametrize("prec", [cf.FP64, cf.FP32])
+def test_checkpoint_bracket_bytes_at_the_knot_limit(engine, n_spots, prec):
+    """The checkpoint variant reloads each state's knot bracket from a byte
+    for grids of up to 255 knots and searches again above: both sides of the
+    limit give the recompute variant's risks and, in FP64, the oracle tape's.
+    Half a year of weekly steps keeps the (steps x spots) adjoint grid in LDS."""
+    mk = W.dupire_args(n_spots)
+    pk = W.barrier_args(maturity=0.5)
+    mdl, prd = cf.Dupire(**mk), cf.UOC(**pk)
+    n = 600
+    ck = cf.mcParallelSimulAAD(prd, mdl, weights=[1.0, 0.5], num=cf.NumericalParam(numPath=n, precision=prec))
+    rc = cf.mcParallelSimulAAD(prd, mdl, weights=[1.0, 0.5],
+                               num=cf.NumericalParam(numPath=n, precision=prec, aadMode=REC))
+    np.testing.assert_array_equal(rc.payoff_means, ck.payoff_means)
+    scale = max(np.abs(ck.risks).max(), 1e-300)
+    tol = 1e-11 if prec == cf.FP64 else 2e-4
+    assert np.abs(rc.risks - ck.risks).max() <= tol * scale, np.abs(rc.risks - ck.risks).max() / scale
+    if prec == cf.FP64:
+        _, _, risks = po.aad(po.Model(**mk), po.Product(**pk), n, weights=[1.0, 0.5])
+        assert np.abs(ck.risks - risks).max() <= 1e-10 * max(np.abs(risks).max(), 1e-300)
