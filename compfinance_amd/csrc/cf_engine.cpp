@@ -869,17 +869,32 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
     Staging st;
     P->o_ss = st.add(md.step_sample);
     if (dup) {
-        if (fp64) {
-            P->o_knots = st.add(md.log_spots);
-            P->o_invdx = st.add(md.inv_dx);
-            P->o_iv = st.add(md.iv);
-        } else {
-            vector<float> ix(md.n_spots), v(md.iv.size());
-            for (int j = 0; j < md.n_spots; ++j) ix[j] = float(md.inv_dx[j]);
-            for (size_t e = 0; e < v.size(); ++e) v[e] = float(md.iv[e]);
-            P->o_knots = st.add(knots_f);
-            P->o_invdx = st.add(ix);
+        // interval tables: interval c in [0, m] starts at knot c - 1 (the
+        // first knot for c = 0) with inverse width 0 outside the knots; rows
+        // padded with their first and last value (cfk::lv_cols)
+        const int m = md.n_spots, mp = cfk::lv_cols(m);
+        auto tables = [&](auto real, const vector<double>& knots_r) {
+            using T = decltype(real);
+            vector<T> kx(size_t(m) + 1), ih(size_t(m) + 1, T(0)), v(size_t(md.n_steps) * mp);
+            kx[0] = T(knots_r[0]);
+            for (int j = 0; j < m; ++j) kx[size_t(j) + 1] = T(knots_r[size_t(j)]);
+            for (int j = 0; j + 1 < m; ++j) ih[size_t(j) + 1] = T(md.inv_dx[size_t(j)]);
+            for (int i = 0; i < md.n_steps; ++i) {
+                const double* r = &md.iv[size_t(i) * m];
+                T* d = &v[size_t(i) * mp];
+                d[0] = T(r[0]);
+                for (int j = 0; j < m; ++j) d[j + 1] = T(r[j]);
+                d[m + 1] = T(r[m - 1]);
+            }
+            P->o_knots = st.add(kx);
+            P->o_invdx = st.add(ih);
             P->o_iv = st.add(v);
+        };
+        if (fp64) {
+            tables(double(), md.log_spots);
+        } else {
+            const vector<double> kf(knots_f.begin(), knots_f.end());   // exact: floats widen
+            tables(float(), kf);
         }
     } else if (fp64) {
         P->o_drift = st.add(md.drift);

@@ -89,9 +89,12 @@ struct LaunchParams {
 
     // model tables (real type of the kernel)
     int n_spots;                 // Dupire knots
-    const void* knots;           // [n_spots] log spots (FP64) / log(spot/S0) (FP32)
-    const void* inv_dx;          // [n_spots] 1 / (knot[j+1] - knot[j]), last unused
-    const void* iv;              // [n_steps][n_spots] sqrt(dt)-scaled local vols
+    // interval tables indexed by the knot count c = #{knots <= x} in [0, m]:
+    const void* knots;           // [n_spots + 1] first knot, then the knots: log spots
+                                 // (FP64) / log(spot/S0) (FP32); interval c starts at [c]
+    const void* inv_dx;          // [n_spots + 1] 0, 1 / (knot[j+1] - knot[j]) for j < m - 1, 0
+    const void* iv;              // [n_steps][n_spots + 2] sqrt(dt)-scaled local vols, each
+                                 // row padded with its first and last value
     const uint16_t* bucket;      // [n_buckets] knot search table
     int n_buckets, bucket_fix;   // table size, knots per bucket at most
     double bucket_x0, bucket_invh;   // exactly representable in the kernel's real type
@@ -152,6 +155,10 @@ CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 // timelines or large grids, keeps only direction numbers 0..5 of each
 // dimension (the low word is formed in registers) and reads the local-vol
 // table from global memory through the caches.
+// Columns of a padded local-vol row: interval c of a row spans columns c and
+// c + 1, flat outside the knots (the first and last value repeated).
+CF_HD inline int lv_cols(int n_spots) { return n_spots + 2; }
+
 CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int n_pay, int waves,
                                     int real_bytes, bool dupire, bool europeans, bool adjoint, bool lean,
                                     int n_buckets, int book_bytes, int ivbar_copies)
@@ -161,10 +168,10 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.lo = o;          o = cf_align16(o + size_t(n_steps) * (lean ? 8 : kWave) * 4);
     L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
-    L.knots = o;       o = cf_align16(o + (dupire ? size_t(n_spots) * real_bytes : 0));
-    L.inv_dx = o;      o = cf_align16(o + (dupire ? size_t(n_spots) * real_bytes : 0));
+    L.knots = o;       o = cf_align16(o + (dupire ? size_t(n_spots + 1) * real_bytes : 0));
+    L.inv_dx = o;      o = cf_align16(o + (dupire ? size_t(n_spots + 1) * real_bytes : 0));
     L.bucket = o;      o = cf_align16(o + (dupire ? size_t(n_buckets) * 2 : 0));
-    L.iv = o;          o = cf_align16(o + (dupire && !lean ? size_t(n_steps) * n_spots * real_bytes : 0));
+    L.iv = o;          o = cf_align16(o + (dupire && !lean ? size_t(n_steps) * lv_cols(n_spots) * real_bytes : 0));
     L.drift = o;       o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
     L.stdv = o;        o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
     L.samples = o;     o = cf_align16(o + (dupire ? 0 : size_t(n_samples) * 3 * 8));
@@ -172,7 +179,8 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
     L.book = o;        o = cf_align16(o + size_t(book_bytes));
     L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
-    L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * n_steps * n_spots * 8 : 0));
+    L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * n_steps * lv_cols(n_spots) * 8
+                                                             : 0));
     L.total = o;
     return L;
 }

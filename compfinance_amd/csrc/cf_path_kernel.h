@@ -181,7 +181,7 @@ template <typename R>
 struct Bracket {
     R vol;
     R slope;      // d vol / d x
-    int j;        // left node
+    int j;        // padded column of the left node (the row's columns j, j + 1)
     R wa, wb;     // adjoint weights of nodes j and j + 1
 };
 
@@ -220,36 +220,33 @@ __device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
     return c;
 }
 
-// Branch-free: every load is in range (nodes clamped), the extrapolation
-// cases are selected afterwards.  lo = count_le(ks, x), the number of knots
-// <= x, found by the caller (the reverse sweep reloads it from the forward's).
+// Interval c = count_le(ks, x), the number of knots <= x, found by the
+// caller (the reverse sweep reloads it from the forward's).  The interval
+// tables make the flat extrapolation an ordinary interval: for c = 0 and
+// c = m the padded row repeats the end value and the inverse width is 0, so
+// t = 0, vol = the end value, slope = 0; inside, t = (x - k[c-1]) / (k[c] -
+// k[c-1]) and vol = y[c-1] + (y[c] - y[c-1]) t.  No clamps, no selects.
+// kx: [m + 1] interval starts, ihx: [m + 1] inverse widths, row: [m + 2].
 template <typename R>
-__device__ __forceinline__ Bracket<R> local_vol_at(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x, int lo)
+__device__ __forceinline__ Bracket<R> local_vol_at(const R* kx, const R* ihx, const R* row, R x, int c)
 {
-    const int m = ks.m;
-    int j = lo - 1;
-    j = j > m - 2 ? m - 2 : j;
-    j = j < 0 ? 0 : j;
-    const int j2 = j + 1 < m ? j + 1 : j;
-    const R y1 = row[j], y2 = row[j2];
-    const R x1 = ks.knots[j], ih = inv_dx[j];
-    const bool right = lo >= m && m > 1;
-    const bool left = !right && (lo == 0 || m == 1);
-    const R t = (x - x1) * ih;
+    const R y1 = row[c], y2 = row[c + 1];
+    const R ih = ihx[c];
+    const R t = (x - kx[c]) * ih;
     const R dy = y2 - y1;
     Bracket<R> b;
-    b.j = j;
-    b.vol = left ? y1 : (right ? y2 : y1 + dy * t);
-    b.slope = (left || right) ? R(0) : dy * ih;
-    b.wa = left ? R(1) : (right ? R(0) : R(1) - t);
-    b.wb = left ? R(0) : (right ? R(1) : t);
+    b.j = c;
+    b.vol = y1 + dy * t;
+    b.slope = dy * ih;
+    b.wa = R(1) - t;
+    b.wb = t;
     return b;
 }
 
 template <typename R>
-__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* inv_dx, const R* row, R x)
+__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* kx, const R* ihx, const R* row, R x)
 {
-    return local_vol_at(ks, inv_dx, row, x, count_le(ks, x));
+    return local_vol_at(kx, ihx, row, x, count_le(ks, x));
 }
 
 __device__ __forceinline__ double recip_mul(double a, double b) { return fast_div(a, b); }
@@ -292,6 +289,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     constexpr bool EUS = PRD == PRD_EUROPEANS;
     const int waves = blockDim.x >> 6;
     const int n = P.n_steps, m = P.n_spots, ns = P.n_samples, npay = P.n_pay;
+    const int mp = lv_cols(m);   // padded local-vol row
     const int stride = out_stride(npay);
     constexpr bool BOOK = PRD == PRD_BOOK;
     const SmemLayout L = smem_layout(n, m, ns, npay, waves, int(sizeof(R)), DUP, EUS, AD, LEAN, P.n_buckets,
@@ -338,10 +336,10 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         const R* gk = static_cast<const R*>(P.knots);
         const R* gi = static_cast<const R*>(P.inv_dx);
         const R* gv = static_cast<const R*>(P.iv);
-        for (int e = tid; e < m; e += nthr) { knots[e] = gk[e]; inv_dx[e] = gi[e]; }
+        for (int e = tid; e <= m; e += nthr) { knots[e] = gk[e]; inv_dx[e] = gi[e]; }
         for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = P.bucket[e];
         if constexpr (!LEAN)
-            for (int e = tid; e < n * m; e += nthr) iv_lds[e] = gv[e];
+            for (int e = tid; e < n * mp; e += nthr) iv_lds[e] = gv[e];
     } else {
         const R* gd = static_cast<const R*>(P.drift);
         const R* gs = static_cast<const R*>(P.stdv);
@@ -359,7 +357,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     }
     for (int e = tid; e < waves * stride; e += nthr) acc[e] = 0.0;
     if constexpr (AD && DUP)
-        for (int e = tid; e < P.ivbar_copies * n * m; e += nthr) ivbar[e] = 0.0;
+        for (int e = tid; e < P.ivbar_copies * n * mp; e += nthr) ivbar[e] = 0.0;
     __syncthreads();
 
     const int wave = tid >> 6, lane = tid & 63;
@@ -397,7 +395,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     const bool ck_lo = CKB && m <= kMaxStoredKnots;
     uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave) : nullptr;
     KnotSearch<R> ks;
-    ks.knots = knots;
+    ks.knots = knots + 1;   // the knots themselves, after the repeated first one
     ks.table = bucket;
     ks.m = m;
     ks.nb = P.n_buckets;
@@ -410,7 +408,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     const R* nrm = static_cast<const R*>(P.normals);
     // this lane's local-vol adjoint grid: odd lanes use the second copy, which
     // halves the lanes contending for one address in an atomic
-    double* ivbl = ivbar + ((P.ivbar_copies > 1 && (lane & 1)) ? n * m : 0);
+    double* ivbl = ivbar + ((P.ivbar_copies > 1 && (lane & 1)) ? n * mp : 0);
     // per-lane parameter adjoints, summed over the lane's paths
     double padj[kMaxParamAdj] = {0.0, 0.0, 0.0, 0.0};
 
@@ -627,7 +625,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 const int lo = count_le(ks, x);
                 if constexpr (CKB)
                     if (ck_lo) ckl[i * kWave + lane] = uint8_t(lo);
-                const Bracket<R> b = local_vol_at(ks, inv_dx, iv + i * m, x, lo);
+                const Bracket<R> b = local_vol_at(knots, inv_dx, iv + i * mp, x, lo);
                 x = euler_step(x, b.vol, g);
             } else {
                 if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
@@ -779,7 +777,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         cj[k] = 0;
                         if (i >= n) continue;
                         const R g = draw(i);
-                        const Bracket<R> b = local_vol(ks, inv_dx, iv + i * m, xi);
+                        const Bracket<R> b = local_vol(ks, knots, inv_dx, iv + i * mp, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol
                         ca[k] = R(1) + ws * b.slope;
                         cw[k] = ws * b.wa;
@@ -799,9 +797,9 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         const int i = i0 + k;
                         if (i >= n) continue;
                         if (active) {
-                            double* row = ivbl + i * m;
+                            double* row = ivbl + i * mp;
                             atomicAdd(row + cj[k], double(abar * cw[k]));
-                            if (m > 1) atomicAdd(row + cj[k] + 1, double(abar * cv[k]));
+                            atomicAdd(row + cj[k] + 1, double(abar * cv[k]));
                         }
                         abar = abar * ca[k] + cb[k];
                     }
@@ -851,7 +849,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         const bool in = i - kPrefetch >= 0;
                         buf[k] = in ? ck[(i - kPrefetch) * kWave + lane] : R(0);
                         lbuf[k] = in && ck_lo ? int(ckl[(i - kPrefetch) * kWave + lane]) : 0;
-                        const Bracket<R> b = local_vol_at(ks, inv_dx, iv + i * m, xi, lo);
+                        const Bracket<R> b = local_vol_at(knots, inv_dx, iv + i * mp, xi, lo);
                         // d x_{i+1} / d vol = g - vol, recovered from the two states:
                         // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
                         if (b.vol != R(0)) {
@@ -886,9 +884,9 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         for (int k = 0; k < kPrefetch; ++k) {
                             const int i = i0 - k;
                             if (i < 0) break;
-                            double* row = ivbl + i * m;
+                            double* row = ivbl + i * mp;
                             atomicAdd(row + js[k], double(vb[k] * wa[k]));
-                            if (m > 1) atomicAdd(row + js[k] + 1, double(vb[k] * wb[k]));
+                            atomicAdd(row + js[k] + 1, double(vb[k] * wb[k]));
                         }
                     }
                 }
@@ -964,7 +962,19 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     }
     if constexpr (AD && DUP) {
         double* dst = P.ivbar_blocks + size_t(blockIdx.x) * n * m;
-        for (int e = tid; e < n * m; e += nthr) dst[e] = P.ivbar_copies > 1 ? ivbar[e] + ivbar[n * m + e] : ivbar[e];
+        // padded columns back onto the grid nodes: column j + 1 is node j,
+        // the end columns 0 and m + 1 belong to the first and last node
+        for (int e = tid; e < n * m; e += nthr) {
+            const int i = e / m, j = e - i * m;
+            double v = 0.0;
+            for (int c = 0; c < P.ivbar_copies; ++c) {
+                const double* r = ivbar + size_t(c) * n * mp + size_t(i) * mp;
+                v += r[j + 1];
+                if (j == 0) v += r[0];
+                if (j == m - 1) v += r[m + 1];
+            }
+            dst[e] = v;
+        }
     }
 }
 
