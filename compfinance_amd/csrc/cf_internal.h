@@ -144,8 +144,8 @@ struct LaunchParams {
 
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
-    size_t lo, hi, step_sample, knots, inv_dx, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book, acc,
-        ivbar, total;
+    size_t lo, hi, step_sample, invn, knots, inv_dx, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book,
+        acc, ivbar, total;
 };
 
 CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -155,6 +155,10 @@ CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 // timelines or large grids, keeps only direction numbers 0..5 of each
 // dimension (the low word is formed in registers) and reads the local-vol
 // table from global memory through the caches.
+// Rows of the FP32 inverse-normal table (four coefficients each): eighths of
+// the 31 binades of 2^32 min(u, 1 - u), and its value at 1/2.
+constexpr int kInvNormalRows = 31 * 8 + 1;
+
 // Columns of a padded local-vol row: interval c of a row spans columns c and
 // c + 1, flat outside the knots (the first and last value repeated).
 CF_HD inline int lv_cols(int n_spots) { return n_spots + 2; }
@@ -168,6 +172,7 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.lo = o;          o = cf_align16(o + size_t(n_steps) * (lean ? 8 : kWave) * 4);
     L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
+    L.invn = o;        o = cf_align16(o + (real_bytes == 4 ? size_t(kInvNormalRows) * 4 * 4 : 0));
     L.knots = o;       o = cf_align16(o + (dupire ? size_t(n_spots + 1) * real_bytes : 0));
     L.inv_dx = o;      o = cf_align16(o + (dupire ? size_t(n_spots + 1) * real_bytes : 0));
     L.bucket = o;      o = cf_align16(o + (dupire ? size_t(n_buckets) * 2 : 0));

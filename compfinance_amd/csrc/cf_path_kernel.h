@@ -122,49 +122,33 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, double)
     return inv_normal_p(2.3283064365387E-10 * double(xi));
 }
 
-// FP32 form: Giles' single-precision erfinv approximation (M. Giles,
-// "Approximating the erfinv function", GPU Computing Gems Jade Edition,
-// 2011), z = sqrt(2) erfinv(2u - 1).  Its central branch covers 99.7% of the
-// draws (Moro's 84%), so a wave rarely runs the tail branch.  The tail mass
-// min(u, 1 - u) is formed exactly in the integer domain (2^32 - xi) before
-// scaling, so no precision is lost near u = 1.  Differences to the FP64
-// Moro values are of the order of FP32 rounding.
-__device__ __forceinline__ float inv_normal(uint32_t xi, float)
+// FP32 form: v = min(u, 1 - u) is formed exactly in the integer domain as
+// xs = 2^32 v in [1, 2^31]; the IEEE bits of float(xs) give the table row
+// (exponent and top three mantissa bits: one eighth of a binade of v) and
+// t in [-1, 1) (the low 20 mantissa bits).  Each row is a cubic in t fitted
+// to the same Moro function in FP64 (inv_normal_f32.inc, written by
+// gen_inv_normal_f32.py).  No logarithm, no branch: one 16-byte LDS read and
+// three FMAs.  Differences to the FP64 Moro values are below 5e-7 (the FP32
+// rounding of the result in the far tail).
+__device__ __forceinline__ float inv_normal(uint32_t xi, const float* tab)
 {
-    const bool sup = xi >= 0x80000000u;              // u > 1/2 for the reference scaling
-    const uint32_t xs = sup ? (0u - xi) : xi;         // 2^32 - xi, exact
-    const float up = float(xs) * 2.3283064e-10f;      // min(u, 1 - u)
-    const float ax = 1.0f - 2.0f * up;                // |2u - 1|
-    const float ln2 = 0.693147180559945f;
-    // w = -ln((1 - x)(1 + x)) = -ln(4 up (1 - up)); the argument is a normal number
-    float w = -__builtin_amdgcn_logf(4.0f * up * (1.0f - up)) * ln2;
-    float p;
-    if (w < 5.0f) {
-        w = w - 2.5f;
-        p = 2.81022636e-08f;
-        p = 3.43273939e-07f + p * w;
-        p = -3.5233877e-06f + p * w;
-        p = -4.39150654e-06f + p * w;
-        p = 0.00021858087f + p * w;
-        p = -0.00125372503f + p * w;
-        p = -0.00417768164f + p * w;
-        p = 0.246640727f + p * w;
-        p = 1.50140941f + p * w;
-    } else {
-        w = __builtin_amdgcn_sqrtf(w) - 3.0f;
-        p = -0.000200214257f;
-        p = 0.000100950558f + p * w;
-        p = 0.00134934322f + p * w;
-        p = -0.00367342844f + p * w;
-        p = 0.00573950773f + p * w;
-        p = -0.0076224613f + p * w;
-        p = 0.00943887047f + p * w;
-        p = 1.00167406f + p * w;
-        p = 2.83297682f + p * w;
-    }
-    const float z = 1.41421356237f * p * ax;
-    return sup ? z : -z;
+    const bool sup = xi >= 0x80000000u;               // u >= 1/2 for the reference scaling
+    const uint32_t xs = sup ? (0u - xi) : xi;         // 2^32 - xi, exact; xi != 0 for Sobol points
+    const uint32_t bits = __float_as_uint(float(xs));
+    const int row = int(bits >> 20) - (127 << 3);     // [0, kInvNormalRows - 1]
+    const float t = fmaf(float(bits & 0xfffffu), 1.0f / 524288.0f, -1.0f);
+    const float4 c = *reinterpret_cast<const float4*>(tab + 4 * row);
+    const float p = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
+    return sup ? -p : p;
 }
+
+// FP64: the reference's function; the table pointer is unused
+__device__ __forceinline__ double inv_normal(uint32_t xi, const double*) { return inv_normal(xi, 0.0); }
+
+// FP32 inverse-normal table: [rows][4 monomial coefficients]
+static __constant__ float kInvNormalF32[kInvNormalRows * 4] = {
+#include "inv_normal_f32.inc"
+};
 
 // Spot from the path state: FP64 state is log(S) (reference), FP32 state is
 // log(S / S0), which keeps more significant bits in single precision.
@@ -311,6 +295,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     double* pay_strike = reinterpret_cast<double*>(smem + L.pay_strike);
     double* acc = reinterpret_cast<double*>(smem + L.acc);
     double* ivbar = reinterpret_cast<double*>(smem + L.ivbar);
+    R* invn = reinterpret_cast<R*>(smem + L.invn);   // FP32 inverse-normal table
 
     const int tid = threadIdx.x, nthr = blockDim.x;
 
@@ -332,6 +317,8 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         }
     }
     for (int e = tid; e <= n; e += nthr) step_sample[e] = P.step_sample[e];
+    if constexpr (sizeof(R) == 4)
+        for (int e = tid; e < kInvNormalRows * 4; e += nthr) invn[e] = kInvNormalF32[e];
     if constexpr (DUP) {
         const R* gk = static_cast<const R*>(P.knots);
         const R* gi = static_cast<const R*>(P.inv_dx);
@@ -465,7 +452,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         const size_t ncol = size_t(active ? pidx : 0);
         auto draw = [&](int i) -> R {
             if constexpr (NBUF) return nrm[size_t(i) * size_t(P.n_path) + ncol];
-            else return inv_normal(hi[i] ^ low_word(i), R());
+            else return inv_normal(hi[i] ^ low_word(i), static_cast<const R*>(invn));
         };
 
         ProductState<R> ps;

@@ -42,7 +42,7 @@ def test_fp64_values_match_reference(engine, name):
     case = g["cases"][name]
     mdl, prd = _objects(g["models"], case)
     res = cf.mcParallelSimul(prd, mdl, case["n_path"], per_path="per_path" in case)
-    # 1e-10 relative on the means over the same paths
+    # the FP64 form's bar: 1e-10 relative on the means over the same paths
     np.testing.assert_allclose(res.means, case["values"], rtol=1e-10, atol=1e-15)
     if "per_path" in case:
         ref = np.asarray(case["per_path"]).reshape(-1, len(case["values"]))
