@@ -144,8 +144,8 @@ struct LaunchParams {
 
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
-    size_t lo, hi, step_sample, invn, knots, inv_dx, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book,
-        acc, ivbar, total;
+    size_t lo, hi, step_sample, invn, knots, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book, acc,
+        ivbar, total;
 };
 
 CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -173,8 +173,8 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
     L.invn = o;        o = cf_align16(o + (real_bytes == 4 ? size_t(kInvNormalRows) * 4 * 4 : 0));
-    L.knots = o;       o = cf_align16(o + (dupire ? size_t(n_spots + 1) * real_bytes : 0));
-    L.inv_dx = o;      o = cf_align16(o + (dupire ? size_t(n_spots + 1) * real_bytes : 0));
+    // interval (start, inverse width) pairs, interleaved
+    L.knots = o;       o = cf_align16(o + (dupire ? size_t(2 * (n_spots + 1)) * real_bytes : 0));
     L.bucket = o;      o = cf_align16(o + (dupire ? size_t(n_buckets) * 2 : 0));
     L.iv = o;          o = cf_align16(o + (dupire && !lean ? size_t(n_steps) * lv_cols(n_spots) * real_bytes : 0));
     L.drift = o;       o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));

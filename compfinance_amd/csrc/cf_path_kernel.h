@@ -176,7 +176,7 @@ struct Bracket {
 // the kernel's own arithmetic, so the answer is exactly upper_bound's.
 template <typename R>
 struct KnotSearch {
-    const R* knots;
+    const R* knots;     // knot c at knots[2 c] (the interleaved interval table from interval 1)
     const uint16_t* table;
     int m, nb, fix;
     R x0, invh;
@@ -194,11 +194,11 @@ __device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
     // the bucket's first two candidate knots, loaded together from clamped
     // (always valid) indices; the knots increase, so each one <= x adds one
     const int top = s.m - 1;
-    const R k0 = s.knots[lo < top ? lo : top];
-    const R k1 = s.knots[lo + 1 < top ? lo + 1 : top];
+    const R k0 = s.knots[2 * (lo < top ? lo : top)];
+    const R k1 = s.knots[2 * (lo + 1 < top ? lo + 1 : top)];
     int c = lo + int((lo <= top) & (k0 <= x)) + int((lo + 1 <= top) & (k1 <= x));
     for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
-        const R kc = s.knots[c < top ? c : top];
+        const R kc = s.knots[2 * (c < top ? c : top)];
         c += int((c <= top) & (kc <= x));
     }
     return c;
@@ -210,13 +210,14 @@ __device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
 // c = m the padded row repeats the end value and the inverse width is 0, so
 // t = 0, vol = the end value, slope = 0; inside, t = (x - k[c-1]) / (k[c] -
 // k[c-1]) and vol = y[c-1] + (y[c] - y[c-1]) t.  No clamps, no selects.
-// kx: [m + 1] interval starts, ihx: [m + 1] inverse widths, row: [m + 2].
+// kxi: [m + 1] (interval start, inverse width) pairs, one aligned read;
+// row: [m + 2].
 template <typename R>
-__device__ __forceinline__ Bracket<R> local_vol_at(const R* kx, const R* ihx, const R* row, R x, int c)
+__device__ __forceinline__ Bracket<R> local_vol_at(const R* kxi, const R* row, R x, int c)
 {
     const R y1 = row[c], y2 = row[c + 1];
-    const R ih = ihx[c];
-    const R t = (x - kx[c]) * ih;
+    const R ih = kxi[2 * c + 1];
+    const R t = (x - kxi[2 * c]) * ih;
     const R dy = y2 - y1;
     Bracket<R> b;
     b.j = c;
@@ -228,9 +229,9 @@ __device__ __forceinline__ Bracket<R> local_vol_at(const R* kx, const R* ihx, co
 }
 
 template <typename R>
-__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* kx, const R* ihx, const R* row, R x)
+__device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* kxi, const R* row, R x)
 {
-    return local_vol_at(kx, ihx, row, x, count_le(ks, x));
+    return local_vol_at(kxi, row, x, count_le(ks, x));
 }
 
 __device__ __forceinline__ double recip_mul(double a, double b) { return fast_div(a, b); }
@@ -282,8 +283,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     uint32_t* lo_tab = reinterpret_cast<uint32_t*>(smem + L.lo);
     uint32_t* hi_all = reinterpret_cast<uint32_t*>(smem + L.hi);
     int* step_sample = reinterpret_cast<int*>(smem + L.step_sample);
-    R* knots = reinterpret_cast<R*>(smem + L.knots);
-    R* inv_dx = reinterpret_cast<R*>(smem + L.inv_dx);
+    R* kxi = reinterpret_cast<R*>(smem + L.knots);   // interval (start, inverse width) pairs
     uint16_t* bucket = reinterpret_cast<uint16_t*>(smem + L.bucket);
     R* iv_lds = reinterpret_cast<R*>(smem + L.iv);
     R* drift = reinterpret_cast<R*>(smem + L.drift);
@@ -323,7 +323,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         const R* gk = static_cast<const R*>(P.knots);
         const R* gi = static_cast<const R*>(P.inv_dx);
         const R* gv = static_cast<const R*>(P.iv);
-        for (int e = tid; e <= m; e += nthr) { knots[e] = gk[e]; inv_dx[e] = gi[e]; }
+        for (int e = tid; e <= m; e += nthr) { kxi[2 * e] = gk[e]; kxi[2 * e + 1] = gi[e]; }
         for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = P.bucket[e];
         if constexpr (!LEAN)
             for (int e = tid; e < n * mp; e += nthr) iv_lds[e] = gv[e];
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     const bool ck_lo = CKB && m <= kMaxStoredKnots;
     uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave) : nullptr;
     KnotSearch<R> ks;
-    ks.knots = knots + 1;   // the knots themselves, after the repeated first one
+    ks.knots = kxi + 2;     // the knots themselves: interval starts from interval 1
     ks.table = bucket;
     ks.m = m;
     ks.nb = P.n_buckets;
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 const int lo = count_le(ks, x);
                 if constexpr (CKB)
                     if (ck_lo) ckl[i * kWave + lane] = uint8_t(lo);
-                const Bracket<R> b = local_vol_at(knots, inv_dx, iv + i * mp, x, lo);
+                const Bracket<R> b = local_vol_at(kxi, iv + i * mp, x, lo);
                 x = euler_step(x, b.vol, g);
             } else {
                 if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
@@ -764,7 +764,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         cj[k] = 0;
                         if (i >= n) continue;
                         const R g = draw(i);
-                        const Bracket<R> b = local_vol(ks, knots, inv_dx, iv + i * mp, xi);
+                        const Bracket<R> b = local_vol(ks, kxi, iv + i * mp, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol
                         ca[k] = R(1) + ws * b.slope;
                         cw[k] = ws * b.wa;
@@ -836,7 +836,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         const bool in = i - kPrefetch >= 0;
                         buf[k] = in ? ck[(i - kPrefetch) * kWave + lane] : R(0);
                         lbuf[k] = in && ck_lo ? int(ckl[(i - kPrefetch) * kWave + lane]) : 0;
-                        const Bracket<R> b = local_vol_at(knots, inv_dx, iv + i * mp, xi, lo);
+                        const Bracket<R> b = local_vol_at(kxi, iv + i * mp, xi, lo);
                         // d x_{i+1} / d vol = g - vol, recovered from the two states:
                         // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
                         if (b.vol != R(0)) {
