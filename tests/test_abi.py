@@ -167,3 +167,22 @@ def test_simulation_without_gpu_fails_loudly():
     with pytest.raises(cf.CfError) as e:
         cf.mcParallelSimul(prd, mdl, 64)
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("kw, code, msg", [
+    (dict(numPath=0), -1, "numPath must be positive"),
+    (dict(numPath=-5), -1, "numPath must be positive"),
+    (dict(numPath=64, firstPath=-1), -1, "first path must be non-negative"),
+    (dict(numPath=64, precision=7), -1, "unknown precision"),
+    (dict(numPath=64, aadMode=9), -1, "unknown AAD mode"),
+])
+def test_numerical_parameters_are_validated_before_the_device(kw, code, msg):
+    """Empty and negative path ranges and unknown options fail with an
+    argument error before any device work (the reference's int path counts
+    keep the last Sobol index below 2^32 - 1)."""
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    with pytest.raises(cf.CfError, match=msg) as e:
+        cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(**kw), per_path=False)
+    assert e.value.code == code
+    with pytest.raises(cf.CfError, match=msg):
+        cf.mcParallelSimulAAD(prd, mdl, num=cf.NumericalParam(**kw))
