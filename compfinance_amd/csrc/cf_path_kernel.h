@@ -15,7 +15,7 @@
 //     per-block LDS table built once, HI_q a per-wave LDS row built per block
 //     of paths;
 //   * local-vol table, log-spot knots, timeline sample map in LDS;
-//   * payoffs reduced per wave with cross-lane shuffles into per-wave LDS
+//   * payoffs reduced per wave with DPP cross-lane moves into per-wave LDS
 //     accumulators, then per block, then across blocks by a separate kernel
 //     in fixed order;
 //   * adjoint: path states checkpointed in a per-wave HBM slot (reused by the
@@ -35,10 +35,33 @@ namespace cfk {
 
 // ------------------------------------------------------------------ helpers
 
+// One DPP move of an FP64 value (both halves, the same lane pattern); lanes
+// the pattern does not write, or whose source lies outside the row, read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(u & 0xffffffffu), CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(u >> 32), CTRL, ROW_MASK, 0xf, true);
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo));
+}
+
+// The wave's sum lands in this lane.
+constexpr int kSumLane = kWave - 1;
+
+// FP64 sum over the wave by data-parallel primitives in registers (no LDS):
+// prefix sums of 2, 4, 8 and 16 lanes within each row of 16 (row_shr), then
+// row 0's sum into row 1 and row 2's into row 3 (row_bcast:15), then rows
+// 0-1 into rows 2-3 (row_bcast:31).  The total is valid in lane kSumLane
+// only; the order of the additions is fixed, so the sum is reproducible.
 __device__ __forceinline__ double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    v += dpp_f64<0x111, 0xf>(v);   // row_shr:1
+    v += dpp_f64<0x112, 0xf>(v);   // row_shr:2
+    v += dpp_f64<0x114, 0xf>(v);   // row_shr:4
+    v += dpp_f64<0x118, 0xf>(v);   // row_shr:8
+    v += dpp_f64<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -472,7 +495,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         auto emit = [&](int col, R pay) {
             const double v = active ? double(pay) : 0.0;
             const double vs = wave_sum(v), vq = wave_sum(v * v);
-            if (lane == 0) { accw[col] += vs; accw[npay + col] += vq; }
+            if (lane == kSumLane) { accw[col] += vs; accw[npay + col] += vq; }
             if (P.per_path && active) P.per_path[pidx * npay + col] = v;
             if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[col]) * pay;
         };
@@ -510,7 +533,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 const R pay = (F - k > R(0) ? F - k : R(0)) * discv / numv;
                 const double v = active ? double(pay) : 0.0;
                 const double vs = wave_sum(v), vq = wave_sum(v * v);
-                if (lane == 0) { accw[0] += vs; accw[npay] += vq; }
+                if (lane == kSumLane) { accw[0] += vs; accw[npay] += vq; }
                 if (P.per_path && active) P.per_path[pidx * npay] = v;
                 if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[0]) * pay;
             } else if constexpr (PRD == PRD_BOOK) {
@@ -589,7 +612,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                     const R pay = (F - kk > R(0) ? F - kk : R(0)) / numv;
                     const double v = active ? double(pay) : 0.0;
                     const double vs = wave_sum(v), vq = wave_sum(v * v);
-                    if (lane == 0) { accw[k] += vs; accw[npay + k] += vq; }
+                    if (lane == kSumLane) { accw[k] += vs; accw[npay + k] += vq; }
                     if (P.per_path && active) P.per_path[pidx * npay + k] = v;
                     if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[k]) * pay;
                 }
@@ -628,7 +651,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
             const R p1 = ps.euro;
             const double v0 = active ? double(p0) : 0.0, v1 = active ? double(p1) : 0.0;
             const double a0 = wave_sum(v0), a1 = wave_sum(v1), b0 = wave_sum(v0 * v0), b1 = wave_sum(v1 * v1);
-            if (lane == 0) { accw[0] += a0; accw[1] += a1; accw[npay] += b0; accw[npay + 1] += b1; }
+            if (lane == kSumLane) { accw[0] += a0; accw[1] += a1; accw[npay] += b0; accw[npay + 1] += b1; }
             if (P.per_path && active) { P.per_path[pidx * 2] = v0; P.per_path[pidx * 2 + 1] = v1; }
             if constexpr (AD) ps.aggp = R(P.weights[0]) * p0 + R(P.weights[1]) * p1;
         }
@@ -636,7 +659,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         if constexpr (AD) {
             const double agg = active ? double(ps.aggp) : 0.0;
             const double as = wave_sum(agg), aq = wave_sum(agg * agg);
-            if (lane == 0) { accw[2 * npay] += as; accw[2 * npay + 1] += aq; }
+            if (lane == kSumLane) { accw[2 * npay] += as; accw[2 * npay + 1] += aq; }
             if (P.per_path_agg && active) P.per_path_agg[pidx] = agg;
 
             // adjoints of the forward, numeraire and discount at sample s
@@ -938,7 +961,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     if constexpr (AD) {
         for (int c = 0; c < kMaxParamAdj; ++c) {
             const double v = wave_sum(padj[c]);
-            if (lane == 0) accw[2 * npay + 2 + c] += v;
+            if (lane == kSumLane) accw[2 * npay + 2 + c] += v;
         }
     }
     __syncthreads();
