@@ -904,7 +904,24 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
         P->o_drift = st.add(a);
         P->o_std = st.add(b);
     }
-    P->o_bucket = st.add(P->bt.table);
+    if (dup) {
+        // bucket entries: the bucket's knot count and its next two knots
+        const int top = md.n_spots - 1;
+        auto entries = [&](auto real, const auto& kn) {
+            using T = decltype(real);
+            vector<cfk::BucketEntry<T>> e(size_t(P->bt.nb));
+            for (int b = 0; b < P->bt.nb; ++b) {
+                const int lo = P->bt.table[size_t(b)];
+                e[size_t(b)].lo = lo;
+                e[size_t(b)].pad = 0;
+                e[size_t(b)].k0 = T(kn[size_t(std::min(lo, top))]);
+                e[size_t(b)].k1 = T(kn[size_t(std::min(lo + 1, top))]);
+            }
+            P->o_bucket = st.add(e);
+        };
+        if (fp64) entries(double(), md.log_spots);
+        else entries(float(), knots_f);
+    }
     P->o_ff = st.add(md.ff);
     P->o_disc = st.add(md.disc);
     P->o_num = st.add(md.num);
@@ -1026,7 +1043,7 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     P.knots = dt + PR.o_knots;
     P.inv_dx = dt + PR.o_invdx;
     P.iv = dt + PR.o_iv;
-    P.bucket = reinterpret_cast<const uint16_t*>(dt + PR.o_bucket);
+    P.bucket = dt + PR.o_bucket;
     P.n_buckets = dup ? PR.bt.nb : 0;
     P.bucket_fix = PR.bt.fix;
     P.bucket_x0 = PR.bt.x0;

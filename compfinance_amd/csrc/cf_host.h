@@ -119,7 +119,7 @@ struct ModelPlan {
     double s_t0 = 0.0;              // first sample's forward (barrier smoothing width)
 };
 
-// Bucket table of the kernels' upper_bound (see KnotSearch in cf_kernels.hip),
+// Bucket table of the kernels' upper_bound (KnotSearch in cf_path_kernel.h),
 // built with the same real type and the same two rounded operations the
 // kernel uses to map a state to its bucket.
 struct BucketTable {

@@ -95,7 +95,7 @@ struct LaunchParams {
     const void* inv_dx;          // [n_spots + 1] 0, 1 / (knot[j+1] - knot[j]) for j < m - 1, 0
     const void* iv;              // [n_steps][n_spots + 2] sqrt(dt)-scaled local vols, each
                                  // row padded with its first and last value
-    const uint16_t* bucket;      // [n_buckets] knot search table
+    const void* bucket;          // [n_buckets] BucketEntry<real>: the knot search table
     int n_buckets, bucket_fix;   // table size, knots per bucket at most
     double bucket_x0, bucket_invh;   // exactly representable in the kernel's real type
     const void* drift;           // Black-Scholes [n_steps]
@@ -155,6 +155,15 @@ CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 // timelines or large grids, keeps only direction numbers 0..5 of each
 // dimension (the low word is formed in registers) and reads the local-vol
 // table from global memory through the caches.
+// Entry of the knot search table: bucket b holds lo, the number of knots in
+// earlier buckets, and the next two knots (indices clamped to the grid), so
+// one LDS access brings all a search compares against.
+template <typename R>
+struct alignas(16) BucketEntry {
+    R k0, k1;
+    int lo, pad;
+};
+
 // Rows of the FP32 inverse-normal table (four coefficients each): eighths of
 // the 31 binades of 2^32 min(u, 1 - u), and its value at 1/2.
 constexpr int kInvNormalRows = 31 * 8 + 1;
@@ -175,7 +184,9 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.invn = o;        o = cf_align16(o + (real_bytes == 4 ? size_t(kInvNormalRows) * 4 * 4 : 0));
     // interval (start, inverse width) pairs, interleaved
     L.knots = o;       o = cf_align16(o + (dupire ? size_t(2 * (n_spots + 1)) * real_bytes : 0));
-    L.bucket = o;      o = cf_align16(o + (dupire ? size_t(n_buckets) * 2 : 0));
+    L.bucket = o;      o = cf_align16(o + (dupire ? size_t(n_buckets) * (real_bytes == 4 ? sizeof(BucketEntry<float>)
+                                                                                  : sizeof(BucketEntry<double>))
+                                                 : 0));
     L.iv = o;          o = cf_align16(o + (dupire && !lean ? size_t(n_steps) * lv_cols(n_spots) * real_bytes : 0));
     L.drift = o;       o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
     L.stdv = o;        o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));

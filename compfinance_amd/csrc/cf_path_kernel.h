@@ -194,13 +194,14 @@ struct Bracket {
 
 // upper_bound on the knots (interp.h:40) through a uniform bucket table:
 // bucket b = min(floor((x - x0) * invh), nb - 1) holds the number of knots
-// in earlier buckets, a lower bound of the answer; at most `fix` knots fall in
-// one bucket and are resolved by comparisons.  The host builds the table with
-// the kernel's own arithmetic, so the answer is exactly upper_bound's.
+// in earlier buckets, a lower bound of the answer, and the next two knots; at
+// most `fix` knots fall in one bucket and are resolved by comparisons.  The
+// host builds the table with the kernel's own arithmetic, so the answer is
+// exactly upper_bound's.
 template <typename R>
 struct KnotSearch {
     const R* knots;     // knot c at knots[2 c] (the interleaved interval table from interval 1)
-    const uint16_t* table;
+    const BucketEntry<R>* table;
     int m, nb, fix;
     R x0, invh;
 };
@@ -213,13 +214,11 @@ __device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
     R bf = (x - s.x0) * s.invh;
     bf = bf > R(0) ? bf : R(0);
     bf = bf < R(s.nb - 1) ? bf : R(s.nb - 1);
-    const int lo = s.table[int(bf)];
-    // the bucket's first two candidate knots, loaded together from clamped
-    // (always valid) indices; the knots increase, so each one <= x adds one
-    const int top = s.m - 1;
-    const R k0 = s.knots[2 * (lo < top ? lo : top)];
-    const R k1 = s.knots[2 * (lo + 1 < top ? lo + 1 : top)];
-    int c = lo + int((lo <= top) & (k0 <= x)) + int((lo + 1 <= top) & (k1 <= x));
+    // one read: the bucket's knot count and its first two candidate knots
+    // (clamped to the grid); the knots increase, so each one <= x adds one
+    const BucketEntry<R> e = s.table[int(bf)];
+    const int lo = e.lo, top = s.m - 1;
+    int c = lo + int((lo <= top) & (e.k0 <= x)) + int((lo + 1 <= top) & (e.k1 <= x));
     for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
         const R kc = s.knots[2 * (c < top ? c : top)];
         c += int((c <= top) & (kc <= x));
@@ -307,7 +306,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     uint32_t* hi_all = reinterpret_cast<uint32_t*>(smem + L.hi);
     int* step_sample = reinterpret_cast<int*>(smem + L.step_sample);
     R* kxi = reinterpret_cast<R*>(smem + L.knots);   // interval (start, inverse width) pairs
-    uint16_t* bucket = reinterpret_cast<uint16_t*>(smem + L.bucket);
+    BucketEntry<R>* bucket = reinterpret_cast<BucketEntry<R>*>(smem + L.bucket);
     R* iv_lds = reinterpret_cast<R*>(smem + L.iv);
     R* drift = reinterpret_cast<R*>(smem + L.drift);
     R* stdv = reinterpret_cast<R*>(smem + L.stdv);
@@ -347,7 +346,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         const R* gi = static_cast<const R*>(P.inv_dx);
         const R* gv = static_cast<const R*>(P.iv);
         for (int e = tid; e <= m; e += nthr) { kxi[2 * e] = gk[e]; kxi[2 * e + 1] = gi[e]; }
-        for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = P.bucket[e];
+        for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = static_cast<const BucketEntry<R>*>(P.bucket)[e];
         if constexpr (!LEAN)
             for (int e = tid; e < n * mp; e += nthr) iv_lds[e] = gv[e];
     } else {
