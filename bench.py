@@ -7,7 +7,8 @@ Dupire local vol on the 30 x 36 grid (1,080 vols), up-and-out call K=120,
 B=150, 5% smoothing, 3y weekly monitoring (156 log-Euler steps), 2^20 Sobol
 paths, price + delta + 1,080 local-vol sensitivities, FP32 (FP64 sums).
 Measured beside it: the same workload in FP64 and with the recompute
-adjoint, the 2^19-path FP64/FP32 pricing run, the product book on shared
+adjoint, the 2^19-path FP64/FP32 pricing run, the Black-Scholes European on
+2^16 paths (one step), the product book on shared
 paths (30 Europeans, 10 barriers, 5 Asians, aggregated risk, 2^20 paths) and a
 replay of 512 uneven requests at 200/s through the request server (p50/p99
 latency).  Inputs are synthetic (the local-vol formula of the
@@ -155,6 +156,19 @@ def main():
         return lambda: cf.mcParallelSimulAAD(book, mdl, weights=bw, num=cf.NumericalParam(
             numPath=W.C4_PATHS, precision=prec))
 
+    # C1: Black-Scholes European call, 2^16 paths, one step (latency-bound)
+    bs1, eu1 = cf.BlackScholes(100.0, 0.2), cf.European(100.0, 1.0)
+
+    def c1(aad_call):
+        n1 = cf.NumericalParam(numPath=1 << 16)
+        if aad_call:
+            return lambda: cf.mcParallelSimulAAD(eu1, bs1, num=n1)
+        return lambda: cf.mcParallelSimul(eu1, bs1, num=n1, per_path=False)
+
+    wc1, _ = _time_calls(c1(False), s64, 1)
+    wc1a, _ = _time_calls(c1(True), s64, 1)
+    c1_val, c1_risk = c1(False)(), c1(True)()
+
     wb32, kb32 = _time_calls(c4(cf.FP32), s64, 1)
     wb64, kb64 = _time_calls(c4(cf.FP64), max(2, s64 // 2), 1)
     b32, b64 = c4(cf.FP32)(), c4(cf.FP64)()
@@ -207,6 +221,10 @@ def main():
                           "fp32_kernel_ms": float(np.mean(kr32)), "fp64_kernel_ms": float(np.mean(kr64)),
                           "fp32_max_risk_diff_vs_checkpoint": float(np.abs(rr32.risks - r32.risks).max()),
                           "fp32_risk_scale": float(np.abs(r32.risks).max())},
+        "bs_european_c1": {"paths": 1 << 16, "time_steps": 1, "fp64_value_wall_ms": 1e3 * wc1 / s64,
+                           "fp64_aad_wall_ms": 1e3 * wc1a / s64, "value": float(c1_val.means[0]),
+                           "std_err": float(c1_val.std_errs[0]), "closed_form": 7.965567455405804,
+                           "delta": float(c1_risk.risks[0]), "vega": float(c1_risk.risks[1])},
         "book_c4": {"paths": W.C4_PATHS, "time_steps": nb_steps, "payoffs": int(len(bw)),
                     "risks": int(b32.risks.size),
                     "fp32_wall_ms": 1e3 * wb32 / s64, "fp64_wall_ms": 1e3 * wb64 / max(2, s64 // 2),
