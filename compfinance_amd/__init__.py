@@ -160,7 +160,40 @@ def _f64(x):
 # ---------------------------------------------------------------- models
 
 
-class BlackScholes:
+class _Described:
+    """Keeps the ctypes descriptor of a model or product, and the payoff and
+    parameter counts the engine reports for it, until a public attribute is
+    assigned again.  The descriptor points at the object's own arrays, so
+    changes made to them in place are seen without a rebuild."""
+
+    def __setattr__(self, name, value):
+        object.__setattr__(self, name, value)
+        if not name.startswith("_"):
+            self.__dict__.pop("_dcache", None)
+
+    def _cache(self):
+        c = self.__dict__.get("_dcache")
+        if c is None:
+            c = {"d": self.descriptor()}
+            self.__dict__["_dcache"] = c
+        return c
+
+
+def _desc(obj):
+    """Descriptor of a model or product, cached where the class allows."""
+    return obj._cache()["d"] if isinstance(obj, _Described) else obj.descriptor()
+
+
+def _count(obj, d, key, fn):
+    if isinstance(obj, _Described):
+        c = obj._cache()
+        if key not in c:
+            c[key] = _check(fn(ctypes.byref(d)))
+        return c[key]
+    return _check(fn(ctypes.byref(d)))
+
+
+class BlackScholes(_Described):
     """BlackScholes(spot, vol, spotMeasure=False, rate=0, div=0) (mcMdlBS.h:72-93)."""
 
     def __init__(self, spot, vol, spotMeasure=False, rate=0.0, div=0.0):
@@ -183,7 +216,7 @@ class BlackScholes:
         return _param_labels(self)
 
 
-class Dupire:
+class Dupire(_Described):
     """Dupire(spot, spots, times, vols, maxDt=0.25) (mcMdlDupire.h:76-111).
 
     vols[i][j] = local vol at spots[i], times[j] (spot major)."""
@@ -217,7 +250,7 @@ class Dupire:
 # ---------------------------------------------------------------- products
 
 
-class European:
+class European(_Described):
     """European(strike, exerciseDate, settlementDate=exerciseDate) (mcPrd.h:44-86)."""
 
     def __init__(self, strike, exerciseDate, settlementDate=None):
@@ -236,7 +269,7 @@ class European:
         return _product_timeline(self)
 
 
-class UOC:
+class UOC(_Described):
     """UOC(strike, barrier, maturity, monitorFreq, smooth, callPut=False) (mcPrd.h:149-208).
 
     Non-positive smoothing is replaced by 1e-8 as putBarrier does (store.h:153)."""
@@ -258,7 +291,7 @@ class UOC:
         return _product_timeline(self)
 
 
-class Europeans:
+class Europeans(_Described):
     """Europeans over (maturity, strike) pairs, maturity major (store.h:184-205, mcPrd.h:305-336)."""
 
     def __init__(self, maturities, strikes):
@@ -280,7 +313,7 @@ class Europeans:
         return _product_timeline(self)
 
 
-class Asian:
+class Asian(_Described):
     """Arithmetic-average option (no reference class; Product<T> interface,
     mcBase.h:122-152): max(mean of the spot on the fixing dates - strike, 0),
     paid at maturity.  Fixings at fixingFreq, 2 fixingFreq, ... up to the
@@ -429,9 +462,9 @@ def _num(num, nPath=None, precision=None):
 def mcParallelSimul(prd, mdl, nPath=None, num=None, per_path=True, precision=None):
     """mcParallelSimul (mcBase.h:314-400): per-path payoff matrix (nPath x nPay)
     when ``per_path``, plus device-reduced means and standard errors."""
-    dm, dp = mdl.descriptor(), prd.descriptor()
+    dm, dp = _desc(mdl), _desc(prd)
     n = _num(num, nPath, precision)
-    npay = _check(lib().cf_num_payoffs(ctypes.byref(dp)))
+    npay = _count(prd, dp, "npay", lib().cf_num_payoffs)
     means = np.zeros(npay)
     ses = np.zeros(npay)
     pp = np.zeros((n.num_path, npay)) if per_path else None
@@ -443,10 +476,10 @@ def mcParallelSimul(prd, mdl, nPath=None, num=None, per_path=True, precision=Non
 def mcParallelSimulAAD(prd, mdl, nPath=None, weights=None, num=None, per_path=False, precision=None):
     """mcParallelSimulAAD (mcBase.h:564-753) with the linear aggregator
     sum_k weights[k] * payoff_k (default: the first payoff, mcBase.h:425)."""
-    dm, dp = mdl.descriptor(), prd.descriptor()
+    dm, dp = _desc(mdl), _desc(prd)
     n = _num(num, nPath, precision)
-    npay = _check(lib().cf_num_payoffs(ctypes.byref(dp)))
-    nparam = _check(lib().cf_num_params(ctypes.byref(dm)))
+    npay = _count(prd, dp, "npay", lib().cf_num_payoffs)
+    nparam = _count(mdl, dm, "nparam", lib().cf_num_params)
     w = np.zeros(npay)
     if weights is None:
         w[0] = 1.0

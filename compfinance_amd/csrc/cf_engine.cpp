@@ -1139,12 +1139,9 @@ void enqueue_sequence(const Sequence& q, hipStream_t stream, hipEvent_t ev_kerne
     int rc = cfk::launch_path_kernel(q.cfg, q.P, q.grid, q.block, q.smem, stream);
     if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
     if (ev_kernel1) hip_check(hipEventRecord(ev_kernel1, stream), "hipEventRecord");
-    rc = cfk::launch_reduce_rows(q.block_out, q.grid, q.stride, q.reduced, stream);
+    rc = cfk::launch_reduce_rows(q.block_out, q.stride, q.reduced, q.ivbar_blocks, int(q.ivn), q.ivbar, q.grid,
+                                 stream);
     if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
-    if (q.ivn) {
-        rc = cfk::launch_reduce_rows(q.ivbar_blocks, q.grid, int(q.ivn), q.ivbar, stream);
-        if (rc != 0) hip_check(hipError_t(rc), "reduce launch");
-    }
 }
 
 void enqueue_call(const Prepared& PR, const CallShape& S, const CallBuffers& B, const cfk::LaunchParams& P,
@@ -1218,10 +1215,11 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     CallBuffers B;
     B.tables = static_cast<const unsigned char*>(PR.tables.p);
     B.block_out = static_cast<double*>(E.block_out.ensure(nb.block_out));
-    B.reduced = static_cast<double*>(E.reduced.ensure(nb.reduced));
+    // the reduced sums and the local-vol adjoints side by side, one download
+    B.reduced = static_cast<double*>(E.reduced.ensure((size_t(S.stride) + S.ivn) * 8));
     if (S.ivn) {
         B.ivbar_blocks = static_cast<double*>(E.ivbar_blocks.ensure(nb.ivbar_blocks));
-        B.ivbar = static_cast<double*>(E.ivbar.ensure(nb.ivbar));
+        B.ivbar = B.reduced + S.stride;
     }
     if (c.adjoint) {
         B.weights = static_cast<double*>(E.weights.ensure(nb.weights));
@@ -1235,17 +1233,25 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
 
     hip_check(hipEventRecord(E.ev[0], E.stream), "hipEventRecord");
     if (c.adjoint) {
-        std::memcpy(h_in, c.weights, size_t(S.npay) * 8);
-        hip_check(hipMemcpyAsync(B.weights, h_in, size_t(S.npay) * 8, hipMemcpyHostToDevice, E.stream), "upload");
+        // the device copy of the weights is reused while the call's weights are
+        // the last ones uploaded to the same buffer (only this path writes it)
+        const size_t wb = size_t(S.npay) * 8;
+        const bool same = E.weights_on_device == B.weights && E.weights_last.size() == size_t(S.npay) &&
+                          std::memcmp(E.weights_last.data(), c.weights, wb) == 0;
+        if (!same) {
+            std::memcpy(h_in, c.weights, wb);
+            hip_check(hipMemcpyAsync(B.weights, h_in, wb, hipMemcpyHostToDevice, E.stream), "upload");
+            E.weights_last.assign(c.weights, c.weights + S.npay);
+            E.weights_on_device = B.weights;
+        }
     }
     const cfk::LaunchParams P = launch_params(E, PR, S, B);
     const double setup_ms = ms_since(t_start);
     enqueue_call(PR, S, B, P, E.stream, E.ev[1], E.ev[2]);
     hip_check(hipEventRecord(E.ev[3], E.stream), "hipEventRecord");
 
-    hip_check(hipMemcpyAsync(h_out, B.reduced, size_t(S.stride) * 8, hipMemcpyDeviceToHost, E.stream), "download");
-    if (S.ivn)
-        hip_check(hipMemcpyAsync(h_out + S.stride, B.ivbar, S.ivn * 8, hipMemcpyDeviceToHost, E.stream), "download");
+    hip_check(hipMemcpyAsync(h_out, B.reduced, (size_t(S.stride) + S.ivn) * 8, hipMemcpyDeviceToHost, E.stream),
+              "download");
     if (B.per_path)
         hip_check(hipMemcpyAsync(c.per_path, B.per_path, size_t(S.n_path) * S.npay * 8, hipMemcpyDeviceToHost,
                                  E.stream), "download");

@@ -217,8 +217,11 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     uint32_t* d_dirs = nullptr;
-    DevBuf block_out, ivbar_blocks, reduced, ivbar, per_path, per_path_agg, ckpt, weights, normals;
+    DevBuf block_out, ivbar_blocks, reduced, per_path, per_path_agg, ckpt, weights, normals;
     PinnedBuf pin_in, pin_out;
+    // the aggregation weights last uploaded to `weights` (skipped when a call repeats them)
+    vector<double> weights_last;
+    const void* weights_on_device = nullptr;
     std::list<std::pair<string, std::shared_ptr<Prepared>>> plans;   // most recent first
     std::mutex plan_mu;   // the plan cache
     std::mutex sync_mu;   // the synchronous entry points' stream and buffers

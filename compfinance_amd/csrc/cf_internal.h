@@ -231,7 +231,10 @@ CF_PICK_DECL(1, 2)
 // Returns 0 on success, else a hipError_t value.
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block,
                        size_t smem, void* stream);
-int launch_reduce_rows(const double* in, int rows, int cols, double* out, void* stream);
+// Column sums of [rows][cols0] into out0 and of [rows][cols1] into out1
+// (cols1 may be 0), one launch.
+int launch_reduce_rows(const double* in0, int cols0, double* out0, const double* in1, int cols1, double* out1,
+                       int rows, void* stream);
 int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* blocks_per_cu);
 
 int launch_copy(const void* in, void* out, size_t bytes, void* stream);

@@ -92,14 +92,22 @@ int launch_mrg_normals(const uint64_t* jump, uint64_t seed1, uint64_t seed2, int
     return int(hipGetLastError());
 }
 
-// Column sums of a row-major [rows][cols] matrix.  A block takes 64 columns;
-// wave w sums rows w, w + 16, ... in order and the 16 partial sums are added
-// in wave order, so the result does not depend on scheduling.
-__global__ void __launch_bounds__(1024) reduce_rows_kernel(const double* in, int rows, int cols, double* out)
+// Column sums of two row-major matrices with the same row count,
+// [rows][cols0] and [rows][cols1] (cols1 may be 0), in one launch: blocks
+// [0, ceil(cols0 / 64)) take the first, the rest the second.  A block takes 64
+// columns; wave w sums rows w, w + 16, ... in order and the 16 partial sums
+// are added in wave order, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(1024) reduce_rows_kernel(const double* in0, int cols0, double* out0,
+                                                           const double* in1, int cols1, double* out1, int rows)
 {
     __shared__ double part[16][kWave];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int c = blockIdx.x * kWave + lane;
+    const int b0 = (cols0 + kWave - 1) / kWave;
+    const bool first = int(blockIdx.x) < b0;
+    const double* in = first ? in0 : in1;
+    double* out = first ? out0 : out1;
+    const int cols = first ? cols0 : cols1;
+    const int c = (first ? int(blockIdx.x) : int(blockIdx.x) - b0) * kWave + lane;
     double s = 0.0;
     if (c < cols)
         for (int r = w; r < rows; r += 16) s += in[size_t(r) * cols + c];
@@ -149,12 +157,14 @@ int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid,
     return int(e);
 }
 
-int launch_reduce_rows(const double* in, int rows, int cols, double* out, void* stream)
+int launch_reduce_rows(const double* in0, int cols0, double* out0, const double* in1, int cols1, double* out1,
+                       int rows, void* stream)
 {
     const int threads = 1024;
-    const int blocks = (cols + kWave - 1) / kWave;
-    hipLaunchKernelGGL(reduce_rows_kernel, dim3(blocks), dim3(threads), 0, static_cast<hipStream_t>(stream), in, rows,
-                       cols, out);
+    const int blocks = (cols0 + kWave - 1) / kWave + (cols1 + kWave - 1) / kWave;
+    if (blocks == 0) return 0;
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(blocks), dim3(threads), 0, static_cast<hipStream_t>(stream), in0,
+                       cols0, out0, in1, cols1, out1, rows);
     return int(hipGetLastError());
 }
 
