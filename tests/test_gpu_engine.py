@@ -162,3 +162,36 @@ def test_itemized_risks_match_oracle_per_payoff(engine, mname, pk):
     np.testing.assert_allclose(rep["risks"], risks, rtol=1e-12, atol=1e-12 * np.abs(risks).max())
     assert rep["payoffs"] == prd.payoffLabels()
     cf.clearStore()
+
+
+def test_repeated_and_changed_inputs_between_calls(engine):
+    """The engine keeps the last weights on the device and the binding keeps
+    each object's descriptor: repeats give the same results (risks to the
+    rounding of the LDS-atomic sums, whose order varies), and changed weights,
+    attributes and arrays (in place) are all seen."""
+    mdl, prd = _c3()
+    num = cf.NumericalParam(numPath=4096)
+    w1, w2 = np.array([1.0, 0.0]), np.array([0.25, 2.0])
+    r1 = cf.mcParallelSimulAAD(prd, mdl, weights=w1, num=num)
+    r2 = cf.mcParallelSimulAAD(prd, mdl, weights=w2, num=num)
+    r1b = cf.mcParallelSimulAAD(prd, mdl, weights=w1, num=num)
+    tol = 1e-12 * np.abs(r1.risks).max()
+    np.testing.assert_allclose(r1b.risks, r1.risks, rtol=1e-12, atol=tol)
+    assert r1b.agg_mean == r1.agg_mean
+    r12 = cf.mcParallelSimulAAD(prd, mdl, weights=w1 + w2, num=num)
+    np.testing.assert_allclose(r12.risks, r1.risks + r2.risks, rtol=1e-12, atol=1e-12 * np.abs(r12.risks).max())
+    # another product in between (more payoffs), then the first weights again
+    eus = cf.Europeans([1.0, 2.0, 3.0], [90.0, 100.0, 110.0])
+    cf.mcParallelSimulAAD(eus, mdl, weights=[1.0, 1.0, 1.0], num=num)
+    np.testing.assert_allclose(cf.mcParallelSimulAAD(prd, mdl, weights=w1, num=num).risks, r1.risks, rtol=1e-12,
+                               atol=tol)
+    # attribute assignment and in-place array changes reach the engine
+    prd.strike = 110.0
+    fresh = cf.UOC(**dict(W.barrier_args(), strike=110.0))
+    np.testing.assert_array_equal(cf.mcParallelSimul(prd, mdl, num=num, per_path=False).means,
+                                  cf.mcParallelSimul(fresh, mdl, num=num, per_path=False).means)
+    mdl.vols[3, 4] += 0.01
+    bumped = cf.Dupire(**W.dupire_args(30))
+    bumped.vols[3, 4] += 0.01
+    np.testing.assert_array_equal(cf.mcParallelSimul(prd, mdl, num=num, per_path=False).means,
+                                  cf.mcParallelSimul(prd, bumped, num=num, per_path=False).means)
