@@ -195,3 +195,24 @@ def test_repeated_and_changed_inputs_between_calls(engine):
     bumped.vols[3, 4] += 0.01
     np.testing.assert_array_equal(cf.mcParallelSimul(prd, mdl, num=num, per_path=False).means,
                                   cf.mcParallelSimul(prd, bumped, num=num, per_path=False).means)
+
+
+@pytest.mark.parametrize("prec, rtol", [(cf.FP64, 1e-12), (cf.FP32, 1e-5)])
+def test_barrier_edges_with_a_frozen_spot(engine, prec, rtol):
+    """Black-Scholes with a vanishing volatility keeps the spot at s0 on the
+    53 weekly dates: inside the smoothing zone the payoff is the closed form
+    f^53 (s0 - K); the top of the zone knocks out, the bottom leaves the
+    vanilla, and a 1e-8 width is the hard barrier."""
+    def pay(s0, smooth):
+        mdl = cf.BlackScholes(s0, 1e-9)
+        prd = cf.UOC(100.0, 150.0, 1.0, 1 / 52, smooth)
+        return cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=64, precision=prec)).payoffs[0]
+
+    s0, smooth = 148.0, 0.05
+    f = (150.0 + s0 * smooth - s0) / (2.0 * s0 * smooth)
+    assert pay(s0, smooth)[0] == pytest.approx(f ** 53 * (s0 - 100.0), rel=max(rtol, 1e-6))
+    assert pay(150.0 / 0.95 * (1 + 1e-6), smooth)[0] == 0.0
+    low = pay(150.0 / 1.05 * (1 - 1e-6), smooth)
+    assert low[0] == pytest.approx(low[1], rel=rtol)
+    assert pay(149.9, 1e-8)[0] == pytest.approx(49.9, rel=max(rtol, 1e-6))
+    assert pay(150.1, 1e-8)[0] == 0.0

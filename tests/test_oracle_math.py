@@ -142,3 +142,39 @@ def test_timeline_weekly_barrier():
     # a European at 3y is filled to maxDt steps
     t2, c2 = po.timeline(m, po.Product(strike=100.0, exerciseDate=3.0))
     assert len(t2) == 157 and c2.sum() == 1 and c2[-1]
+
+
+def _frozen_barrier_payoffs(s0, smooth):
+    """Per-path UOC payoffs (alive x vanilla, vanilla) under a Black-Scholes
+    model with a vanishing volatility, so the spot stays at s0 on every
+    monitoring date: alive = f^(dates) with f = (B + w - s0) / (2w), w = s0 x
+    smooth (mcPrd.h:247-287)."""
+    m = po.Model(spot=s0, vol=1e-9)
+    p = po.Product(strike=100.0, barrier=150.0, maturity=1.0, monitorFreq=1 / 52, smooth=smooth)
+    return po.value_paths(m, p, 4)[0]
+
+
+def test_barrier_is_continuous_at_both_ends_of_the_smoothing_zone():
+    """The smoothed knock-out has no jump: at the zone's top (s0 = B/(1 - smooth))
+    the factor reaches 0, at its bottom (s0 = B/(1 + smooth)) it reaches 1."""
+    smooth, eps = 0.05, 1e-7
+    top, bottom = 150.0 / (1.0 - smooth), 150.0 / (1.0 + smooth)
+    up, down = _frozen_barrier_payoffs(top + eps, smooth), _frozen_barrier_payoffs(top - eps, smooth)
+    assert up[0] == 0.0 and abs(down[0]) < 1e-30          # knocked out / alive factor ~ (eps/2w)^53
+    up, down = _frozen_barrier_payoffs(bottom + eps, smooth), _frozen_barrier_payoffs(bottom - eps, smooth)
+    assert down[0] == pytest.approx(down[1], rel=1e-12)    # below the zone: the vanilla
+    assert abs(up[0] - down[0]) < 1e-4                      # no jump entering the zone
+    # inside the zone the factor is the closed form f^n over the 53 dates (t = 0 included)
+    s0 = 148.0
+    w = s0 * smooth
+    f = (150.0 + w - s0) / (2.0 * w)
+    pay = _frozen_barrier_payoffs(s0, smooth)
+    assert pay[0] == pytest.approx(f ** 53 * (s0 - 100.0), rel=1e-6)
+
+
+def test_barrier_without_smoothing_is_the_hard_barrier():
+    """smooth -> 0 (putBarrier replaces a non-positive width by 1e-8): the
+    payoff is the vanilla below the barrier and 0 above it."""
+    below, above = _frozen_barrier_payoffs(149.9, 1e-8), _frozen_barrier_payoffs(150.1, 1e-8)
+    assert below[0] == pytest.approx(49.9, rel=1e-6) and below[0] == pytest.approx(below[1], rel=1e-12)
+    assert above[0] == 0.0 and above[1] == pytest.approx(50.1, rel=1e-6)
