@@ -181,12 +181,14 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.lo = o;          o = cf_align16(o + size_t(n_steps) * (lean ? 8 : kWave) * 4);
     L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
-    L.invn = o;        o = cf_align16(o + (real_bytes == 4 ? size_t(kInvNormalRows) * 4 * 4 : 0));
+    // LEAN reads the inverse-normal table and the bucket entries, like the
+    // local-vol table, from global memory through L1
+    L.invn = o;        o = cf_align16(o + (real_bytes == 4 && !lean ? size_t(kInvNormalRows) * 4 * 4 : 0));
     // interval (start, inverse width) pairs, interleaved
     L.knots = o;       o = cf_align16(o + (dupire ? size_t(2 * (n_spots + 1)) * real_bytes : 0));
-    L.bucket = o;      o = cf_align16(o + (dupire ? size_t(n_buckets) * (real_bytes == 4 ? sizeof(BucketEntry<float>)
-                                                                                  : sizeof(BucketEntry<double>))
-                                                 : 0));
+    L.bucket = o;      o = cf_align16(o + (dupire && !lean ? size_t(n_buckets) * (real_bytes == 4 ? sizeof(BucketEntry<float>)
+                                                                                            : sizeof(BucketEntry<double>))
+                                                           : 0));
     L.iv = o;          o = cf_align16(o + (dupire && !lean ? size_t(n_steps) * lv_cols(n_spots) * real_bytes : 0));
     L.drift = o;       o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));
     L.stdv = o;        o = cf_align16(o + (dupire ? 0 : size_t(n_steps) * real_bytes));

@@ -339,14 +339,15 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         }
     }
     for (int e = tid; e <= n; e += nthr) step_sample[e] = P.step_sample[e];
-    if constexpr (sizeof(R) == 4)
+    if constexpr (sizeof(R) == 4 && !LEAN)
         for (int e = tid; e < kInvNormalRows * 4; e += nthr) invn[e] = kInvNormalF32[e];
     if constexpr (DUP) {
         const R* gk = static_cast<const R*>(P.knots);
         const R* gi = static_cast<const R*>(P.inv_dx);
         const R* gv = static_cast<const R*>(P.iv);
         for (int e = tid; e <= m; e += nthr) { kxi[2 * e] = gk[e]; kxi[2 * e + 1] = gi[e]; }
-        for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = static_cast<const BucketEntry<R>*>(P.bucket)[e];
+        if constexpr (!LEAN)
+            for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = static_cast<const BucketEntry<R>*>(P.bucket)[e];
         if constexpr (!LEAN)
             for (int e = tid; e < n * mp; e += nthr) iv_lds[e] = gv[e];
     } else {
@@ -405,7 +406,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave) : nullptr;
     KnotSearch<R> ks;
     ks.knots = kxi + 2;     // the knots themselves: interval starts from interval 1
-    ks.table = bucket;
+    if constexpr (LEAN) ks.table = static_cast<const BucketEntry<R>*>(P.bucket); else ks.table = bucket;
     ks.m = m;
     ks.nb = P.n_buckets;
     ks.fix = P.bucket_fix;
@@ -472,9 +473,13 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         // normal of step i: from the Sobol point, or from the buffer of
         // pre-generated normals ([step][path] of the call's paths)
         const size_t ncol = size_t(active ? pidx : 0);
+        // the inverse-normal table: LDS, or (LEAN) the constant array itself
+        const R* invn_tab;
+        if constexpr (LEAN && sizeof(R) == 4) invn_tab = reinterpret_cast<const R*>(kInvNormalF32);
+        else invn_tab = invn;
         auto draw = [&](int i) -> R {
             if constexpr (NBUF) return nrm[size_t(i) * size_t(P.n_path) + ncol];
-            else return inv_normal(hi[i] ^ low_word(i), static_cast<const R*>(invn));
+            else return inv_normal(hi[i] ^ low_word(i), invn_tab);
         };
 
         ProductState<R> ps;

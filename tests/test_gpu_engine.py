@@ -216,3 +216,26 @@ def test_barrier_edges_with_a_frozen_spot(engine, prec, rtol):
     assert low[0] == pytest.approx(low[1], rel=rtol)
     assert pay(149.9, 1e-8)[0] == pytest.approx(49.9, rel=max(rtol, 1e-6))
     assert pay(150.1, 1e-8)[0] == 0.0
+
+
+@pytest.mark.parametrize("mode", [cf.AAD_CHECKPOINT, cf.AAD_RECOMPUTE])
+@pytest.mark.parametrize("prec", [cf.FP64, cf.FP32])
+def test_largest_serving_grid_fits(engine, prec, mode):
+    """The largest grid and timeline of the serving mix, 60 spots x 60 months
+    with 5 years of weekly monitoring (260 steps, the LEAN layout), runs in
+    both precisions and adjoint variants; FP64 matches the oracle tape."""
+    mk = W.dupire_args(60, 60)
+    pk = W.barrier_args(maturity=5.0)
+    mdl, prd = cf.Dupire(**mk), cf.UOC(**pk)
+    n = 512
+    res = cf.mcParallelSimulAAD(prd, mdl, n, weights=[1.0, 0.0], per_path=True,
+                                num=cf.NumericalParam(numPath=n, precision=prec, aadMode=mode))
+    assert cf.last_kernel_stats()["lean_layout"] == 1
+    pay, agg, risks = po.aad(po.Model(**mk), po.Product(**pk), n, 0)
+    scale = max(np.abs(risks).max(), 1e-300)
+    if prec == cf.FP64:
+        np.testing.assert_allclose(res.payoffs, pay, rtol=1e-11, atol=1e-11)
+        assert np.abs(res.risks - risks).max() <= 1e-10 * scale
+    else:
+        assert abs(res.agg_mean - agg.mean()) <= 1e-4 * max(abs(agg.mean()), 1e-12)
+        assert np.abs(res.risks - risks).max() <= 2e-3 * scale
