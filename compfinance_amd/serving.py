@@ -97,6 +97,7 @@ def summarize(reqs, results, wall, stats):
     ps = sum(r["path_steps"] for r in reqs)
     return {
         "requests": len(reqs),
+        "failed": int(stats["failed"]),
         "aad_requests": int(sum(r["aad"] for r in reqs)),
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
