@@ -14,7 +14,10 @@
 //     so the point of lane r in dimension d is  LO[d][r] ^ HI_q[d]:  LO is a
 //     per-block LDS table built once, HI_q a per-wave LDS row built per block
 //     of paths;
-//   * local-vol table, log-spot knots, timeline sample map in LDS;
+//   * local-vol table, interval table, knot search table, inverse-normal
+//     table and timeline sample map in LDS (the LEAN layout of long timelines
+//     and large grids reads the local-vol, knot search and inverse-normal
+//     tables from global memory through L1 instead);
 //   * payoffs reduced per wave with DPP cross-lane moves into per-wave LDS
 //     accumulators, then per block, then across blocks by a separate kernel
 //     in fixed order;
@@ -24,7 +27,8 @@
 //     local-vol adjoints accumulated with LDS atomics on a per-block
 //     (step x spot) grid.
 // This header holds the device helpers and the path kernel template; each
-// cf_kern_inst.hip build instantiates one (precision, adjoint) family.
+// cf_kern_inst.hip build instantiates one (precision, adjoint, source of
+// normals) family.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -150,8 +154,8 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, double)
 // (exponent and top three mantissa bits: one eighth of a binade of v) and
 // t in [-1, 1) (the low 20 mantissa bits).  Each row is a cubic in t fitted
 // to the same Moro function in FP64 (inv_normal_f32.inc, written by
-// gen_inv_normal_f32.py).  No logarithm, no branch: one 16-byte LDS read and
-// three FMAs.  Differences to the FP64 Moro values are below 5e-7 (the FP32
+// gen_inv_normal_f32.py).  No logarithm, no branch: one 16-byte table read
+// and three FMAs.  Differences to the FP64 Moro values are below 5e-7 (the FP32
 // rounding of the result in the far tail).
 __device__ __forceinline__ float inv_normal(uint32_t xi, const float* tab)
 {
