@@ -456,7 +456,8 @@ BookTables build_book(const ProductPlan& d, double width0)
     }
     BookTables B;
     auto put = [&B](const void* p, size_t bytes) {
-        const size_t off = (B.blob.size() + 7) & ~size_t(7);
+        // 16-byte aligned parts: the kernel reads a barrier's four parameters at once
+        const size_t off = (B.blob.size() + 15) & ~size_t(15);
         B.blob.resize(off + bytes);
         if (bytes) std::memcpy(B.blob.data() + off, p, bytes);
         return int(off);

@@ -445,7 +445,12 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     // the kernel's precision
     const R* bk_par = reinterpret_cast<const R*>(bkb + (sizeof(R) == 4 ? P.bk.bar_par_f : P.bk.bar_par));
     const R* bk_ginv = reinterpret_cast<const R*>(bkb + (sizeof(R) == 4 ? P.bk.grp_inv_f : P.bk.grp_inv));
-    auto bar_par = [&](int b, int k) { return bk_par[4 * b + k]; };
+    // barrier b's (strike, B + w, B - w, 1 / 2w): one aligned 16-byte (FP32)
+    // or two (FP64) LDS reads
+    struct alignas(16) BarPar {
+        R k, up, lo, i2s;
+    };
+    auto bar_par = [&](int b) { return reinterpret_cast<const BarPar*>(bk_par)[b]; };
     constexpr int NBAR = BOOK ? kMaxBookBarriers : 1, NGRP = BOOK ? kMaxBookGroups : 1;
     // Book barrier state in one real per barrier:
     //   v in [0, 1]   no exactly-zero smoothing factor yet, v = product of the
@@ -556,12 +561,14 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 if (bm | bl) {
 #pragma unroll
                     for (int b = 0; b < NBAR; ++b) {
+                        if (!(((bm | bl) >> b) & 1u)) continue;
+                        const BarPar bp = bar_par(b);
                         if ((bm >> b) & 1u) {
-                            const R bs = bar_par(b, 1), ms = bar_par(b, 2);
+                            const R bs = bp.up, ms = bp.lo;
                             const R v = bst[b];
                             const bool live = v > R(kBreached);
                             const bool zone = live && F > ms && !(F > bs);
-                            const R f = (bs - F) * bar_par(b, 3);
+                            const R f = (bs - F) * bp.i2s;
                             R nv = v;
                             nv = zone && f != R(0) && v >= R(-1) ? v * f : nv;          // one more factor
                             nv = zone && f == R(0) ? (v >= R(0) ? -v : R(kTwoZeros)) : nv;   // a zero factor
@@ -569,7 +576,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                             bst[b] = nv;
                         }
                         if ((bl >> b) & 1u) {
-                            const R k = bar_par(b, 0);
+                            const R k = bp.k;
                             const R intrinsic = bk_bar_put[b] ? (k - F > R(0) ? k - F : R(0))
                                                                          : (F - k > R(0) ? F - k : R(0));
                             const R euro = intrinsic / numv;
@@ -712,9 +719,11 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                     if (bm | bl) {
 #pragma unroll
                         for (int b = 0; b < NBAR; ++b) {
+                            if (!(((bm | bl) >> b) & 1u)) continue;
+                            const BarPar bp = bar_par(b);
                             if ((bl >> b) & 1u) {
                                 const int col = bk_bar_col[b];
-                                const R k = bar_par(b, 0);
+                                const R k = bp.k;
                                 const bool put = bk_bar_put[b] != 0;
                                 const R intrinsic = put ? (k - F > R(0) ? k - F : R(0))
                                                         : (F - k > R(0) ? F - k : R(0));
@@ -727,9 +736,9 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                             if ((bm >> b) & 1u) {
                                 // d alive / dF = -(product of the other factors) / 2w
                                 const R v = bst[b];
-                                if (v > R(kBreached) && F > bar_par(b, 2)) {   // smoothing zone
-                                    const R i2s = bar_par(b, 3);
-                                    const R f = (bar_par(b, 1) - F) * i2s;
+                                if (v > R(kBreached) && F > bp.lo) {   // smoothing zone
+                                    const R i2s = bp.i2s;
+                                    const R f = (bp.up - F) * i2s;
                                     // product of the other factors: v / f without a zero
                                     // factor, the non-zero ones for the only zero factor
                                     // (v >= 0 with f == 0 only after the product underflowed
