@@ -563,7 +563,9 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                     for (int b = 0; b < NBAR; ++b) {
                         if (!(((bm | bl) >> b) & 1u)) continue;
                         const BarPar bp = bar_par(b);
-                        if ((bm >> b) & 1u) {
+                        // the state changes only at or above the zone: skipped
+                        // when no lane of the wave is there
+                        if (((bm >> b) & 1u) && __any(F > bp.lo)) {
                             const R bs = bp.up, ms = bp.lo;
                             const R v = bst[b];
                             const bool live = v > R(kBreached);
