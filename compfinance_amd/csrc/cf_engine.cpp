@@ -1070,6 +1070,13 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
         P.inv_two_smooth = 1.0 / P.two_smooth;
         P.bar_smooth = pd.barrier + smooth;
         P.minus_smooth = pd.barrier - smooth;
+        // the state is log S in FP64 and log(S / S0) in FP32; a state at or
+        // below bar_xlo gives a spot under B - smooth after exp's rounding
+        P.bar_xlo = -HUGE_VAL;
+        if (P.minus_smooth > 0) {
+            const double xl = S.fp64 ? std::log(P.minus_smooth) : std::log(P.minus_smooth / md.spot);
+            P.bar_xlo = xl - (S.fp64 ? 1e-10 : 1e-4) * (1 + std::fabs(xl));
+        }
     }
     P.put = pd.put ? 1 : 0;
     P.pay_first = reinterpret_cast<const int*>(dt + PR.o_pf);

@@ -120,6 +120,8 @@ struct LaunchParams {
     int n_pay;
     double strike;
     double bar_smooth, minus_smooth, two_smooth, inv_two_smooth;
+    double bar_xlo;              // Dupire barrier: at or below this state the spot is under the
+                                 // smoothing zone (with a margin over exp's rounding)
     int put;
     const int* pay_first;        // Europeans [n_samples + 1]: payoffs of sample s
     const double* pay_strike;    // Europeans [n_pay]

@@ -637,6 +637,11 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         };
 
         // ---- forward sweep
+        // Dupire barrier: a sample other than the last one changes nothing
+        // while the spot is under the smoothing zone, so the spot is not
+        // formed there (forward and reverse)
+        constexpr bool UNDER = DUP && PRD == PRD_BARRIER;
+        const R x_lo = UNDER ? R(P.bar_xlo) : R(0);
         R x;
         if constexpr (DUP) x = R(P.x0); else x = R(s0);   // Dupire: log-spot state; BS: spot
         {
@@ -659,7 +664,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 x = x * rexp(drift[i] + stdv[i] * g);      // mcMdlBS.h:344-345
             }
             const int s = step_sample[i + 1];
-            if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
+            if (s >= 0 && (!UNDER || s == last || x > x_lo)) observe(s, DUP ? state_spot(x, s0) : x);
         }
 
         // ---- payoffs of the barrier (known at the end of the path)
@@ -813,7 +818,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         cv[k] = ws * b.wb;
                         cj[k] = b.j;
                         const int smp = step_sample[i];
-                        if (smp >= 0) {
+                        if (smp >= 0 && (!UNDER || smp == last || xi > x_lo)) {
                             const R S = state_spot(xi, s0);
                             R Fb, nb, db;
                             sample_adj(smp, S, R(1), R(1), Fb, nb, db);
@@ -891,7 +896,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         wb[k] = b.wb;
                         js[k] = b.j;
                         const int smp = step_sample[i];
-                        if (smp >= 0) {
+                        if (smp >= 0 && (!UNDER || smp == last || xi > x_lo)) {
                             const R S = state_spot(xi, s0);
                             R Fb, nb, db;
                             sample_adj(smp, S, R(1), R(1), Fb, nb, db);
