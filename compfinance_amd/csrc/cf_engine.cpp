@@ -475,6 +475,7 @@ BookTables build_book(const ProductPlan& d, double width0)
     L.bar_par = put(bar_par.data(), bar_par.size() * 8);
     const vector<float> bar_par_f(bar_par.begin(), bar_par.end());
     L.bar_par_f = put(bar_par_f.data(), bar_par_f.size() * 4);
+    for (size_t b = 0; b < bar_col.size(); ++b) B.lo_min = std::min(B.lo_min, bar_par[4 * b + 2]);
     L.bar_col = put(bar_col.data(), bar_col.size() * 4);
     L.bar_put = put(bar_put.data(), bar_put.size() * 4);
     L.grp_mask = put(grp_mask.data(), grp_mask.size() * 4);
@@ -1083,6 +1084,7 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     P.pay_strike = reinterpret_cast<const double*>(dt + PR.o_pk);
     P.book = dt + PR.o_book;
     P.bk = PR.book.L;
+    P.book_lo = PR.book.lo_min;
 
     P.block_out = B.block_out;
     P.per_path = B.per_path;

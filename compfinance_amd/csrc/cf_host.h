@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <list>
 #include <memory>
@@ -99,6 +100,7 @@ struct ProductPlan {
 struct BookTables {
     vector<unsigned char> blob;
     cfk::BookLayout L{};
+    double lo_min = HUGE_VAL;      // the lowest B - w of the barriers
 };
 
 struct ModelPlan {

@@ -127,6 +127,7 @@ struct LaunchParams {
     const double* pay_strike;    // Europeans [n_pay]
     const void* book;            // book tables, layout bk
     BookLayout bk;
+    double book_lo;              // the lowest B - w of the book's barriers (+inf without one)
 
     // pre-generated normals [n_steps][n_path] (mrg32k3a), or null: Sobol
     const void* normals;

@@ -451,6 +451,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         R k, up, lo, i2s;
     };
     auto bar_par = [&](int b) { return reinterpret_cast<const BarPar*>(bk_par)[b]; };
+    const R bk_lo = R(P.book_lo);   // the lowest B - w (rounding keeps it the lowest)
     constexpr int NBAR = BOOK ? kMaxBookBarriers : 1, NGRP = BOOK ? kMaxBookGroups : 1;
     // Book barrier state in one real per barrier:
     //   v in [0, 1]   no exactly-zero smoothing factor yet, v = product of the
@@ -556,8 +557,10 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                     emit(bk_van_col[k], (F - kk > R(0) ? F - kk : R(0)) / numv);
                 }
                 // up-and-out barriers (mcPrd.h:256-287, one state per barrier),
-                // branch free: breach, smoothing zone, exactly-zero factor
-                const uint32_t bm = bk_bar_mask[s], bl = bk_bar_last[s];
+                // branch free: breach, smoothing zone, exactly-zero factor;
+                // no state changes while no lane is at or above the lowest zone
+                const uint32_t bm0 = bk_bar_mask[s], bl = bk_bar_last[s];
+                const uint32_t bm = bm0 && __any(F > bk_lo) ? bm0 : 0u;
                 if (bm | bl) {
 #pragma unroll
                     for (int b = 0; b < NBAR; ++b) {
@@ -722,7 +725,9 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                             nb += -wk * (F - kk) / (numv * numv);
                         }
                     }
-                    const uint32_t bm = bk_bar_mask[s], bl = bk_bar_last[s];
+                    // no lane at or above the lowest zone: no survival adjoint
+                    const uint32_t bm0 = bk_bar_mask[s], bl = bk_bar_last[s];
+                    const uint32_t bm = bm0 && __any(F > bk_lo) ? bm0 : 0u;
                     if (bm | bl) {
 #pragma unroll
                         for (int b = 0; b < NBAR; ++b) {
