@@ -1388,6 +1388,7 @@ int cf_product_timeline(const cf_product* prd, double* times, int cap)
 {
     return guarded([&] {
         const auto p = plan_product(prd);
+        if (cap > 0 && !times) fail(CF_ERR_ARG, "null times");
         for (int i = 0; i < int(p.timeline.size()) && i < cap; ++i) times[i] = p.timeline[i];
         return int(p.timeline.size());
     });

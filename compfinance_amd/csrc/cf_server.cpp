@@ -150,6 +150,7 @@ struct Request {
     CallShape shape, shape_rc;
     size_t bytes = 0, bytes_rc = 0;
     bool use_rc = false;
+    bool waited = false;   // a cf_server_wait holds it
     int waits = 0;   // admission passes that found no room
     // results
     Call call;
@@ -617,6 +618,9 @@ int cf_server_wait(cf_server* srv, int64_t id, cf_result* out)
         Server& S = *reinterpret_cast<Server*>(srv);
         std::unique_lock<std::mutex> lk(S.mu);
         Request* r = find(S, id);
+        // one waiter per request: the waiter releases it
+        if (r->waited) fail(CF_ERR_ARG, "request is already being waited on");
+        r->waited = true;
         S.cv_done.wait(lk, [r] { return r->done; });
         std::unique_ptr<Request> owned = std::move(S.reqs[id]);
         S.reqs.erase(id);

@@ -329,7 +329,8 @@ int cf_server_submit(cf_server* srv, const cf_request* req, int64_t* id);
 /* 1 when the request's results are ready, 0 when not, negative on error. */
 int cf_server_poll(cf_server* srv, int64_t id);
 /* Blocks until the request is done, writes its results and releases it;
- * returns its status. */
+ * returns its status.  One wait per request: a second wait on the same id,
+ * while the first blocks or after it, returns an error. */
 int cf_server_wait(cf_server* srv, int64_t id, cf_result* out);
 int cf_server_get_stats(cf_server* srv, cf_server_stats* out);
 /* Finishes every queued request, then releases the server. */

@@ -148,3 +148,26 @@ def test_graphs_with_changing_seeds_and_weights(engine):
         ref = cf.mcParallelSimulAAD(prd, mdl, weights=w, num=num)
         assert abs(g["agg_mean"] - ref.agg_mean) <= 1e-12 * abs(ref.agg_mean)
         np.testing.assert_allclose(g["risks"], ref.risks, rtol=1e-12, atol=1e-12 * np.abs(ref.risks).max())
+
+
+def test_second_wait_on_a_request_is_an_error(engine):
+    """cf_server_wait releases a request: a second wait on its id, made while
+    the first blocks or after it returned, is an error and the first gets the
+    results."""
+    import ctypes
+    import threading
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    num = cf.NumericalParam(numPath=1 << 20, precision=cf.FP32)
+    lib = cf._server_lib()
+    with cf.Server(slots=1, hbmBudget=1 << 30) as srv:
+        rid = srv.submit(prd, mdl, num, aad=True)
+        first = {}
+        th = threading.Thread(target=lambda: first.update(res=srv.wait(rid)))
+        th.start()
+        out = cf.cf_result()
+        rc = lib.cf_server_wait(srv._h, int(rid), ctypes.byref(out))
+        th.join()
+        assert rc < 0
+        ref = cf.mcParallelSimulAAD(prd, mdl, num=num)
+        np.testing.assert_array_equal(first["res"]["means"], ref.payoff_means)
+        assert lib.cf_server_wait(srv._h, int(rid), ctypes.byref(out)) < 0   # released
