@@ -640,6 +640,9 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         };
 
         // ---- forward sweep
+        // adjoint kernels: waves in the forward (vector-ALU bound) take issue
+        // priority over waves in the reverse (bound by LDS atomics)
+        if constexpr (AD) __builtin_amdgcn_s_setprio(1);
         // Dupire barrier: a sample other than the last one changes nothing
         // while the spot is under the smoothing zone, so the spot is not
         // formed there (forward and reverse)
@@ -682,6 +685,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         }
 
         if constexpr (AD) {
+            __builtin_amdgcn_s_setprio(0);
             const double agg = active ? double(ps.aggp) : 0.0;
             const double as = wave_sum(agg), aq = wave_sum(agg * agg);
             if (lane == kSumLane) { accw[2 * npay] += as; accw[2 * npay + 1] += aq; }

@@ -151,9 +151,9 @@ def test_graphs_with_changing_seeds_and_weights(engine):
 
 
 def test_second_wait_on_a_request_is_an_error(engine):
-    """cf_server_wait releases a request: a second wait on its id, made while
-    the first blocks or after it returned, is an error and the first gets the
-    results."""
+    """cf_server_wait releases a request: of two concurrent waits on one id
+    exactly one gets the results, the other an error; a later wait is an
+    error too."""
     import ctypes
     import threading
     mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
@@ -161,13 +161,21 @@ def test_second_wait_on_a_request_is_an_error(engine):
     lib = cf._server_lib()
     with cf.Server(slots=1, hbmBudget=1 << 30) as srv:
         rid = srv.submit(prd, mdl, num, aad=True)
-        first = {}
-        th = threading.Thread(target=lambda: first.update(res=srv.wait(rid)))
-        th.start()
-        out = cf.cf_result()
-        rc = lib.cf_server_wait(srv._h, int(rid), ctypes.byref(out))
-        th.join()
-        assert rc < 0
+        means = [np.zeros(2), np.zeros(2)]
+        outs = [cf.cf_result(), cf.cf_result()]
+        rcs = [None, None]
+
+        def waiter(k):
+            outs[k].means, outs[k].cap_pay = cf._dptr(means[k]), 2
+            rcs[k] = lib.cf_server_wait(srv._h, int(rid), ctypes.byref(outs[k]))
+
+        ths = [threading.Thread(target=waiter, args=(k,)) for k in range(2)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        assert sorted(rc == 0 for rc in rcs) == [False, True], rcs
+        got = means[rcs.index(0)]
         ref = cf.mcParallelSimulAAD(prd, mdl, num=num)
-        np.testing.assert_array_equal(first["res"]["means"], ref.payoff_means)
-        assert lib.cf_server_wait(srv._h, int(rid), ctypes.byref(out)) < 0   # released
+        np.testing.assert_array_equal(got, ref.payoff_means)
+        assert lib.cf_server_wait(srv._h, int(rid), ctypes.byref(outs[0])) < 0   # released
