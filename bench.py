@@ -86,7 +86,7 @@ def _cpu_baseline(n_steps):
             return {"value": n * n_steps / (ms / 1e3), "unit": "paths*steps/s", "cores": cores,
                     "kind": "reference",
                     "sample": "reference AADriskOne (mcParallelSimulAAD, ThreadPool %d threads), 2^20 paths x %d "
-                              "steps, 30x36 grid, median of 5: %.1f ms" % (cores, n_steps, ms)}
+                              "steps, 30x36 grid, median of 5: %.1f ms" % (cores, n_steps, ms)}, ms
         except (subprocess.CalledProcessError, OSError, ValueError):
             pass
     n = 1 << 14
@@ -94,7 +94,38 @@ def _cpu_baseline(n_steps):
     po.aad(mdl, prd, n, 0)
     dt = time.perf_counter() - t0
     return {"value": n * n_steps / dt, "unit": "paths*steps/s", "cores": 1, "kind": "port",
-            "sample": "FP64 CPU restatement with tape AAD, 2^14 paths x %d steps, one thread: %.1f s" % (n_steps, dt)}
+            "sample": "FP64 CPU restatement with tape AAD, 2^14 paths x %d steps, one thread: %.1f s" % (n_steps,
+                                                                                                      dt)}, None
+
+
+def _cpu_c4_parts(uoc_ms, weights, gpu_wall_ms):
+    """The reference has no book and no Asian: beside the book, its
+    AADriskAggregate on the book's 30 Europeans alone (the book's notionals),
+    and its AADriskOne on one barrier (the C3 run: each of the book's 10
+    barriers costs the same on the same grid and steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    if uoc_ms is None or not po.ref_available():
+        return None
+    cores = os.cpu_count() or 1
+    try:
+        cores = min(cores, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    kind, kw = W.c4_parts()[0]
+    assert kind == "europeans"
+    nots = {i: float(weights[i]) for i in range(len(kw["strikes"]))}
+    try:
+        out = po.ref_aad(po.Model(**W.dupire_args(30)), po.Product(**kw), W.C4_PATHS, threads=cores, repeats=3,
+                         notionals=nots)
+    except (subprocess.CalledProcessError, OSError, ValueError):
+        return None
+    eu = out["wall_ms_median"]
+    total = eu + 10 * uoc_ms
+    return {"europeans_ms": eu, "uoc_ms": uoc_ms, "sum_of_parts_ms": total, "cores": cores, "kind": "reference",
+            "book_fp32_wall_ms": gpu_wall_ms, "sum_of_parts_over_book_fp32": total / gpu_wall_ms,
+            "sample": "reference AADriskAggregate on the 30 Europeans (2^20 paths, median of 3) plus 10 x its "
+                      "AADriskOne on one barrier (median of 5); the 5 Asians have no reference class"}
 
 
 def main():
@@ -247,7 +278,10 @@ def main():
         from compfinance_amd import serving
         out["serving_c5"] = serving.run_c5()
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = _cpu_baseline(n_steps)
+        out["cpu_baseline"], uoc_ms = _cpu_baseline(n_steps)
+        parts = _cpu_c4_parts(uoc_ms, bw, out["book_c4"]["fp32_wall_ms"])
+        if parts is not None:
+            out["cpu_c4_parts"] = parts
     print(json.dumps(out))
 
 
