@@ -470,16 +470,21 @@ BookTables build_book(const ProductPlan& d, double width0)
     L.van_first = put(van_first.data(), van_first.size() * 4);
     L.van_strike = put(van_strike.data(), van_strike.size() * 8);
     L.van_col = put(van_col.data(), van_col.size() * 4);
-    L.bar_mask = put(bar_mask.data(), bar_mask.size() * 4);
-    L.bar_last = put(bar_last.data(), bar_last.size() * 4);
+    // the four masks of a sample in one 16-byte row
+    vector<uint32_t> masks(size_t(ns) * 4);
+    for (int k = 0; k < ns; ++k) {
+        masks[4 * k] = bar_mask[k];
+        masks[4 * k + 1] = bar_last[k];
+        masks[4 * k + 2] = grp_mask[k];
+        masks[4 * k + 3] = grp_last[k];
+    }
+    L.masks = put(masks.data(), masks.size() * 4);
     L.bar_par = put(bar_par.data(), bar_par.size() * 8);
     const vector<float> bar_par_f(bar_par.begin(), bar_par.end());
     L.bar_par_f = put(bar_par_f.data(), bar_par_f.size() * 4);
     for (size_t b = 0; b < bar_col.size(); ++b) B.lo_min = std::min(B.lo_min, bar_par[4 * b + 2]);
     L.bar_col = put(bar_col.data(), bar_col.size() * 4);
     L.bar_put = put(bar_put.data(), bar_put.size() * 4);
-    L.grp_mask = put(grp_mask.data(), grp_mask.size() * 4);
-    L.grp_last = put(grp_last.data(), grp_last.size() * 4);
     L.grp_inv = put(grp_inv.data(), grp_inv.size() * 8);
     const vector<float> grp_inv_f(grp_inv.begin(), grp_inv.end());
     L.grp_inv_f = put(grp_inv_f.data(), grp_inv_f.size() * 4);

@@ -46,14 +46,12 @@ struct BookLayout {
     int van_first;    // [n_samples + 1] int: vanillas of sample s
     int van_strike;   // [n_van] double
     int van_col;      // [n_van] int: payoff column
-    int bar_mask;     // [n_samples] uint32: barriers observing at s
-    int bar_last;     // [n_samples] uint32: barriers maturing at s
+    int masks;        // [n_samples][4] uint32, 16-byte rows: barriers observing at s,
+                      // barriers maturing at s, schedules fixing at s, schedules ending at s
     int bar_par;      // [n_bar][4] double: strike, B + w, B - w, 1 / 2w
     int bar_par_f;    // [n_bar][4] the same in float
     int bar_col;      // [n_bar] int: column of the barrier payoff (vanilla next)
     int bar_put;      // [n_bar] int
-    int grp_mask;     // [n_samples] uint32: schedules fixing at s
-    int grp_last;     // [n_samples] uint32: schedules ending at s
     int grp_inv;      // [n_grp] double: 1 / number of fixings
     int grp_inv_f;    // [n_grp] float
     int grp_first;    // [n_grp + 1] int: Asians of each schedule

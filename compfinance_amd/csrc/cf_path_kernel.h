@@ -431,10 +431,12 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     const int* bk_van_first = reinterpret_cast<const int*>(bkb + P.bk.van_first);
     const double* bk_van_strike = reinterpret_cast<const double*>(bkb + P.bk.van_strike);
     const int* bk_van_col = reinterpret_cast<const int*>(bkb + P.bk.van_col);
-    const uint32_t* bk_bar_mask = reinterpret_cast<const uint32_t*>(bkb + P.bk.bar_mask);
-    const uint32_t* bk_bar_last = reinterpret_cast<const uint32_t*>(bkb + P.bk.bar_last);
-    const uint32_t* bk_grp_mask = reinterpret_cast<const uint32_t*>(bkb + P.bk.grp_mask);
-    const uint32_t* bk_grp_last = reinterpret_cast<const uint32_t*>(bkb + P.bk.grp_last);
+    // per sample: barriers observing, barriers maturing, schedules fixing,
+    // schedules ending (one aligned 16-byte LDS read)
+    struct alignas(16) SampleMasks {
+        uint32_t bar, bar_last, grp, grp_last;
+    };
+    const SampleMasks* bk_masks = reinterpret_cast<const SampleMasks*>(bkb + P.bk.masks);
     const int* bk_grp_first = reinterpret_cast<const int*>(bkb + P.bk.grp_first);
     const double* bk_as_strike = reinterpret_cast<const double*>(bkb + P.bk.as_strike);
     const int* bk_as_col = reinterpret_cast<const int*>(bkb + P.bk.as_col);
@@ -559,16 +561,18 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 // up-and-out barriers (mcPrd.h:256-287, one state per barrier),
                 // branch free: breach, smoothing zone, exactly-zero factor;
                 // no state changes while no lane is at or above the lowest zone
-                const uint32_t bm0 = bk_bar_mask[s], bl = bk_bar_last[s];
+                const SampleMasks sm = bk_masks[s];
+                const uint32_t bm0 = sm.bar, bl = sm.bar_last;
                 const uint32_t bm = bm0 && __any(F > bk_lo) ? bm0 : 0u;
                 if (bm | bl) {
 #pragma unroll
                     for (int b = 0; b < NBAR; ++b) {
                         if (!(((bm | bl) >> b) & 1u)) continue;
                         const BarPar bp = bar_par(b);
-                        // the state changes only at or above the zone: skipped
-                        // when no lane of the wave is there
-                        if (((bm >> b) & 1u) && __any(F > bp.lo)) {
+                        // the state changes only at or above the zone and only
+                        // before a breach: skipped when no live lane of the wave
+                        // is there
+                        if (((bm >> b) & 1u) && __any(F > bp.lo && bst[b] > R(kBreached))) {
                             const R bs = bp.up, ms = bp.lo;
                             const R v = bst[b];
                             const bool live = v > R(kBreached);
@@ -594,7 +598,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                     }
                 }
                 // arithmetic averages: running sums per fixing schedule
-                const uint32_t gm = bk_grp_mask[s], gl = bk_grp_last[s];
+                const uint32_t gm = sm.grp, gl = sm.grp_last;
                 if (gm) {
 #pragma unroll
                     for (int g = 0; g < NGRP; ++g) {
@@ -730,7 +734,8 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                         }
                     }
                     // no lane at or above the lowest zone: no survival adjoint
-                    const uint32_t bm0 = bk_bar_mask[s], bl = bk_bar_last[s];
+                    const SampleMasks sm = bk_masks[s];
+                    const uint32_t bm0 = sm.bar, bl = sm.bar_last;
                     const uint32_t bm = bm0 && __any(F > bk_lo) ? bm0 : 0u;
                     if (bm | bl) {
 #pragma unroll
@@ -766,7 +771,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                             }
                         }
                     }
-                    const uint32_t gm = bk_grp_mask[s], gl = bk_grp_last[s];
+                    const uint32_t gm = sm.grp, gl = sm.grp_last;
                     if (gm) {
 #pragma unroll
                         for (int g = 0; g < NGRP; ++g) {
