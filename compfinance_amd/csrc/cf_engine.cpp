@@ -23,6 +23,7 @@
 #include <cstring>
 #include <functional>
 #include <iomanip>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -912,17 +913,19 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
         P->o_std = st.add(b);
     }
     if (dup) {
-        // bucket entries: the bucket's knot count and its next two knots
+        // bucket entries: the bucket's knot count and its next two knots; NaN
+        // past the last knot, which no state (not even +inf) is >= to
         const int top = md.n_spots - 1;
         auto entries = [&](auto real, const auto& kn) {
             using T = decltype(real);
+            const T none = std::numeric_limits<T>::quiet_NaN();
             vector<cfk::BucketEntry<T>> e(size_t(P->bt.nb));
             for (int b = 0; b < P->bt.nb; ++b) {
                 const int lo = P->bt.table[size_t(b)];
                 e[size_t(b)].lo = lo;
                 e[size_t(b)].pad = 0;
-                e[size_t(b)].k0 = T(kn[size_t(std::min(lo, top))]);
-                e[size_t(b)].k1 = T(kn[size_t(std::min(lo + 1, top))]);
+                e[size_t(b)].k0 = lo <= top ? T(kn[size_t(lo)]) : none;
+                e[size_t(b)].k1 = lo + 1 <= top ? T(kn[size_t(lo + 1)]) : none;
             }
             P->o_bucket = st.add(e);
         };

@@ -157,8 +157,9 @@ CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 // dimension (the low word is formed in registers) and reads the local-vol
 // table from global memory through the caches.
 // Entry of the knot search table: bucket b holds lo, the number of knots in
-// earlier buckets, and the next two knots (indices clamped to the grid), so
-// one LDS access brings all a search compares against.
+// earlier buckets, and the next two knots (NaN past the last knot, so that
+// their comparisons fail), so one LDS access brings all a search compares
+// against.
 template <typename R>
 struct alignas(16) BucketEntry {
     R k0, k1;

@@ -219,10 +219,10 @@ __device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
     bf = bf > R(0) ? bf : R(0);
     bf = bf < R(s.nb - 1) ? bf : R(s.nb - 1);
     // one read: the bucket's knot count and its first two candidate knots
-    // (clamped to the grid); the knots increase, so each one <= x adds one
+    // (NaN past the last knot); the knots increase, so each one <= x adds one
     const BucketEntry<R> e = s.table[int(bf)];
     const int lo = e.lo, top = s.m - 1;
-    int c = lo + int((lo <= top) & (e.k0 <= x)) + int((lo + 1 <= top) & (e.k1 <= x));
+    int c = lo + int(e.k0 <= x) + int(e.k1 <= x);
     for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
         const R kc = s.knots[2 * (c < top ? c : top)];
         c += int((c <= top) & (kc <= x));
