@@ -999,7 +999,9 @@ CallShape shape_call(const Engine& E, const Prepared& PR, const cf_num* num, boo
     S.fp64 = PR.cfg.precision == 0;
     S.waves = PR.block / cfk::kWave;
     const int64_t q0 = (S.first + 1) >> 6, q1 = (S.first + S.n_path) >> 6;
-    const int64_t want = (q1 - q0 + 1 + S.waves - 1) / S.waves;
+    S.merge = cfk::merge_tail_supported(S.fp64, PR.cfg.adjoint) && cfk::merge_tail_blocks(S.first, S.n_path);
+    const int64_t blocks = q1 - q0 + 1 - (S.merge ? 1 : 0);
+    const int64_t want = (blocks + S.waves - 1) / S.waves;
     S.grid = int(std::min<int64_t>(want, int64_t(PR.per_cu) * E.num_cus));
     S.slots = S.grid * S.waves;
     S.npay = PR.pd.n_pay;
@@ -1047,6 +1049,7 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     std::memset(&P, 0, sizeof P);   // padding too: sequences compare byte for byte
     P.first_path = S.first;
     P.n_path = S.n_path;
+    P.merge_tail = S.merge ? 1 : 0;
     P.n_steps = md.n_steps;
     P.dirs = E.d_dirs;
     P.n_spots = md.n_spots;

@@ -269,6 +269,7 @@ struct CallShape {
     int grid = 0, waves = 0, slots = 0, stride = 0, npay = 0, ck_rows = 0, n_steps = 0;
     size_t ivn = 0;                 // Dupire adjoint: steps x spots
     bool adjoint = false, fp64 = false;
+    bool merge = false;             // the last block of 64 Sobol indices runs in the first block's lanes
 };
 CallShape shape_call(const Engine& E, const Prepared& PR, const cf_num* num, bool adjoint);
 

@@ -82,6 +82,8 @@ struct LaunchParams {
     // paths [first_path, first_path + n_path); path p uses Sobol point p + 1
     int64_t first_path;
     int64_t n_path;
+    int merge_tail;              // 1: the last block of 64 Sobol indices runs in the first block's
+                                 // idle low lanes (see merge_tail_blocks)
     int n_steps;                 // simulation steps = Sobol dimension
     const uint32_t* dirs;        // [32][kSobolDims] direction numbers
 
@@ -145,11 +147,26 @@ struct LaunchParams {
 
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
-    size_t lo, hi, step_sample, invn, knots, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book, acc,
+    size_t lo, hi, hi_tail, step_sample, invn, knots, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book, acc,
         ivbar, total;
 };
 
 CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+// Paths [first, first + n) use Sobol indices [first + 1, first + n], taken in
+// blocks of 64 aligned indices.  When the last block's indices (lanes 0..(hi
+// mod 64)) all fall below the first block's first lane (lo mod 64), the last
+// block runs in the first block's idle lanes.  Blocks of 64 then number
+// (hi >> 6) - (lo >> 6) instead of one more.
+CF_HD inline bool merge_tail_blocks(int64_t first, int64_t n)
+{
+    const int64_t lo = first + 1, hi = first + n;
+    return (hi >> 6) > (lo >> 6) && (hi & 63) < (lo & 63);
+}
+
+// Kernels that run the merge: the adjoint kernels and the FP64 ones.  The
+// host sets merge_tail only for these, and sizes the grid to match.
+CF_HD constexpr bool merge_tail_supported(bool fp64, bool adjoint) { return fp64 || adjoint; }
 
 // Two layouts.  FULL keeps the per-lane low Sobol words (one 32-bit word per
 // dimension and lane) and the local-vol table in LDS.  LEAN, for long
@@ -182,6 +199,7 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     size_t o = 0;
     L.lo = o;          o = cf_align16(o + size_t(n_steps) * (lean ? 8 : kWave) * 4);
     L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
+    L.hi_tail = o;     o = cf_align16(o + (merge_tail_supported(real_bytes == 8, adjoint) ? size_t(n_steps) * 4 : 0));
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
     // LEAN reads the inverse-normal table and the bucket entries, like the
     // local-vol table, from global memory through L1

@@ -399,6 +399,16 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     double* accw = acc + wave * stride;
     const int64_t nlo = P.first_path + 1, nhi = P.first_path + P.n_path;   // Sobol indices
     const int64_t q0 = nlo >> 6, q1 = nhi >> 6;
+    // the last block's paths ride in the first block's idle low lanes when
+    // they fit there (e.g. 2^k paths from path 0: Sobol indices 1..2^k), so no
+    // wave makes an extra pass for them; those lanes read their high words
+    // from the block's tail row
+    // (adjoint and FP64 kernels: merge_tail_supported; the FP32 pricing
+    // kernel measured slower with the merge code)
+    constexpr bool MERGE = merge_tail_supported(sizeof(R) == 8, AD);
+    const bool merge = MERGE && P.merge_tail != 0;
+    const int64_t qe = merge ? q1 - 1 : q1;
+    uint32_t* hi_tail = reinterpret_cast<uint32_t*>(smem + L.hi_tail);
     const int slot = blockIdx.x * waves + wave;
     // checkpoint slot of the wave: [n][64] states (Dupire) or normals
     // (Black-Scholes); recompute variant: [ceil(n / kSeg)][64] Dupire states
@@ -463,23 +473,33 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     //   v = -4        breached (survival 0, no derivative)
     constexpr float kTwoZeros = -2.0f, kBreached = -4.0f;
 
-    for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= q1; q += int64_t(gridDim.x) * waves) {
-        // ---- high Sobol words of this block of 64 indices
+    for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= qe; q += int64_t(gridDim.x) * waves) {
+        const bool tail_wave = MERGE && merge && q == q0;
+        // ---- high Sobol words of this block of 64 indices (and of the
+        // merged last block)
         if constexpr (!NBUF) {
-            const uint32_t gq = uint32_t(q ^ (q >> 1));
-            for (int d = lane; d < n; d += kWave) {
-                uint32_t x = (q & 1) ? P.dirs[5 * kSobolDims + d] : 0u;
-                uint32_t b = gq;
-                while (b) {
-                    const int j = __builtin_ctz(b);
-                    x ^= P.dirs[(j + 6) * kSobolDims + d];
-                    b &= b - 1u;
+            auto fill_hi = [&](uint32_t* row, int64_t qb) {
+                const uint32_t gq = uint32_t(qb ^ (qb >> 1));
+                for (int d = lane; d < n; d += kWave) {
+                    uint32_t x = (qb & 1) ? P.dirs[5 * kSobolDims + d] : 0u;
+                    uint32_t b = gq;
+                    while (b) {
+                        const int j = __builtin_ctz(b);
+                        x ^= P.dirs[(j + 6) * kSobolDims + d];
+                        b &= b - 1u;
+                    }
+                    row[d] = x;
                 }
-                hi[d] = x;
-            }
+            };
+            fill_hi(hi, q);
+            if constexpr (MERGE)
+                if (tail_wave) fill_hi(hi_tail, q1);
             wave_sync_lds();
         }
-        const int64_t sidx = q * kWave + lane;
+        // lanes 0..(nhi mod 64) of the first block take the last block's paths
+        const bool tail = MERGE && tail_wave && lane <= int(nhi & (kWave - 1));
+        const int64_t sidx = (MERGE && tail ? q1 : q) * kWave + lane;
+        const uint32_t* hl = MERGE && tail ? hi_tail : hi;
         const bool active = sidx >= nlo && sidx <= nhi;
         const int64_t pidx = sidx - nlo;
         // normal of step i: from the Sobol point, or from the buffer of
@@ -491,7 +511,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
         else invn_tab = invn;
         auto draw = [&](int i) -> R {
             if constexpr (NBUF) return nrm[size_t(i) * size_t(P.n_path) + ncol];
-            else return inv_normal(hi[i] ^ low_word(i), invn_tab);
+            else return inv_normal(hl[i] ^ low_word(i), invn_tab);
         };
 
         ProductState<R> ps;

@@ -43,13 +43,15 @@ def test_path_ranges_reproduce_the_full_run(engine, prec):
     n = 5000
     full = cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=n, precision=prec)).payoffs
     parts = []
-    for first, cnt in [(0, 1), (1, 63), (64, 1), (65, 1000), (1065, 3935)]:
+    # (1, 63), (1065, 3935) and (4900, 100): the last block of 64 Sobol indices
+    # runs in the first block's idle lanes
+    for first, cnt in [(0, 1), (1, 63), (64, 1), (65, 1000), (1065, 3835), (4900, 100)]:
         parts.append(cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=cnt, firstPath=first,
                                                                         precision=prec)).payoffs)
     np.testing.assert_array_equal(np.concatenate(parts), full)
 
 
-@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 129])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 128, 129, 192])
 def test_ragged_path_counts(engine, n):
     mdl, prd = _c3()
     om, op = po.Model(**W.dupire_args(30)), po.Product(**W.barrier_args())
