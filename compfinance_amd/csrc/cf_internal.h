@@ -191,6 +191,13 @@ constexpr int kInvNormalRows = 31 * 8 + 1;
 // c + 1, flat outside the knots (the first and last value repeated).
 CF_HD inline int lv_cols(int n_spots) { return n_spots + 2; }
 
+// Doubles between the starts of the two LDS local-vol adjoint grids: the grid
+// plus 8 doubles (64 bytes, half the banks), so that a node of the second grid
+// shares no bank with the same node of the first (an even and an odd lane on
+// one node in the same atomic instruction).
+constexpr int kIvbarCopyPad = 8;
+CF_HD inline int ivbar_copy_stride(int n_steps, int n_spots) { return n_steps * lv_cols(n_spots) + kIvbarCopyPad; }
+
 CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int n_pay, int waves,
                                     int real_bytes, bool dupire, bool europeans, bool adjoint, bool lean,
                                     int n_buckets, int book_bytes, int ivbar_copies)
@@ -217,7 +224,7 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
     L.book = o;        o = cf_align16(o + size_t(book_bytes));
     L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
-    L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * n_steps * lv_cols(n_spots) * 8
+    L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * ivbar_copy_stride(n_steps, n_spots) * 8
                                                              : 0));
     L.total = o;
     return L;

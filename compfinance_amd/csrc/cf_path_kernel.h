@@ -371,7 +371,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     }
     for (int e = tid; e < waves * stride; e += nthr) acc[e] = 0.0;
     if constexpr (AD && DUP)
-        for (int e = tid; e < P.ivbar_copies * n * mp; e += nthr) ivbar[e] = 0.0;
+        for (int e = tid; e < P.ivbar_copies * ivbar_copy_stride(n, m); e += nthr) ivbar[e] = 0.0;
     __syncthreads();
 
     const int wave = tid >> 6, lane = tid & 63;
@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     const R* nrm = static_cast<const R*>(P.normals);
     // this lane's local-vol adjoint grid: odd lanes use the second copy, which
     // halves the lanes contending for one address in an atomic
-    double* ivbl = ivbar + ((P.ivbar_copies > 1 && (lane & 1)) ? n * mp : 0);
+    double* ivbl = ivbar + ((P.ivbar_copies > 1 && (lane & 1)) ? ivbar_copy_stride(n, m) : 0);
     // per-lane parameter adjoints, summed over the lane's paths
     double padj[kMaxParamAdj] = {0.0, 0.0, 0.0, 0.0};
 
@@ -1036,7 +1036,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
             const int i = e / m, j = e - i * m;
             double v = 0.0;
             for (int c = 0; c < P.ivbar_copies; ++c) {
-                const double* r = ivbar + size_t(c) * n * mp + size_t(i) * mp;
+                const double* r = ivbar + size_t(c) * ivbar_copy_stride(n, m) + size_t(i) * mp;
                 v += r[j + 1];
                 if (j == 0) v += r[0];
                 if (j == m - 1) v += r[m + 1];
