@@ -107,9 +107,10 @@ def lib():
             "cf_param_label": (ctypes.c_int, [P(cf_model), ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]),
             "cf_product_timeline": (ctypes.c_int, [P(cf_product), _c_double_p, ctypes.c_int]),
             "cf_sim_timeline": (ctypes.c_int, [P(cf_model), P(cf_product), _c_double_p, P(ctypes.c_int), ctypes.c_int]),
-            "cf_mc_simul": (ctypes.c_int, [P(cf_model), P(cf_product), P(cf_num), _c_double_p, _c_double_p, _c_double_p]),
-            "cf_mc_simul_aad": (ctypes.c_int, [P(cf_model), P(cf_product), P(cf_num), _c_double_p, _c_double_p,
-                                               _c_double_p, _c_double_p, _c_double_p, _c_double_p]),
+            # the simulation entry points take raw addresses (see _addr): one
+            # integer per buffer is the cheapest argument to convert
+            "cf_mc_simul": (ctypes.c_int, [P(cf_model), P(cf_product), P(cf_num)] + [ctypes.c_void_p] * 3),
+            "cf_mc_simul_aad": (ctypes.c_int, [P(cf_model), P(cf_product), P(cf_num)] + [ctypes.c_void_p] * 6),
             "cf_aad_footprint": (ctypes.c_int64, [P(cf_model), P(cf_product), P(cf_num)]),
             "cf_last_error": (ctypes.c_char_p, []),
             "cf_last_kernel_stats": (ctypes.c_int, [P(cf_kernel_stats)]),
@@ -151,6 +152,11 @@ def _check(rc):
 
 def _dptr(a):
     return a.ctypes.data_as(_c_double_p) if a is not None else None
+
+
+def _addr(a):
+    """Address of a contiguous float64 array's data, or None."""
+    return a.ctypes.data if a is not None else None
 
 
 def _f64(x):
@@ -465,11 +471,12 @@ def mcParallelSimul(prd, mdl, nPath=None, num=None, per_path=True, precision=Non
     dm, dp = _desc(mdl), _desc(prd)
     n = _num(num, nPath, precision)
     npay = _count(prd, dp, "npay", lib().cf_num_payoffs)
-    means = np.zeros(npay)
-    ses = np.zeros(npay)
+    # one buffer for the means and the standard errors
+    buf = np.zeros(2 * npay)
+    means, ses = buf[:npay], buf[npay:]
     pp = np.zeros((n.num_path, npay)) if per_path else None
-    _check(lib().cf_mc_simul(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n), _dptr(means), _dptr(ses),
-                             _dptr(pp)))
+    base = buf.ctypes.data
+    _check(lib().cf_mc_simul(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n), base, base + 8 * npay, _addr(pp)))
     return SimulResults(means, ses, pp)
 
 
@@ -480,19 +487,19 @@ def mcParallelSimulAAD(prd, mdl, nPath=None, weights=None, num=None, per_path=Fa
     n = _num(num, nPath, precision)
     npay = _count(prd, dp, "npay", lib().cf_num_payoffs)
     nparam = _count(mdl, dm, "nparam", lib().cf_num_params)
-    w = np.zeros(npay)
+    # one buffer: weights, payoff means, aggregate mean, risks
+    buf = np.zeros(2 * npay + 1 + nparam)
+    w, means, risks = buf[:npay], buf[npay:2 * npay], buf[2 * npay + 1:]
     if weights is None:
         w[0] = 1.0
     else:
         w[:] = _f64(weights)
-    means = np.zeros(npay)
-    agg = np.zeros(1)
-    risks = np.zeros(nparam)
     pp = np.zeros((n.num_path, npay)) if per_path else None
     pa = np.zeros(n.num_path) if per_path else None
-    _check(lib().cf_mc_simul_aad(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n), _dptr(w), _dptr(means),
-                                 _dptr(agg), _dptr(risks), _dptr(pp), _dptr(pa)))
-    return AADSimulResults(pp, pa, risks, means, float(agg[0]))
+    base = buf.ctypes.data
+    _check(lib().cf_mc_simul_aad(ctypes.byref(dm), ctypes.byref(dp), ctypes.byref(n), base, base + 8 * npay,
+                                 base + 16 * npay, base + 8 * (2 * npay + 1), _addr(pp), _addr(pa)))
+    return AADSimulResults(pp, pa, risks, means, float(buf[2 * npay]))
 
 
 def aad_footprint(prd, mdl, num):
