@@ -402,9 +402,8 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     // the last block's paths ride in the first block's idle low lanes when
     // they fit there (e.g. 2^k paths from path 0: Sobol indices 1..2^k), so no
     // wave makes an extra pass for them; those lanes read their high words
-    // from the block's tail row
-    // (adjoint and FP64 kernels: merge_tail_supported; the FP32 pricing
-    // kernel measured slower with the merge code)
+    // from the block's tail row.  Adjoint and FP64 kernels only
+    // (merge_tail_supported): the FP32 pricing kernel measured slower with it.
     constexpr bool MERGE = merge_tail_supported(sizeof(R) == 8, AD);
     const bool merge = MERGE && P.merge_tail != 0;
     const int64_t qe = merge ? q1 - 1 : q1;
