@@ -999,7 +999,7 @@ CallShape shape_call(const Engine& E, const Prepared& PR, const cf_num* num, boo
     S.fp64 = PR.cfg.precision == 0;
     S.waves = PR.block / cfk::kWave;
     const int64_t q0 = (S.first + 1) >> 6, q1 = (S.first + S.n_path) >> 6;
-    S.merge = cfk::merge_tail_supported(S.fp64, PR.cfg.adjoint) && cfk::merge_tail_blocks(S.first, S.n_path);
+    S.merge = cfk::merge_tail_blocks(S.first, S.n_path);
     const int64_t blocks = q1 - q0 + 1 - (S.merge ? 1 : 0);
     const int64_t want = (blocks + S.waves - 1) / S.waves;
     S.grid = int(std::min<int64_t>(want, int64_t(PR.per_cu) * E.num_cus));

@@ -164,10 +164,6 @@ CF_HD inline bool merge_tail_blocks(int64_t first, int64_t n)
     return (hi >> 6) > (lo >> 6) && (hi & 63) < (lo & 63);
 }
 
-// Kernels that run the merge: the adjoint kernels and the FP64 ones.  The
-// host sets merge_tail only for these, and sizes the grid to match.
-CF_HD constexpr bool merge_tail_supported(bool fp64, bool adjoint) { return fp64 || adjoint; }
-
 // Two layouts.  FULL keeps the per-lane low Sobol words (one 32-bit word per
 // dimension and lane) and the local-vol table in LDS.  LEAN, for long
 // timelines or large grids, keeps only direction numbers 0..5 of each
@@ -206,7 +202,7 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     size_t o = 0;
     L.lo = o;          o = cf_align16(o + size_t(n_steps) * (lean ? 8 : kWave) * 4);
     L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
-    L.hi_tail = o;     o = cf_align16(o + (merge_tail_supported(real_bytes == 8, adjoint) ? size_t(n_steps) * 4 : 0));
+    L.hi_tail = o;     o = cf_align16(o + size_t(n_steps) * 4);
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
     // LEAN reads the inverse-normal table and the bucket entries, like the
     // local-vol table, from global memory through L1

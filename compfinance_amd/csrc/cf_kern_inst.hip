@@ -5,11 +5,18 @@
 // families compile in parallel.
 #include <type_traits>
 
-#include "cf_path_kernel.h"
-
 #if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ) || !defined(CF_INST_NBUF)
 #error "CF_INST_REAL, CF_INST_ADJ and CF_INST_NBUF must be defined"
 #endif
+
+// FP32 pricing kernels run two 1024-thread blocks per CU (LDS allows it for
+// the benchmark grids).  Measured on gfx950: at 92 scalar registers only one
+// block stayed resident (half the waves); at most 80 keeps both.
+#if CF_INST_REAL == 1 && CF_INST_ADJ == 0
+#define CF_PATH_KERNEL_ATTR __attribute__((amdgpu_num_sgpr(80)))
+#endif
+
+#include "cf_path_kernel.h"
 
 namespace cfk {
 namespace {

@@ -290,7 +290,11 @@ struct ProductState {
 // ------------------------------------------------------------------ kernel
 
 template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF>
-__global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) path_kernel(const LaunchParams P)
+#ifndef CF_PATH_KERNEL_ATTR
+#define CF_PATH_KERNEL_ATTR
+#endif
+__global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) CF_PATH_KERNEL_ATTR
+path_kernel(const LaunchParams P)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool DUP = MODEL == MODEL_DUPIRE;
@@ -402,10 +406,8 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     // the last block's paths ride in the first block's idle low lanes when
     // they fit there (e.g. 2^k paths from path 0: Sobol indices 1..2^k), so no
     // wave makes an extra pass for them; those lanes read their high words
-    // from the block's tail row.  Adjoint and FP64 kernels only
-    // (merge_tail_supported): the FP32 pricing kernel measured slower with it.
-    constexpr bool MERGE = merge_tail_supported(sizeof(R) == 8, AD);
-    const bool merge = MERGE && P.merge_tail != 0;
+    // from the block's tail row
+    const bool merge = P.merge_tail != 0;
     const int64_t qe = merge ? q1 - 1 : q1;
     uint32_t* hi_tail = reinterpret_cast<uint32_t*>(smem + L.hi_tail);
     const int slot = blockIdx.x * waves + wave;
@@ -473,7 +475,7 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
     constexpr float kTwoZeros = -2.0f, kBreached = -4.0f;
 
     for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= qe; q += int64_t(gridDim.x) * waves) {
-        const bool tail_wave = MERGE && merge && q == q0;
+        const bool tail_wave = merge && q == q0;
         // ---- high Sobol words of this block of 64 indices (and of the
         // merged last block)
         if constexpr (!NBUF) {
@@ -491,14 +493,13 @@ __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) pa
                 }
             };
             fill_hi(hi, q);
-            if constexpr (MERGE)
-                if (tail_wave) fill_hi(hi_tail, q1);
+            if (tail_wave) fill_hi(hi_tail, q1);
             wave_sync_lds();
         }
         // lanes 0..(nhi mod 64) of the first block take the last block's paths
-        const bool tail = MERGE && tail_wave && lane <= int(nhi & (kWave - 1));
-        const int64_t sidx = (MERGE && tail ? q1 : q) * kWave + lane;
-        const uint32_t* hl = MERGE && tail ? hi_tail : hi;
+        const bool tail = tail_wave && lane <= int(nhi & (kWave - 1));
+        const int64_t sidx = (tail ? q1 : q) * kWave + lane;
+        const uint32_t* hl = tail ? hi_tail : hi;
         const bool active = sidx >= nlo && sidx <= nhi;
         const int64_t pidx = sidx - nlo;
         // normal of step i: from the Sobol point, or from the buffer of
