@@ -160,7 +160,9 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, double)
 __device__ __forceinline__ float inv_normal(uint32_t xi, const float* tab)
 {
     const bool sup = xi >= 0x80000000u;               // u >= 1/2 for the reference scaling
-    const uint32_t xs = sup ? (0u - xi) : xi;         // 2^32 - xi, exact; xi != 0 for Sobol points
+    // 2^32 - xi, exact.  xi = 0 would give row -1016: the kernel never passes
+    // it (only Sobol index 0 has a zero word, and its lane reads 2^31 instead)
+    const uint32_t xs = sup ? (0u - xi) : xi;
     const uint32_t bits = __float_as_uint(float(xs));
     const int row = int(bits >> 20) - (127 << 3);     // [0, kInvNormalRows - 1]
     const float t = fmaf(float(bits & 0xfffffu), 1.0f / 524288.0f, -1.0f);
@@ -494,12 +496,18 @@ path_kernel(const LaunchParams P)
             };
             fill_hi(hi, q);
             if (tail_wave) fill_hi(hi_tail, q1);
+            // Sobol index 0 (lane 0 of block 0, never a path) has every word 0,
+            // outside the inverse normal's domain; without a merge the tail row
+            // is free, so that lane reads words of 2^31 from it instead (its
+            // low words are 0) and its draws stay inside the table
+            if (!merge && q == 0)
+                for (int d = lane; d < n; d += kWave) hi_tail[d] = 0x80000000u;
             wave_sync_lds();
         }
         // lanes 0..(nhi mod 64) of the first block take the last block's paths
         const bool tail = tail_wave && lane <= int(nhi & (kWave - 1));
         const int64_t sidx = (tail ? q1 : q) * kWave + lane;
-        const uint32_t* hl = tail ? hi_tail : hi;
+        const uint32_t* hl = (tail || sidx == 0) ? hi_tail : hi;
         const bool active = sidx >= nlo && sidx <= nhi;
         const int64_t pidx = sidx - nlo;
         // normal of step i: from the Sobol point, or from the buffer of

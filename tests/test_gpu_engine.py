@@ -43,7 +43,7 @@ def test_path_ranges_reproduce_the_full_run(engine, prec):
     n = 5000
     full = cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=n, precision=prec)).payoffs
     parts = []
-    # (1, 63), (1065, 3935) and (4900, 100): the last block of 64 Sobol indices
+    # (1, 63), (1065, 3835) and (4900, 100): the last block of 64 Sobol indices
     # runs in the first block's idle lanes
     for first, cnt in [(0, 1), (1, 63), (64, 1), (65, 1000), (1065, 3835), (4900, 100)]:
         parts.append(cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=cnt, firstPath=first,
@@ -220,16 +220,19 @@ def test_barrier_edges_with_a_frozen_spot(engine, prec, rtol):
     assert pay(150.1, 1e-8)[0] == 0.0
 
 
+@pytest.mark.parametrize("n", [512, 1000])
 @pytest.mark.parametrize("mode", [cf.AAD_CHECKPOINT, cf.AAD_RECOMPUTE])
 @pytest.mark.parametrize("prec", [cf.FP64, cf.FP32])
-def test_largest_serving_grid_fits(engine, prec, mode):
+def test_largest_serving_grid_fits(engine, prec, mode, n):
     """The largest grid and timeline of the serving mix, 60 spots x 60 months
     with 5 years of weekly monitoring (260 steps, the LEAN layout), runs in
-    both precisions and adjoint variants; FP64 matches the oracle tape."""
+    both precisions and adjoint variants; FP64 matches the oracle tape.
+    1000 paths from path 0 do not merge the last block of 64 Sobol indices,
+    so lane 0 of the first block (Sobol index 0, no path) stays idle and
+    draws through the global inverse-normal table."""
     mk = W.dupire_args(60, 60)
     pk = W.barrier_args(maturity=5.0)
     mdl, prd = cf.Dupire(**mk), cf.UOC(**pk)
-    n = 512
     res = cf.mcParallelSimulAAD(prd, mdl, n, weights=[1.0, 0.0], per_path=True,
                                 num=cf.NumericalParam(numPath=n, precision=prec, aadMode=mode))
     assert cf.last_kernel_stats()["lean_layout"] == 1

@@ -60,7 +60,7 @@ def test_fp32_evaluation_within_an_ulp_of_the_tail():
     edges = [1, 2, 3, 255, 256, 2 ** 24 - 1, 2 ** 24 + 1, 2 ** 31 - 1, 2 ** 31, 2 ** 31 + 1, 2 ** 32 - 2, 2 ** 32 - 1]
     # every row boundary from below and above
     bounds = [int((1.0 + q / 8.0) * 2.0 ** e) for e in range(31) for q in range(8)]
-    near = sorted({v for b in bounds for v in (b - 1, b, b + 1) if v >= 1})    # Sobol words are never 0
+    near = sorted({v for b in bounds for v in (b - 1, b, b + 1) if v >= 1})    # 0: see the next test
     xi = np.concatenate([rng.integers(1, 2 ** 32, size=400_000, dtype=np.uint64),
                          np.array(edges + near, dtype=np.uint64)])
     z, row = _kernel_eval(tab, xi)
@@ -72,3 +72,19 @@ def test_fp32_evaluation_within_an_ulp_of_the_tail():
     assert err.mean() < 1e-7
     assert z[xi == 2 ** 31][0] == 0.0                    # u = 1/2
     np.testing.assert_array_equal(_kernel_eval(tab, 2 ** 32 - xi[:1000])[0], -z[:1000])   # odd symmetry
+
+
+def test_zero_word_never_reaches_the_table():
+    """Only Sobol index 0 (no path: path p uses point p + 1) has a zero word,
+    and word 0 maps to row -1016, outside the table.  The kernel gives that
+    idle lane words of 2^31 instead (its low words are 0 and its high row is
+    filled with 2^31), which land on the table's middle row: z = 0."""
+    tab = _committed()
+    row0 = int(np.float32(0).view(np.uint32) >> 20) - (127 << 3)
+    assert row0 == -1016
+    z, row = _kernel_eval(tab, np.array([2 ** 31], dtype=np.uint64))
+    assert row[0] == 248 and z[0] == 0.0
+    # the reference's points of paths (Sobol index >= 1) have no zero word
+    pts = np.load(os.path.join(ROOT, "tests", "golden", "sobol_points.npz"))
+    for key in pts.files:
+        assert (pts[key] != 0).all(), key
