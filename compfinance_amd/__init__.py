@@ -27,6 +27,7 @@ __all__ = [
     "value", "AADriskOne", "AADriskAggregate", "AADriskMulti", "dupireAADRisk", "clearStore",
     "mcParallelSimulAADMulti",
     "CfError", "lib", "last_kernel_stats", "LIB_PATH", "Server", "dupireCalib", "dupireSuperbucket",
+    "sobol_points",
 ]
 
 FP64, FP32 = 0, 1
@@ -116,7 +117,8 @@ def lib():
             "cf_last_kernel_stats": (ctypes.c_int, [P(cf_kernel_stats)]),
             "cf_version": (ctypes.c_char_p, []),
             "cf_device_synchronize": (ctypes.c_int, []),
-            "cf_diag_copy": (ctypes.c_int, [ctypes.c_size_t, ctypes.c_int]),
+            "cf_sobol_points": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
             "cf_put_black_scholes": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                                     ctypes.c_double, ctypes.c_char_p]),
             "cf_put_dupire": (ctypes.c_int, [ctypes.c_double, _c_double_p, ctypes.c_int, _c_double_p, ctypes.c_int,
@@ -235,12 +237,22 @@ class Dupire(_Described):
         self.maxDt = float(maxDt)
 
     def descriptor(self):
+        # attributes may have been reassigned: coerce them to contiguous
+        # float64 again (kept on the object, so later in-place changes are
+        # seen) and check the grid's shape before the engine reads it
+        spots, times = _f64(self.spots).ravel(), _f64(self.times).ravel()
+        vols = _f64(self.vols)
+        if vols.size != spots.size * times.size:
+            raise ValueError("vols must hold len(spots) x len(times) values (spot major)")
+        vols = vols.reshape(spots.size, times.size)
+        for name, a in (("spots", spots), ("times", times), ("vols", vols)):
+            object.__setattr__(self, name, a)
         d = cf_model()
-        d.kind, d.spot = 1, self.spot
-        d.n_spots, d.n_times = len(self.spots), len(self.times)
-        self._keep = (self.spots, self.times, np.ascontiguousarray(self.vols))
+        d.kind, d.spot = 1, float(self.spot)
+        d.n_spots, d.n_times = spots.size, times.size
+        self._keep = (spots, times, vols)
         d.spots, d.times, d.vols = (_dptr(a) for a in self._keep)
-        d.max_dt = self.maxDt
+        d.max_dt = float(self.maxDt)
         return d
 
     def parameters(self):
@@ -306,10 +318,16 @@ class Europeans(_Described):
             raise ValueError("maturities and strikes must have the same length")
 
     def descriptor(self):
+        # attributes may have been reassigned: coerced and checked again
+        mats, strikes = _f64(self.maturities).ravel(), _f64(self.strikes).ravel()
+        if mats.shape != strikes.shape:
+            raise ValueError("maturities and strikes must have the same length")
+        object.__setattr__(self, "maturities", mats)
+        object.__setattr__(self, "strikes", strikes)
         d = cf_product()
-        d.kind, d.n_options = 2, len(self.maturities)
-        self._keep = (self.maturities, self.strikes)
-        d.maturities, d.strikes = _dptr(self.maturities), _dptr(self.strikes)
+        d.kind, d.n_options = 2, mats.size
+        self._keep = (mats, strikes)
+        d.maturities, d.strikes = _dptr(mats), _dptr(strikes)
         return d
 
     def payoffLabels(self):
@@ -423,6 +441,19 @@ def simulation_timeline(mdl, prd):
     lib().cf_sim_timeline(ctypes.byref(dm), ctypes.byref(dp), _dptr(t),
                           c.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n)
     return t, c.astype(bool)
+
+
+def sobol_points(first_path, n_paths, dim, precision=FP64, layout=-1):
+    """Sobol points of paths first_path .. first_path + n_paths - 1 (path p
+    uses index p + 1: skipTo and next, sobol.h:77-151) in `dim` dimensions, as
+    the path kernels draw them on the device: (words [n_paths][dim] uint32,
+    normals [n_paths][dim] float64 or float32).  layout: 0 FULL, 1 LEAN, -1
+    FULL when it fits."""
+    words = np.zeros((int(n_paths), int(dim)), dtype=np.uint32)
+    normals = np.zeros((int(n_paths), int(dim)), dtype=np.float64 if precision == FP64 else np.float32)
+    _check(lib().cf_sobol_points(int(first_path), int(n_paths), int(dim), int(precision), int(layout),
+                                 words.ctypes.data, normals.ctypes.data))
+    return words, normals
 
 
 def synchronize():

@@ -550,6 +550,8 @@ ModelPlan plan_model(const cf_model* m, const ProductPlan& p)
         for (size_t i = 0; i < d.timeline.size(); ++i)
             if (std::binary_search(p.timeline.begin(), p.timeline.end(), d.timeline[i])) d.step_sample[i] = s++;
         if (size_t(s) != ns) fail(CF_ERR_ARG, "product dates closer than half a day cannot be simulated");
+        // the kernels take the last sample at the last timeline point
+        if (ns == 0 || d.step_sample.back() != int(ns) - 1) fail(CF_ERR_ARG, "product has no sample at its last date");
         // mcMdlDupire.h:91-92, :195-217
         d.log_spots.resize(d.n_spots);
         for (int i = 0; i < d.n_spots; ++i) d.log_spots[i] = std::log(d.spots[i]);
@@ -583,6 +585,8 @@ ModelPlan plan_model(const cf_model* m, const ProductPlan& p)
         if (todayOn) d.step_sample[0] = s++;
         for (int i = 1; i <= d.n_steps; ++i) d.step_sample[i] = s++;
         if (size_t(s) != ns) fail(CF_ERR_ARG, "product timeline not simulable in Black-Scholes");
+        // the kernels take the last sample at the last timeline point
+        if (ns == 0 || d.step_sample.back() != int(ns) - 1) fail(CF_ERR_ARG, "product has no sample at its last date");
         // mcMdlBS.h:197-277
         const double mu = d.rate - d.div;
         d.drift.resize(d.n_steps);
@@ -627,38 +631,6 @@ vector<string> param_labels(const cf_model* m)
             for (int j = 0; j < m->n_times; ++j) out.push_back("lvol " + fmt2(m->spots[i]) + " " + fmt2(m->times[j]));
     }
     return out;
-}
-
-// ------------------------------------------------------------------ knot search
-
-
-template <class R>
-BucketTable build_buckets(const vector<R>& knots)
-{
-    BucketTable t;
-    const int m = int(knots.size());
-    const R x0 = knots[0];
-    const double range = double(knots[m - 1]) - double(x0);
-    double dmin = range;
-    for (int j = 0; j + 1 < m; ++j) dmin = std::min(dmin, double(knots[j + 1]) - double(knots[j]));
-    if (m >= 2 && range > 0.0 && dmin > 0.0) t.nb = int(std::min(16384.0, std::ceil(2.0 * range / dmin) + 1.0));
-    const R invh = t.nb > 1 ? R(double(t.nb) / range) : R(0);
-    vector<int> cnt(t.nb, 0);
-    for (int k = 0; k < m; ++k) {
-        R bf = (knots[k] - x0) * invh;
-        bf = bf < R(t.nb - 1) ? bf : R(t.nb - 1);
-        cnt[int(bf)]++;
-    }
-    t.table.assign(t.nb, 0);
-    int acc = 0;
-    for (int b = 0; b < t.nb; ++b) {
-        t.table[b] = uint16_t(acc);
-        acc += cnt[b];
-        t.fix = std::max(t.fix, cnt[b]);
-    }
-    t.x0 = double(x0);
-    t.invh = double(invh);
-    return t;
 }
 
 // ------------------------------------------------------------------ device
@@ -707,8 +679,9 @@ Engine& engine()
             }
             for (int k = 0; k < 32; ++k) dirs[size_t(k) * cfk::kSobolDims + d] = v[k];
         }
-        hip_check(hipMalloc(&e->d_dirs, dirs.size() * 4), "hipMalloc");
-        hip_check(hipMemcpy(e->d_dirs, dirs.data(), dirs.size() * 4, hipMemcpyHostToDevice), "hipMemcpy");
+        // into the library's module variable: creating the engine allocates no device memory
+        const int rc = cfk::upload_sobol_dirs(dirs.data(), &e->d_dirs);
+        if (rc != 0) hip_check(hipError_t(rc), "direction numbers");
         slot = std::move(e);
     }
     return *slot;
@@ -825,15 +798,19 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
     const bool dup = md.kind == CF_MODEL_DUPIRE;
 
     vector<float> knots_f;
+    cfk::KnotTable<double> kt64;
+    cfk::KnotTable<float> kt32;
     if (dup) {
         if (fp64) {
-            P->bt = build_buckets(md.log_spots);
+            kt64 = cfk::build_knot_table(md.log_spots.data(), md.n_spots);
+            P->bt = {int(kt64.entries.size()), kt64.fix, kt64.invh, kt64.c0};
         } else {
             // FP32 state is log(S / S0): knots shifted by log(S0) in FP64 first
             const double ls0 = std::log(md.spot);
             knots_f.resize(md.n_spots);
             for (int j = 0; j < md.n_spots; ++j) knots_f[j] = float(md.log_spots[j] - ls0);
-            P->bt = build_buckets(knots_f);
+            kt32 = cfk::build_knot_table(knots_f.data(), md.n_spots);
+            P->bt = {int(kt32.entries.size()), kt32.fix, double(kt32.invh), double(kt32.c0)};
         }
     }
     const bool book = kernel_product(pd) == cfk::PRD_BOOK;
@@ -849,7 +826,10 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
             .total;
     };
     P->ivbar_copies = 1;
-    for (int lean = forced == 1 ? 1 : 0; lean < 2; ++lean) {
+    // the FULL checkpoint kernels keep each state's bracket in a byte, and the
+    // FULL kernels' knot search reads at most two knots of a bucket
+    const int lean_from = forced == 1 || (dup && (md.n_spots > cfk::kMaxStoredKnots || P->bt.fix > 2)) ? 1 : 0;
+    for (int lean = lean_from; lean < 2; ++lean) {
         P->cfg.lean = lean != 0;
         P->smem = lds_bytes(1);
         if (P->smem <= size_t(E.max_lds)) break;
@@ -915,22 +895,8 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
     if (dup) {
         // bucket entries: the bucket's knot count and its next two knots; NaN
         // past the last knot, which no state (not even +inf) is >= to
-        const int top = md.n_spots - 1;
-        auto entries = [&](auto real, const auto& kn) {
-            using T = decltype(real);
-            const T none = std::numeric_limits<T>::quiet_NaN();
-            vector<cfk::BucketEntry<T>> e(size_t(P->bt.nb));
-            for (int b = 0; b < P->bt.nb; ++b) {
-                const int lo = P->bt.table[size_t(b)];
-                e[size_t(b)].lo = lo;
-                e[size_t(b)].pad = 0;
-                e[size_t(b)].k0 = lo <= top ? T(kn[size_t(lo)]) : none;
-                e[size_t(b)].k1 = lo + 1 <= top ? T(kn[size_t(lo + 1)]) : none;
-            }
-            P->o_bucket = st.add(e);
-        };
-        if (fp64) entries(double(), md.log_spots);
-        else entries(float(), knots_f);
+        if (fp64) P->o_bucket = st.add(kt64.entries);
+        else P->o_bucket = st.add(kt32.entries);
     }
     P->o_ff = st.add(md.ff);
     P->o_disc = st.add(md.disc);
@@ -1059,8 +1025,8 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     P.bucket = dt + PR.o_bucket;
     P.n_buckets = dup ? PR.bt.nb : 0;
     P.bucket_fix = PR.bt.fix;
-    P.bucket_x0 = PR.bt.x0;
     P.bucket_invh = PR.bt.invh;
+    P.bucket_c0 = PR.bt.c0;
     P.drift = dt + PR.o_drift;
     P.stdv = dt + PR.o_std;
     P.spot = md.spot;
@@ -1315,23 +1281,34 @@ const char* cf_last_error(void) { return g_err.c_str(); }
 
 const char* cf_version(void) { return "compfinance-amd 0.1 (gfx950)"; }
 
-int cf_diag_copy(size_t bytes, int reps)
+int cf_sobol_points(int64_t first_path, int64_t n_paths, int dim, int precision, int layout, uint32_t* words,
+                    void* normals)
 {
     return guarded([&] {
-        if (bytes < 16 || reps < 1) fail(CF_ERR_ARG, "bad copy size");
+        if (first_path < 0 || n_paths < 1 || first_path + n_paths > int64_t(0xffffffff))
+            fail(CF_ERR_ARG, "Sobol path range out of range");
+        if (dim < 1 || dim > cfk::kSobolDims) fail(CF_ERR_ARG, "Sobol dimension out of range");
+        if (precision != CF_FP64 && precision != CF_FP32) fail(CF_ERR_ARG, "unknown precision");
+        if (layout < -1 || layout > 1) fail(CF_ERR_ARG, "unknown layout");
         Engine& E = engine();
+        const int rb = precision == CF_FP64 ? 8 : 4;
+        bool lean = layout == 1;
+        if (layout == -1) lean = cfk::sobol_points_smem(rb, false, dim, 256) > size_t(E.max_lds);
+        if (cfk::sobol_points_smem(rb, lean, dim, 256) > size_t(E.max_lds))
+            fail(CF_ERR_CAPACITY, "Sobol tables do not fit the block's shared memory");
+        const size_t n = size_t(n_paths) * size_t(dim);
         std::lock_guard<std::mutex> lk(E.sync_mu);
-        void *a = nullptr, *b = nullptr;
-        hip_check(hipMalloc(&a, bytes), "hipMalloc");
-        hip_check(hipMalloc(&b, bytes), "hipMalloc");
-        hip_check(hipMemsetAsync(a, 1, bytes, E.stream), "hipMemset");
-        for (int r = 0; r < reps; ++r) {
-            const int rc = cfk::launch_copy(a, b, bytes, E.stream);
-            if (rc != 0) hip_check(hipError_t(rc), "copy launch");
-        }
-        hip_check(hipStreamSynchronize(E.stream), "copy");
-        hip_check(hipFree(a), "hipFree");
-        hip_check(hipFree(b), "hipFree");
+        struct Owned : DevBuf {
+            ~Owned() { if (p) (void)hipFree(p); }
+        } dw, dn;
+        if (words) dw.ensure(n * 4);
+        if (normals) dn.ensure(n * size_t(rb));
+        const int rc = cfk::launch_sobol_points(rb, lean, E.d_dirs, dim, first_path, n_paths,
+                                                static_cast<uint32_t*>(dw.p), dn.p, E.stream);
+        if (rc != 0) hip_check(hipError_t(rc), "Sobol points launch");
+        if (words) hip_check(hipMemcpyAsync(words, dw.p, n * 4, hipMemcpyDeviceToHost, E.stream), "copy");
+        if (normals) hip_check(hipMemcpyAsync(normals, dn.p, n * size_t(rb), hipMemcpyDeviceToHost, E.stream), "copy");
+        hip_check(hipStreamSynchronize(E.stream), "Sobol points");
         return CF_OK;
     });
 }

@@ -124,10 +124,10 @@ struct ModelPlan {
 // Bucket table of the kernels' upper_bound (KnotSearch in cf_path_kernel.h),
 // built with the same real type and the same two rounded operations the
 // kernel uses to map a state to its bucket.
+// Shape of the knot search table (cfk::build_knot_table) handed to the kernel.
 struct BucketTable {
-    vector<uint16_t> table;
     int nb = 1, fix = 1;
-    double x0 = 0.0, invh = 0.0;
+    double invh = 0.0, c0 = 0.0;   // values of the kernel's real type
 };
 
 // ------------------------------------------------------------------ device
@@ -254,6 +254,12 @@ vector<double> fill_data(const vector<double>& original, double maxDx, double mi
 ProductPlan plan_product(const cf_product* p);
 ModelPlan plan_model(const cf_model* m, const ProductPlan& p);
 vector<string> payoff_labels(const ProductPlan& d);
+
+// adj: cfk::AdjointKind.  The host plan of a model and product (tables
+// staged on the host, none on the device) and its cache key.
+std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj);
+string plan_key(const cf_model* m, const cf_product* p, int precision, int adj);
+int adjoint_kind(const Call& c);
 
 // adj: cfk::AdjointKind.  Host plan from the engine's cache (prepared on a
 // miss); with_tables also makes its tables resident on the device.

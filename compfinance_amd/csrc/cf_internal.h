@@ -2,8 +2,10 @@
 // the host engine and the HIP kernels.
 #pragma once
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #ifdef __HIPCC__
 #define CF_HD __host__ __device__
@@ -63,10 +65,13 @@ struct BookLayout {
 // Rows of 64 reals in a wave's checkpoint slot.  The Dupire checkpoint
 // variant also keeps, per step and lane, one byte: the knot count of the
 // state's bracket (grids of up to kMaxStoredKnots knots).
+// The Dupire checkpoint slot starts with kCkptGuardRows rows that the reverse
+// sweep's prefetch may read below step 0 (values never used).
 constexpr int kMaxStoredKnots = 255;
+constexpr int kCkptGuardRows = 4;
 CF_HD inline int ckpt_rows(int n_steps, bool dupire, bool recompute, int real_bytes)
 {
-    if (!recompute) return dupire ? n_steps + (n_steps + real_bytes - 1) / real_bytes : n_steps;
+    if (!recompute) return dupire ? kCkptGuardRows + n_steps + (n_steps + real_bytes - 1) / real_bytes : n_steps;
     if (!dupire) return 0;
     const int seg = real_bytes == 4 ? kSegF32 : kSegF64;
     return (n_steps + seg - 1) / seg;
@@ -97,7 +102,7 @@ struct LaunchParams {
                                  // row padded with its first and last value
     const void* bucket;          // [n_buckets] BucketEntry<real>: the knot search table
     int n_buckets, bucket_fix;   // table size, knots per bucket at most
-    double bucket_x0, bucket_invh;   // exactly representable in the kernel's real type
+    double bucket_invh, bucket_c0;   // bucket_pos coefficients, values of the kernel's real type
     const void* drift;           // Black-Scholes [n_steps]
     const void* stdv;            // Black-Scholes [n_steps]
     double spot;                 // S0
@@ -178,6 +183,109 @@ struct alignas(16) BucketEntry {
     R k0, k1;
     int lo, pad;
 };
+
+// ---- knot search (upper_bound on the knots, interp.h:40), one definition for
+// the kernels and for the host that builds the table.  The bucket of state x
+// is floor(clamp(fma(x, invh, c0), 0, nb - 1)): monotone in x, so the knots
+// of earlier buckets are all <= x and those of later buckets all > x.
+// Bucket b holds lo, the number of knots in earlier buckets, and its first two
+// knots (NaN past the last knot); at most `fix` knots share a bucket.
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ inline float clamp_bucket(float b, float top) { return __builtin_amdgcn_fmed3f(b, 0.0f, top); }
+#else
+inline float clamp_bucket(float b, float top)
+{
+    b = b > 0.0f ? b : 0.0f;
+    return b < top ? b : top;
+}
+#endif
+CF_HD inline double clamp_bucket(double b, double top)
+{
+    b = b > 0.0 ? b : 0.0;
+    return b < top ? b : top;
+}
+CF_HD inline float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+CF_HD inline double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Bucket of x, as a real in [0, top] (NaN: converts to bucket 0 on the device).
+template <typename R>
+CF_HD inline R bucket_pos(R x, R invh, R c0, R top)
+{
+    return clamp_bucket(fma_r(x, invh, c0), top);
+}
+
+template <typename R>
+struct KnotSearch {
+    const R* knots;     // knot c at knots[stride c]
+    const BucketEntry<R>* table;
+    int m, fix, stride;
+    R invh, c0, top;    // top = number of buckets - 1
+};
+
+// #{knots <= x}, exactly as upper_bound (interp.h:40): states below the first
+// knot map to bucket 0, whose count is 0 and whose comparisons all fail.
+// WIDE: tables with more than two knots in a bucket (fix > 2); the FULL layout
+// is only chosen for tables with fix <= 2 and compiles without the loop.
+template <bool WIDE = true, typename R>
+CF_HD inline int count_le(const KnotSearch<R>& s, R x)
+{
+    // one read: the bucket's knot count and its first two candidate knots
+    // (NaN past the last knot); the knots increase, so each one <= x adds one
+    const BucketEntry<R> e = s.table[int(bucket_pos(x, s.invh, s.c0, s.top))];
+    const int top = s.m - 1;
+    int c = e.lo + int(e.k0 <= x) + int(e.k1 <= x);
+    if (WIDE) {
+        for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
+            const R kc = s.knots[s.stride * (c < top ? c : top)];
+            c += int((c <= top) & (kc <= x));
+        }
+    }
+    return c;
+}
+
+// Host: the knot search table of increasing knots (at least one), in the
+// kernel's real type.  Buckets of width at most half the smallest knot gap
+// (at most 16384 of them), so that normally no more than two knots share one.
+template <typename R>
+struct KnotTable {
+    std::vector<BucketEntry<R>> entries;
+    int fix = 1;
+    R invh = R(0), c0 = R(0);
+};
+
+template <typename R>
+inline KnotTable<R> build_knot_table(const R* knots, int m)
+{
+    KnotTable<R> t;
+    int nb = 1;
+    const R x0 = knots[0];
+    const double range = double(knots[m - 1]) - double(x0);
+    double dmin = range;
+    for (int j = 0; j + 1 < m; ++j) dmin = dmin < double(knots[j + 1]) - double(knots[j])
+                                               ? dmin : double(knots[j + 1]) - double(knots[j]);
+    if (m >= 2 && range > 0.0 && dmin > 0.0) {
+        const double want = std::ceil(2.0 * range / dmin) + 1.0;
+        nb = int(want < 16384.0 ? want : 16384.0);
+    }
+    t.invh = nb > 1 ? R(double(nb) / range) : R(0);
+    t.c0 = R(-double(x0) * double(t.invh));
+    const R top = R(nb - 1);
+    std::vector<int> cnt(size_t(nb), 0);
+    for (int k = 0; k < m; ++k) cnt[size_t(int(bucket_pos(knots[k], t.invh, t.c0, top)))]++;
+    t.entries.resize(size_t(nb));
+    const R none = R(NAN);
+    int acc = 0;
+    for (int b = 0; b < nb; ++b) {
+        BucketEntry<R>& e = t.entries[size_t(b)];
+        e.lo = acc;
+        e.pad = 0;
+        e.k0 = acc < m ? knots[acc] : none;
+        e.k1 = acc + 1 < m ? knots[acc + 1] : none;
+        acc += cnt[size_t(b)];
+        t.fix = t.fix > cnt[size_t(b)] ? t.fix : cnt[size_t(b)];
+    }
+    return t;
+}
 
 // Rows of the FP32 inverse-normal table (four coefficients each): eighths of
 // the 31 binades of 2^32 min(u, 1 - u), and its value at 1/2.
@@ -262,7 +370,16 @@ int launch_reduce_rows(const double* in0, int cols0, double* out0, const double*
                        int rows, void* stream);
 int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* blocks_per_cu);
 
-int launch_copy(const void* in, void* out, size_t bytes, void* stream);
+// Copies the direction numbers [32][kSobolDims] into the device's module
+// variable (the current device) and returns its address there.
+int upload_sobol_dirs(const uint32_t* host, uint32_t** device);
+
+// Sobol words ([npath][dim], nullable) and normals ([npath][dim] of the real
+// type, nullable) of paths [first, first + npath), as the path kernels form
+// them, in the FULL or LEAN form.
+size_t sobol_points_smem(int real_bytes, bool lean, int dim, int block);
+int launch_sobol_points(int real_bytes, bool lean, const uint32_t* dirs, int dim, int64_t first, int64_t npath,
+                        uint32_t* words, void* normals, void* stream);
 
 // mrg32k3a normals of paths [first_path, first_path + n_path), dim steps,
 // written [step][path] in FP64 (real_bytes 8) or FP32.  jump: [32][2][9] the

@@ -1,8 +1,9 @@
-// Cross-block reductions, the diagnostic copy kernel, and the dispatch of
+// Cross-block reductions, the Sobol points kernel, and the dispatch of
 // the path kernels (instantiated per family in cf_kern_inst.hip; the kernel
 // itself is in cf_path_kernel.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -121,19 +122,99 @@ __global__ void __launch_bounds__(1024) reduce_rows_kernel(const double* in0, in
     }
 }
 
-// Plain streaming copy, 16 bytes per lane per iteration (diagnostics).
-__global__ void copy_kernel(const uint4* __restrict__ in, uint4* __restrict__ out, size_t n)
+// ------------------------------------------------------------------ Sobol directions
+
+// Direction numbers [32][kSobolDims] of every device, in a module variable
+// (no allocation): filled once per device by the engine.
+__device__ uint32_t g_sobol_dirs[32 * kSobolDims];
+
+int upload_sobol_dirs(const uint32_t* host, uint32_t** device)
 {
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
-        out[i] = in[i];
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_sobol_dirs));
+    if (e != hipSuccess) return int(e);
+    e = hipMemcpy(p, host, sizeof(g_sobol_dirs), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return int(e);
+    *device = static_cast<uint32_t*>(p);
+    return 0;
 }
 
-int launch_copy(const void* in, void* out, size_t bytes, void* stream)
+// ------------------------------------------------------------------ Sobol points
+
+// Words and normals of paths [first, first + npath) in dim dimensions, formed
+// exactly as the path kernels form them (sobol_stage_lo, sobol_block,
+// sobol_low_word, inv_normal); written [path][dim].  256-thread blocks; LDS:
+// LO, one high row per wave, the tail row, and (FP32 FULL) the inverse-normal
+// table.
+template <typename R, bool LEAN>
+__global__ void __launch_bounds__(256) sobol_points_kernel(const uint32_t* dirs, int dim, int64_t first, int64_t npath,
+                                                           int merge_tail, uint32_t* words, R* normals)
 {
-    const size_t n = bytes / 16;
-    hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const uint4*>(in), static_cast<uint4*>(out), n);
-    return int(hipGetLastError());
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int waves = blockDim.x >> 6, tid = threadIdx.x, nthr = blockDim.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const size_t o_hi = cf_align16(size_t(dim) * (LEAN ? 8 : kWave) * 4);
+    const size_t o_tail = cf_align16(o_hi + size_t(waves) * dim * 4);
+    const size_t o_invn = cf_align16(o_tail + size_t(dim) * 4);
+    uint32_t* lo_tab = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* hi = reinterpret_cast<uint32_t*>(smem + o_hi) + wave * dim;
+    uint32_t* hi_tail = reinterpret_cast<uint32_t*>(smem + o_tail);
+    float* invn = reinterpret_cast<float*>(smem + o_invn);
+    sobol_stage_lo<LEAN>(lo_tab, dirs, dim, tid, nthr);
+    if constexpr (sizeof(R) == 4 && !LEAN)
+        for (int e = tid; e < kInvNormalRows * 4; e += nthr) invn[e] = kInvNormalF32[e];
+    __syncthreads();
+    const R* tab;
+    if constexpr (LEAN && sizeof(R) == 4) tab = reinterpret_cast<const R*>(kInvNormalF32);
+    else tab = reinterpret_cast<const R*>(invn);
+    tab -= 4 * kInvNormalRowBias;
+    const SobolLaneMasks lm = sobol_lane_masks(lane);
+    const int64_t nlo = first + 1, nhi = first + npath;
+    const int64_t q0 = nlo >> 6, q1 = nhi >> 6;
+    const bool merge = merge_tail != 0;
+    const int64_t qe = merge ? q1 - 1 : q1;
+    for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= qe; q += int64_t(gridDim.x) * waves) {
+        const SobolLane sl = sobol_block(hi, hi_tail, dirs, dim, q, q0, q1, nlo, nhi, merge, lane, true);
+        if (sl.active) {
+            for (int i = 0; i < dim; ++i) {
+                const uint32_t w = sl.hl[i] ^ sobol_low_word<LEAN>(lo_tab, lm, lane, i);
+                if (words) words[size_t(sl.pidx) * dim + i] = w;
+                if (normals) normals[size_t(sl.pidx) * dim + i] = inv_normal(w, tab);
+            }
+        }
+        // the rows are rewritten for the wave's next block
+        wave_sync_lds();
+    }
+}
+
+size_t sobol_points_smem(int real_bytes, bool lean, int dim, int block)
+{
+    const size_t o_hi = cf_align16(size_t(dim) * (lean ? 8 : kWave) * 4);
+    const size_t o_tail = cf_align16(o_hi + size_t(block / kWave) * dim * 4);
+    const size_t o_invn = cf_align16(o_tail + size_t(dim) * 4);
+    return o_invn + (real_bytes == 4 && !lean ? size_t(kInvNormalRows) * 16 : 0);
+}
+
+int launch_sobol_points(int real_bytes, bool lean, const uint32_t* dirs, int dim, int64_t first, int64_t npath,
+                        uint32_t* words, void* normals, void* stream)
+{
+    const int block = 256;
+    const size_t smem = sobol_points_smem(real_bytes, lean, dim, block);
+    const bool merge = merge_tail_blocks(first, npath);
+    const int64_t blocks64 = ((first + npath) >> 6) - ((first + 1) >> 6) + 1 - (merge ? 1 : 0);
+    const int grid = int(std::min<int64_t>((blocks64 + block / kWave - 1) / (block / kWave), 4096));
+    const void* f;
+    if (real_bytes == 4) f = lean ? reinterpret_cast<const void*>(&sobol_points_kernel<float, true>)
+                                  : reinterpret_cast<const void*>(&sobol_points_kernel<float, false>);
+    else f = lean ? reinterpret_cast<const void*>(&sobol_points_kernel<double, true>)
+                  : reinterpret_cast<const void*>(&sobol_points_kernel<double, false>);
+    if (smem > 65536) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
+        if (e != hipSuccess) return int(e);
+    }
+    int mt = merge ? 1 : 0;
+    void* args[] = {const_cast<uint32_t**>(&dirs), &dim, &first, &npath, &mt, &words, &normals};
+    return int(hipLaunchKernel(f, dim3(grid), dim3(block), args, smem, static_cast<hipStream_t>(stream)));
 }
 
 // ------------------------------------------------------------------ dispatch

@@ -33,6 +33,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "cf_internal.h"
 
 namespace cfk {
@@ -157,18 +159,23 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, double)
 // gen_inv_normal_f32.py).  No logarithm, no branch: one 16-byte table read
 // and three FMAs.  Differences to the FP64 Moro values are below 5e-7 (the FP32
 // rounding of the result in the far tail).
-__device__ __forceinline__ float inv_normal(uint32_t xi, const float* tab)
+// `tabm` is the table's address less 1016 rows (kInvNormalRowBias): row r of
+// the table is at tabm + 4 (r + 1016), so the row address is formed from the
+// float's bits by one mask and one add.
+constexpr int kInvNormalRowBias = 127 << 3;
+__device__ __forceinline__ float inv_normal(uint32_t xi, const float* tabm)
 {
-    const bool sup = xi >= 0x80000000u;               // u >= 1/2 for the reference scaling
+    const uint32_t sgn = xi & 0x80000000u;            // u >= 1/2 for the reference scaling
     // 2^32 - xi, exact.  xi = 0 would give row -1016: the kernel never passes
     // it (only Sobol index 0 has a zero word, and its lane reads 2^31 instead)
-    const uint32_t xs = sup ? (0u - xi) : xi;
+    // (as signed integers xi < 0 exactly when u >= 1/2, and xi = 2^31 maps to itself)
+    const uint32_t xs = uint32_t(max(int(xi), int(0u - xi)));
     const uint32_t bits = __float_as_uint(float(xs));
-    const int row = int(bits >> 20) - (127 << 3);     // [0, kInvNormalRows - 1]
+    // row (bits >> 20) - 1016 in [0, kInvNormalRows - 1], 16 bytes a row
+    const float4 c = *reinterpret_cast<const float4*>(tabm + ((bits >> 18) & 0x3ffcu));
     const float t = fmaf(float(bits & 0xfffffu), 1.0f / 524288.0f, -1.0f);
-    const float4 c = *reinterpret_cast<const float4*>(tab + 4 * row);
     const float p = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-    return sup ? -p : p;
+    return __uint_as_float(__float_as_uint(p) ^ sgn);   // -p for u >= 1/2
 }
 
 // FP64: the reference's function; the table pointer is unused
@@ -198,40 +205,6 @@ struct Bracket {
     R wa, wb;     // adjoint weights of nodes j and j + 1
 };
 
-// upper_bound on the knots (interp.h:40) through a uniform bucket table:
-// bucket b = min(floor((x - x0) * invh), nb - 1) holds the number of knots
-// in earlier buckets, a lower bound of the answer, and the next two knots; at
-// most `fix` knots fall in one bucket and are resolved by comparisons.  The
-// host builds the table with the kernel's own arithmetic, so the answer is
-// exactly upper_bound's.
-template <typename R>
-struct KnotSearch {
-    const R* knots;     // knot c at knots[2 c] (the interleaved interval table from interval 1)
-    const BucketEntry<R>* table;
-    int m, nb, fix;
-    R x0, invh;
-};
-
-template <typename R>
-__device__ __forceinline__ int count_le(const KnotSearch<R>& s, R x)
-{
-    // states below the first knot map to bucket 0, whose count is 0 and whose
-    // comparisons all fail; NaN maps there too
-    R bf = (x - s.x0) * s.invh;
-    bf = bf > R(0) ? bf : R(0);
-    bf = bf < R(s.nb - 1) ? bf : R(s.nb - 1);
-    // one read: the bucket's knot count and its first two candidate knots
-    // (NaN past the last knot); the knots increase, so each one <= x adds one
-    const BucketEntry<R> e = s.table[int(bf)];
-    const int lo = e.lo, top = s.m - 1;
-    int c = lo + int(e.k0 <= x) + int(e.k1 <= x);
-    for (int k = 2; k < s.fix; ++k) {   // grids with more than two knots in a bucket
-        const R kc = s.knots[2 * (c < top ? c : top)];
-        c += int((c <= top) & (kc <= x));
-    }
-    return c;
-}
-
 // Interval c = count_le(ks, x), the number of knots <= x, found by the
 // caller (the reverse sweep reloads it from the forward's).  The interval
 // tables make the flat extrapolation an ordinary interval: for c = 0 and
@@ -256,10 +229,10 @@ __device__ __forceinline__ Bracket<R> local_vol_at(const R* kxi, const R* row, R
     return b;
 }
 
-template <typename R>
+template <bool LEAN, typename R>
 __device__ __forceinline__ Bracket<R> local_vol(const KnotSearch<R>& ks, const R* kxi, const R* row, R x)
 {
-    return local_vol_at(kxi, row, x, count_le(ks, x));
+    return local_vol_at(kxi, row, x, count_le<LEAN>(ks, x));
 }
 
 __device__ __forceinline__ double recip_mul(double a, double b) { return fast_div(a, b); }
@@ -274,13 +247,124 @@ __device__ __forceinline__ R euler_step(R x, R vol, R g)
     return x + vol * (R(-0.5) * vol + g);
 }
 
+// ------------------------------------------------------------------ Sobol
+
+// Sobol points of a block of 64 indices n = 64 q + r (sobol.h:55-151):
+// gray(64q + r) = gray6(r) ^ (q & 1) << 5 ^ gray(q) << 6, so the word of lane r
+// in dimension d is LO[d][r] ^ HI_q[d].  The path kernels and the points
+// kernel share these functions.
+
+// LO staged in LDS by a block: FULL keeps the 64 lanes' low words of each
+// dimension, LEAN only direction numbers 0..5 (the low word is formed in
+// registers from the lane's masks).
+template <bool LEAN>
+__device__ __forceinline__ void sobol_stage_lo(uint32_t* lo_tab, const uint32_t* dirs, int n, int tid, int nthr)
+{
+    if constexpr (LEAN) {
+        for (int e = tid; e < n * 8; e += nthr) {
+            const int d = e >> 3, k = e & 7;
+            lo_tab[e] = k < 6 ? dirs[k * kSobolDims + d] : 0u;
+        }
+    } else {
+        for (int e = tid; e < n * kWave; e += nthr) {
+            const int d = e >> 6, r = e & 63;
+            const uint32_t g = uint32_t(r ^ (r >> 1));
+            uint32_t x = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if ((g >> k) & 1u) x ^= dirs[k * kSobolDims + d];
+            lo_tab[e] = x;
+        }
+    }
+}
+
+// LEAN: masks of the bits of gray6(lane), selecting direction numbers 0..5
+struct SobolLaneMasks {
+    uint32_t m[6];
+};
+__device__ __forceinline__ SobolLaneMasks sobol_lane_masks(int lane)
+{
+    SobolLaneMasks lm;
+    const uint32_t g6 = uint32_t(lane ^ (lane >> 1));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) lm.m[k] = ((g6 >> k) & 1u) ? 0xffffffffu : 0u;
+    return lm;
+}
+
+// low word of dimension i for this lane
+template <bool LEAN>
+__device__ __forceinline__ uint32_t sobol_low_word(const uint32_t* lo_tab, const SobolLaneMasks& lm, int lane, int i)
+{
+    if constexpr (LEAN) {
+        const uint32_t* v = lo_tab + i * 8;
+        return (lm.m[0] & v[0]) ^ (lm.m[1] & v[1]) ^ (lm.m[2] & v[2]) ^ (lm.m[3] & v[3]) ^ (lm.m[4] & v[4]) ^
+               (lm.m[5] & v[5]);
+    } else {
+        return lo_tab[i * kWave + lane];
+    }
+}
+
+// HI_qb of the n dimensions into a wave's LDS row, formed by the wave's lanes
+__device__ __forceinline__ void sobol_fill_hi(uint32_t* row, const uint32_t* dirs, int n, int64_t qb, int lane)
+{
+    const uint32_t gq = uint32_t(qb ^ (qb >> 1));
+    for (int d = lane; d < n; d += kWave) {
+        uint32_t x = (qb & 1) ? dirs[5 * kSobolDims + d] : 0u;
+        uint32_t b = gq;
+        while (b) {
+            const int j = __builtin_ctz(b);
+            x ^= dirs[(j + 6) * kSobolDims + d];
+            b &= b - 1u;
+        }
+        row[d] = x;
+    }
+}
+
+// One lane's share of a wave's block of 64 Sobol indices: its high-word row,
+// Sobol index, whether it holds a path of [nlo, nhi] and which.
+struct SobolLane {
+    const uint32_t* hl;
+    int64_t sidx;
+    bool active;
+    int64_t pidx;
+};
+
+// Fills the wave's high rows for block q (and, for the first block when the
+// last block is merged into its idle low lanes, the tail row of block q1) and
+// selects this lane's.  Sobol index 0 (lane 0 of block 0, never a path) has
+// every word 0, outside the FP32 inverse normal's domain; without a merge the
+// tail row is free, so that lane reads words of 2^31 from it instead (its low
+// words are 0) and its draws stay inside the table.  fill: the rows are
+// formed (false when the words are not used).
+__device__ __forceinline__ SobolLane sobol_block(uint32_t* hi, uint32_t* hi_tail, const uint32_t* dirs, int n,
+                                                 int64_t q, int64_t q0, int64_t q1, int64_t nlo, int64_t nhi,
+                                                 bool merge, int lane, bool fill)
+{
+    const bool tail_wave = merge && q == q0;
+    if (fill) {
+        sobol_fill_hi(hi, dirs, n, q, lane);
+        if (tail_wave) sobol_fill_hi(hi_tail, dirs, n, q1, lane);
+        if (!merge && q == 0)
+            for (int d = lane; d < n; d += kWave) hi_tail[d] = 0x80000000u;
+        wave_sync_lds();
+    }
+    // lanes 0..(nhi mod 64) of the first block take the last block's paths
+    const bool tail = tail_wave && lane <= int(nhi & (kWave - 1));
+    SobolLane sl;
+    sl.sidx = (tail ? q1 : q) * kWave + lane;
+    sl.hl = (tail || sl.sidx == 0) ? hi_tail : hi;
+    sl.active = sl.sidx >= nlo && sl.sidx <= nhi;
+    sl.pidx = sl.sidx - nlo;
+    return sl;
+}
+
 // ------------------------------------------------------------------ products
 
 // Per-lane product state carried along the path.
 template <typename R>
 struct ProductState {
     R alive;        // barrier: product of smoothing factors (before a breach)
-    R pnz;          // barrier: product of the non-zero smoothing factors
+    R pnz;          // barrier: after an exactly-zero smoothing factor, the product of the non-zero ones
     int nzero;      // barrier: number of exactly-zero factors
     bool dead;      // barrier: breached (alive forced to 0, no derivative)
     R euro;         // barrier: vanilla payoff at maturity
@@ -332,22 +416,7 @@ path_kernel(const LaunchParams P)
     const int tid = threadIdx.x, nthr = blockDim.x;
 
     // ---- stage the block's tables in LDS
-    if constexpr (LEAN) {
-        for (int e = tid; e < n * 8; e += nthr) {
-            const int d = e >> 3, k = e & 7;
-            lo_tab[e] = k < 6 ? P.dirs[k * kSobolDims + d] : 0u;
-        }
-    } else {
-        for (int e = tid; e < n * kWave; e += nthr) {
-            const int d = e >> 6, r = e & 63;
-            const uint32_t g = uint32_t(r ^ (r >> 1));
-            uint32_t x = 0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k)
-                if ((g >> k) & 1u) x ^= P.dirs[k * kSobolDims + d];
-            lo_tab[e] = x;
-        }
-    }
+    sobol_stage_lo<LEAN>(lo_tab, P.dirs, n, tid, nthr);
     for (int e = tid; e <= n; e += nthr) step_sample[e] = P.step_sample[e];
     if constexpr (sizeof(R) == 4 && !LEAN)
         for (int e = tid; e < kInvNormalRows * 4; e += nthr) invn[e] = kInvNormalF32[e];
@@ -380,27 +449,14 @@ path_kernel(const LaunchParams P)
         for (int e = tid; e < P.ivbar_copies * ivbar_copy_stride(n, m); e += nthr) ivbar[e] = 0.0;
     __syncthreads();
 
-    const int wave = tid >> 6, lane = tid & 63;
+    // the wave index is uniform: block-level offsets derived from it stay in
+    // scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     // local-vol table: LDS (FULL) or global memory (LEAN)
     const R* iv;
     if constexpr (LEAN) iv = static_cast<const R*>(P.iv); else iv = iv_lds;
-    // LEAN: masks of the bits of gray6(lane), selecting direction numbers 0..5
-    uint32_t lmask[6];
-    {
-        const uint32_t g6 = uint32_t(lane ^ (lane >> 1));
-#pragma unroll
-        for (int k = 0; k < 6; ++k) lmask[k] = ((g6 >> k) & 1u) ? 0xffffffffu : 0u;
-    }
-    // low Sobol word of dimension i for this lane
-    auto low_word = [&](int i) -> uint32_t {
-        if constexpr (LEAN) {
-            const uint32_t* v = lo_tab + i * 8;
-            return (lmask[0] & v[0]) ^ (lmask[1] & v[1]) ^ (lmask[2] & v[2]) ^ (lmask[3] & v[3]) ^
-                   (lmask[4] & v[4]) ^ (lmask[5] & v[5]);
-        } else {
-            return lo_tab[i * kWave + lane];
-        }
-    };
+    const SobolLaneMasks lmask = sobol_lane_masks(lane);
+    auto low_word = [&](int i) -> uint32_t { return sobol_low_word<LEAN>(lo_tab, lmask, lane, i); };
     uint32_t* hi = hi_all + wave * n;
     double* accw = acc + wave * stride;
     const int64_t nlo = P.first_path + 1, nhi = P.first_path + P.n_path;   // Sobol indices
@@ -416,19 +472,22 @@ path_kernel(const LaunchParams P)
     // checkpoint slot of the wave: [n][64] states (Dupire) or normals
     // (Black-Scholes); recompute variant: [ceil(n / kSeg)][64] Dupire states
     const int ck_rows = ckpt_rows(n, DUP, RECOMP, int(sizeof(R)));
-    R* ck = AD ? static_cast<R*>(P.ckpt) + size_t(slot) * ck_rows * kWave : nullptr;
     // bracket bytes after the states: the knot count of x_i at each step
     constexpr bool CKB = DUP && ADJ == ADJ_CHECKPOINT;
-    const bool ck_lo = CKB && m <= kMaxStoredKnots;
+    // (the Dupire checkpoint slot starts with kCkptGuardRows guard rows)
+    R* ck = AD ? static_cast<R*>(P.ckpt) + (size_t(slot) * ck_rows + (CKB ? kCkptGuardRows : 0)) * kWave : nullptr;
+    // (the FULL layout is only chosen for grids of at most kMaxStoredKnots knots)
+    const bool ck_lo = CKB && (!LEAN || m <= kMaxStoredKnots);
     uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave) : nullptr;
     KnotSearch<R> ks;
     ks.knots = kxi + 2;     // the knots themselves: interval starts from interval 1
+    ks.stride = 2;
     if constexpr (LEAN) ks.table = static_cast<const BucketEntry<R>*>(P.bucket); else ks.table = bucket;
     ks.m = m;
-    ks.nb = P.n_buckets;
     ks.fix = P.bucket_fix;
-    ks.x0 = R(P.bucket_x0);
     ks.invh = R(P.bucket_invh);
+    ks.c0 = R(P.bucket_c0);
+    ks.top = R(P.n_buckets - 1);
     const double s0 = P.spot;
     const int last = ns - 1;
 
@@ -477,46 +536,21 @@ path_kernel(const LaunchParams P)
     constexpr float kTwoZeros = -2.0f, kBreached = -4.0f;
 
     for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= qe; q += int64_t(gridDim.x) * waves) {
-        const bool tail_wave = merge && q == q0;
         // ---- high Sobol words of this block of 64 indices (and of the
-        // merged last block)
-        if constexpr (!NBUF) {
-            auto fill_hi = [&](uint32_t* row, int64_t qb) {
-                const uint32_t gq = uint32_t(qb ^ (qb >> 1));
-                for (int d = lane; d < n; d += kWave) {
-                    uint32_t x = (qb & 1) ? P.dirs[5 * kSobolDims + d] : 0u;
-                    uint32_t b = gq;
-                    while (b) {
-                        const int j = __builtin_ctz(b);
-                        x ^= P.dirs[(j + 6) * kSobolDims + d];
-                        b &= b - 1u;
-                    }
-                    row[d] = x;
-                }
-            };
-            fill_hi(hi, q);
-            if (tail_wave) fill_hi(hi_tail, q1);
-            // Sobol index 0 (lane 0 of block 0, never a path) has every word 0,
-            // outside the inverse normal's domain; without a merge the tail row
-            // is free, so that lane reads words of 2^31 from it instead (its
-            // low words are 0) and its draws stay inside the table
-            if (!merge && q == 0)
-                for (int d = lane; d < n; d += kWave) hi_tail[d] = 0x80000000u;
-            wave_sync_lds();
-        }
-        // lanes 0..(nhi mod 64) of the first block take the last block's paths
-        const bool tail = tail_wave && lane <= int(nhi & (kWave - 1));
-        const int64_t sidx = (tail ? q1 : q) * kWave + lane;
-        const uint32_t* hl = (tail || sidx == 0) ? hi_tail : hi;
-        const bool active = sidx >= nlo && sidx <= nhi;
-        const int64_t pidx = sidx - nlo;
+        // merged last block); this lane's row, index and path
+        const SobolLane sl = sobol_block(hi, hi_tail, P.dirs, n, q, q0, q1, nlo, nhi, merge, lane, !NBUF);
+        const uint32_t* hl = sl.hl;
+        const bool active = sl.active;
+        const int64_t pidx = sl.pidx;
         // normal of step i: from the Sobol point, or from the buffer of
         // pre-generated normals ([step][path] of the call's paths)
         const size_t ncol = size_t(active ? pidx : 0);
-        // the inverse-normal table: LDS, or (LEAN) the constant array itself
+        // the inverse-normal table (LDS, or for LEAN the constant array
+        // itself) less kInvNormalRowBias rows
         const R* invn_tab;
         if constexpr (LEAN && sizeof(R) == 4) invn_tab = reinterpret_cast<const R*>(kInvNormalF32);
         else invn_tab = invn;
+        invn_tab -= 4 * kInvNormalRowBias;
         auto draw = [&](int i) -> R {
             if constexpr (NBUF) return nrm[size_t(i) * size_t(P.n_path) + ncol];
             else return inv_normal(hl[i] ^ low_word(i), invn_tab);
@@ -524,6 +558,7 @@ path_kernel(const LaunchParams P)
 
         ProductState<R> ps;
         ps.alive = R(1); ps.pnz = R(1); ps.nzero = 0; ps.dead = false;
+        bool nzw = false;   // barrier: some lane of the wave has an exactly-zero smoothing factor
         ps.euro = R(0); ps.fin_f = R(0); ps.fin_num = R(1); ps.aggp = R(0);
         // book: per barrier the state above and (after its maturity) weight x
         // vanilla, the adjoint of its survival factor; per fixing schedule
@@ -554,23 +589,29 @@ path_kernel(const LaunchParams P)
                 discv = R(s_disc[s]);
             }
             if constexpr (PRD == PRD_BARRIER) {
-                // mcPrd.h:256-287
-                if (!ps.dead) {
-                    if (F > R(P.bar_smooth)) {
-                        ps.dead = true;
-                    } else if (F > R(P.minus_smooth)) {
-                        const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
-                        ps.alive = ps.alive * f;
-                        if (f == R(0)) ps.nzero++; else ps.pnz = ps.pnz * f;
+                // mcPrd.h:256-287, branch free: a breach kills the path, the
+                // smoothing zone multiplies in (B + w - F) / 2w
+                const bool live = !ps.dead;
+                const bool brk = live && F > R(P.bar_smooth);
+                const bool zone = live && !brk && F > R(P.minus_smooth);
+                const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
+                const R a_prev = ps.alive;
+                ps.dead = ps.dead || brk;
+                if (zone) ps.alive = a_prev * f;
+                // exactly-zero factors (F = B + w), rare: from the first one on,
+                // pnz holds the product of the non-zero factors
+                const bool fz = zone && f == R(0);
+                nzw = nzw | (__builtin_amdgcn_ballot_w64(fz) != 0);
+                if (nzw) {
+                    if (fz) {
+                        if (ps.nzero == 0) ps.pnz = a_prev;
+                        ps.nzero++;
+                    } else if (zone && ps.nzero > 0) {
+                        ps.pnz = ps.pnz * f;
                     }
                 }
-                if (s == last) {
-                    const R k = R(P.strike);
-                    const R intrinsic = P.put ? (k - F > R(0) ? k - F : R(0)) : (F - k > R(0) ? F - k : R(0));
-                    ps.euro = intrinsic / numv;
-                    ps.fin_f = F;
-                    ps.fin_num = numv;
-                }
+                // the vanilla of the last sample: after the path (the last
+                // sample is the last timeline point)
             } else if constexpr (PRD == PRD_EUROPEAN) {
                 // mcPrd.h:113-125
                 const R k = R(P.strike);
@@ -686,27 +727,55 @@ path_kernel(const LaunchParams P)
             const int s = step_sample[0];
             if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
         }
-        for (int i = 0; i < n; ++i) {
-            const R g = draw(i);
+        // step i0 + k, with every per-step address a fixed offset (k) from a
+        // base formed once per group of steps at i0
+        auto fwd_step = [&](int i0, int k) {
+            const int i = i0 + k;
+            R g;
+            if constexpr (NBUF || LEAN) g = draw(i);
+            else g = inv_normal(hl[i0 + k] ^ (lo_tab + i0 * kWave + lane)[k * kWave], invn_tab);
             if constexpr (DUP) {
-                if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = x;
+                R* ckg = ck + i0 * kWave + lane;
+                if constexpr (ADJ == ADJ_CHECKPOINT) ckg[k * kWave] = x;
                 if constexpr (RECOMP)
                     if ((i & (kSeg - 1)) == 0) ck[(i / kSeg) * kWave + lane] = x;
-                const int lo = count_le(ks, x);
+                const int lo = count_le<LEAN>(ks, x);
                 if constexpr (CKB)
-                    if (ck_lo) ckl[i * kWave + lane] = uint8_t(lo);
-                const Bracket<R> b = local_vol_at(kxi, iv + i * mp, x, lo);
+                    if (ck_lo) (ckl + i0 * kWave + lane)[k * kWave] = uint8_t(lo);
+                const Bracket<R> b = local_vol_at(kxi, (iv + i0 * mp) + k * mp, x, lo);
                 x = euler_step(x, b.vol, g);
             } else {
                 if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
                 x = x * rexp(drift[i] + stdv[i] * g);      // mcMdlBS.h:344-345
             }
-            const int s = step_sample[i + 1];
-            if (s >= 0 && (!UNDER || s == last || x > x_lo)) observe(s, DUP ? state_spot(x, s0) : x);
+            const int s = __builtin_amdgcn_readfirstlane((step_sample + i0 + 1)[k]);
+            // a wave-uniform test: the sample's code runs with all lanes (a lane
+            // under the zone changes nothing there)
+            if (s >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(x > x_lo) != 0)) observe(s, DUP ? state_spot(x, s0) : x);
+        };
+        // Dupire adjoint kernels: groups of four steps
+        constexpr int UF = AD && DUP ? 4 : 1;
+        int i = 0;
+        if constexpr (UF > 1) {
+            for (; i + UF <= n; i += UF) {
+#pragma unroll
+                for (int k = 0; k < UF; ++k) fwd_step(i, k);
+            }
         }
+        for (; i < n; ++i) fwd_step(i, 0);
 
         // ---- payoffs of the barrier (known at the end of the path)
         if constexpr (PRD == PRD_BARRIER) {
+            if (last >= 0) {   // mcPrd.h:281-287, at the last sample (timeline point n)
+                const R S = DUP ? state_spot(x, s0) : x;
+                const R F = DUP ? S : S * R(s_ff[last]);
+                const R numv = DUP ? R(1) : (P.spot_measure ? R(s_num[last]) * S : R(s_num[last]));
+                const R k = R(P.strike);
+                const R intrinsic = P.put ? (k - F > R(0) ? k - F : R(0)) : (F - k > R(0) ? F - k : R(0));
+                ps.euro = intrinsic / numv;
+                ps.fin_f = F;
+                ps.fin_num = numv;
+            }
             const R p0 = ps.dead ? R(0) : ps.alive * ps.euro;
             const R p1 = ps.euro;
             const double v0 = active ? double(p0) : 0.0, v1 = active ? double(p1) : 0.0;
@@ -729,17 +798,21 @@ path_kernel(const LaunchParams P)
             const R alive_eff = ps.dead ? R(0) : ps.alive;
             const R euro_bar = w0 * alive_eff + w1;       // barrier: adjoint of the vanilla
             const R alive_bar = w0 * ps.euro;             // barrier: adjoint of alive
-            auto sample_adj = [&](int s, R F, R numv, R discv, R& Fb, R& nb, R& db) {
+            // at_last: s is the last sample (the last timeline point); the
+            // sweeps pass a constant false inside their step loops
+            auto sample_adj = [&](int s, R F, R numv, R discv, R& Fb, R& nb, R& db, bool at_last) {
                 Fb = R(0); nb = R(0); db = R(0);
                 if constexpr (PRD == PRD_BARRIER) {
-                    if (!ps.dead && F > R(P.minus_smooth)) {
-                        const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
-                        R others;
-                        if (ps.nzero == 0) others = recip_mul(ps.alive, f);
-                        else others = (ps.nzero == 1 && f == R(0)) ? ps.pnz : R(0);
-                        Fb -= alive_bar * others * R(P.inv_two_smooth);
-                    }
-                    if (s == last) {
+                    // d alive / dF = -(product of the other factors) / 2w: alive / f
+                    // without a zero factor; with one, pnz for the zero factor
+                    // and 0 for the others; 0 with two or more
+                    const bool zone = !ps.dead && F > R(P.minus_smooth);
+                    const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
+                    R others = recip_mul(ps.alive, f);
+                    if (__builtin_expect(nzw, 0))
+                        others = ps.nzero == 0 ? others : ((ps.nzero == 1 && f == R(0)) ? ps.pnz : R(0));
+                    if (zone) Fb -= alive_bar * others * R(P.inv_two_smooth);
+                    if (at_last) {
                         const R k = R(P.strike);
                         const R dmax = P.put ? (k > F ? R(-1) : R(0)) : (F > k ? R(1) : R(0));
                         Fb += euro_bar * dmax / numv;
@@ -833,7 +906,7 @@ path_kernel(const LaunchParams P)
                     if (s >= 0) {
                         const R S = state_spot(x, s0);
                         R Fb, nb, db;
-                        sample_adj(s, S, R(1), R(1), Fb, nb, db);
+                        sample_adj(s, S, R(1), R(1), Fb, nb, db, true);   // step_sample[n] is the last sample
                         abar = S * Fb;
                     }
                 }
@@ -853,17 +926,17 @@ path_kernel(const LaunchParams P)
                         cj[k] = 0;
                         if (i >= n) continue;
                         const R g = draw(i);
-                        const Bracket<R> b = local_vol(ks, kxi, iv + i * mp, xi);
+                        const Bracket<R> b = local_vol<LEAN>(ks, kxi, iv + i * mp, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol
                         ca[k] = R(1) + ws * b.slope;
                         cw[k] = ws * b.wa;
                         cv[k] = ws * b.wb;
                         cj[k] = b.j;
-                        const int smp = step_sample[i];
-                        if (smp >= 0 && (!UNDER || smp == last || xi > x_lo)) {
+                        const int smp = __builtin_amdgcn_readfirstlane(step_sample[i]);
+                        if (smp >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)) {
                             const R S = state_spot(xi, s0);
                             R Fb, nb, db;
-                            sample_adj(smp, S, R(1), R(1), Fb, nb, db);
+                            sample_adj(smp, S, R(1), R(1), Fb, nb, db, false);
                             cb[k] = S * Fb;
                         }
                         xi = euler_step(xi, b.vol, g);
@@ -889,29 +962,41 @@ path_kernel(const LaunchParams P)
                     if (s >= 0) {
                         const R S = state_spot(x, s0);
                         R Fb, nb, db;
-                        sample_adj(s, S, R(1), R(1), Fb, nb, db);
+                        sample_adj(s, S, R(1), R(1), Fb, nb, db, true);   // step_sample[n] is the last sample
                         abar = S * Fb;
                     }
                 }
                 R xnext = x;     // state at point i + 1
-                // checkpoint loads run kPrefetch steps ahead of their use
-                // FP64 and the book hold more registers
-                constexpr int kPrefetch = sizeof(R) == 8 ? 1 : (BOOK ? 1 : 4);
-                R buf[kPrefetch];
-                int lbuf[kPrefetch];
+                // groups of U steps, i0 down to i0 - U + 1; the checkpoint
+                // loads of the next group are issued before the current group
+                // is worked (FP64 and the book hold more registers: U = 1).
+                // Rows below 0 read the slot's guard rows (never used).
+                constexpr int U = sizeof(R) == 8 ? 1 : (BOOK ? 1 : 4);
+                static_assert(U <= kCkptGuardRows, "the prefetch reads at most kCkptGuardRows rows below 0");
+                R buf[U];
+                int lbuf[U];
+                // states and brackets of rows top, top - 1, ..., top - U + 1
+                auto load_group = [&](int top, R* xb, int* lb) {
+                    const R* src = ck + (top - (U - 1)) * kWave + lane;
+                    const uint8_t* srcl = ckl + (top - (U - 1)) * kWave + lane;
 #pragma unroll
-                for (int k = 0; k < kPrefetch; ++k) {
-                    const bool in = n - 1 - k >= 0;
-                    buf[k] = in ? ck[(n - 1 - k) * kWave + lane] : R(0);
-                    lbuf[k] = in && ck_lo ? int(ckl[(n - 1 - k) * kWave + lane]) : 0;
-                }
-                for (int i0 = n - 1; i0 >= 0; i0 -= kPrefetch) {
+                    for (int k = 0; k < U; ++k) {
+                        xb[k] = src[(U - 1 - k) * kWave];
+                        lb[k] = ck_lo ? int(srcl[(U - 1 - k) * kWave]) : 0;
+                    }
+                };
+                if (n > 0) load_group(n - 1, buf, lbuf);
+                auto group = [&](int i0, auto partial) {
+                    constexpr bool PART = decltype(partial)::value;
+                    R nbuf[U];
+                    int nlbuf[U];
+                    if constexpr (!PART) load_group(i0 - U, nbuf, nlbuf);
                     // phase 1, independent of the adjoint chain: brackets, the
                     // derivative of each step to its vol, the sample adjoints
-                    R ws[kPrefetch], sl[kPrefetch], wa[kPrefetch], wb[kPrefetch], sf[kPrefetch];
-                    int js[kPrefetch];
+                    R ws[U], sl[U], wa[U], wb[U], sf[U];
+                    int js[U];
 #pragma unroll
-                    for (int k = 0; k < kPrefetch; ++k) {
+                    for (int k = 0; k < U; ++k) {
                         const int i = i0 - k;
                         ws[k] = R(0);
                         sl[k] = R(0);
@@ -919,12 +1004,9 @@ path_kernel(const LaunchParams P)
                         wb[k] = R(0);
                         sf[k] = R(0);
                         js[k] = 0;
-                        if (i < 0) continue;
+                        if (PART && i < 0) continue;
                         const R xi = buf[k];
-                        const int lo = ck_lo ? lbuf[k] : count_le(ks, xi);
-                        const bool in = i - kPrefetch >= 0;
-                        buf[k] = in ? ck[(i - kPrefetch) * kWave + lane] : R(0);
-                        lbuf[k] = in && ck_lo ? int(ckl[(i - kPrefetch) * kWave + lane]) : 0;
+                        const int lo = ck_lo ? lbuf[k] : count_le<LEAN>(ks, xi);
                         const Bracket<R> b = local_vol_at(kxi, iv + i * mp, xi, lo);
                         // d x_{i+1} / d vol = g - vol, recovered from the two states:
                         // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
@@ -937,35 +1019,43 @@ path_kernel(const LaunchParams P)
                         wa[k] = b.wa;
                         wb[k] = b.wb;
                         js[k] = b.j;
-                        const int smp = step_sample[i];
-                        if (smp >= 0 && (!UNDER || smp == last || xi > x_lo)) {
+                        const int smp = __builtin_amdgcn_readfirstlane(step_sample[i]);
+                        if (smp >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)) {
                             const R S = state_spot(xi, s0);
                             R Fb, nb, db;
-                            sample_adj(smp, S, R(1), R(1), Fb, nb, db);
+                            sample_adj(smp, S, R(1), R(1), Fb, nb, db, false);
                             sf[k] = S * Fb;
                         }
                         xnext = xi;
                     }
                     // phase 2: the adjoint of the state, step by step backwards
-                    R vb[kPrefetch];
+                    R vb[U];
 #pragma unroll
-                    for (int k = 0; k < kPrefetch; ++k) {
+                    for (int k = 0; k < U; ++k) {
                         vb[k] = abar * ws[k];
                         abar = abar + vb[k] * sl[k];
                         abar = abar + sf[k];
                     }
                     // phase 3: scatter onto the two bracketing local-vol nodes
+                    // (rows i0 - U + 1 .. i0 at fixed offsets from one base)
                     if (active) {
+                        double* rows = ivbl + (i0 - (U - 1)) * mp;
 #pragma unroll
-                        for (int k = 0; k < kPrefetch; ++k) {
-                            const int i = i0 - k;
-                            if (i < 0) break;
-                            double* row = ivbl + i * mp;
+                        for (int k = 0; k < U; ++k) {
+                            if (PART && i0 - k < 0) break;
+                            double* row = rows + (U - 1 - k) * mp;
                             atomicAdd(row + js[k], double(vb[k] * wa[k]));
                             atomicAdd(row + js[k] + 1, double(vb[k] * wb[k]));
                         }
                     }
-                }
+                    if constexpr (!PART) {
+#pragma unroll
+                        for (int k = 0; k < U; ++k) { buf[k] = nbuf[k]; lbuf[k] = nlbuf[k]; }
+                    }
+                };
+                int i0 = n - 1;
+                for (; i0 >= U - 1; i0 -= U) group(i0, std::false_type{});
+                if (i0 >= 0) group(i0, std::true_type{});
                 if (active) padj[0] += double(abar) / s0;      // d log(S0) / d S0
             } else {
                 // ---- Black-Scholes: forward tangents of the spot with respect
@@ -984,7 +1074,7 @@ path_kernel(const LaunchParams P)
                     const R numv = P.spot_measure ? R(s_num[s]) * S : R(s_num[s]);
                     const R discv = R(s_disc[s]);
                     R Fb, nb, db;
-                    sample_adj(s, F, numv, discv, Fb, nb, db);
+                    sample_adj(s, F, numv, discv, Fb, nb, db, s == last);
                     // dF = F (tau + dff/ff), ff = exp(mu (Tf - t))
                     const R tfs = R(tf[s]), tds = R(td[s]), ts = R(tl[s]);
                     R dF[4] = {F * tau[0], F * tau[1], F * (tau[2] + tfs), F * (tau[3] - tfs)};

@@ -361,10 +361,17 @@ int cf_last_kernel_stats(cf_kernel_stats* out);
  * calls already return only when their results are on the host). */
 int cf_device_synchronize(void);
 
-/* Diagnostics: copies `bytes` bytes between two device buffers with a plain
- * streaming kernel, `reps` times.  Used to calibrate hardware byte counters
- * (a known traffic of 2 x bytes per launch).  No counterpart in the reference. */
-int cf_diag_copy(size_t bytes, int reps);
+/* Sobol points of paths [first_path, first_path + n_paths) in dim (1..1101)
+ * dimensions, drawn on the device exactly as the path kernels draw them:
+ * path p uses Sobol index p + 1, the state after skipTo(p) and next()
+ * (sobol.h:77-151).  words: [n_paths][dim] uint32 state words (nullable);
+ * normals: [n_paths][dim] Gaussians in the precision's real type, the
+ * reference's nextG (sobol.h:110-116, invNormalCdf gaussians.h:47-87) in
+ * FP64, the kernels' table form in FP32 (nullable).  layout: 0 the kernels'
+ * FULL form (low words staged per lane), 1 LEAN (formed from six direction
+ * numbers), -1 FULL when its tables fit the block's shared memory. */
+int cf_sobol_points(int64_t first_path, int64_t n_paths, int dim, int precision, int layout, uint32_t* words,
+                    void* normals);
 
 /* Library version string. */
 const char* cf_version(void);

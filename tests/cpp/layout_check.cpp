@@ -1,14 +1,64 @@
 // Host-side checks of the launch arithmetic shared with the kernels
 // (compfinance_amd/csrc/cf_internal.h): the merge of the last block of 64
 // Sobol indices into the first block's idle lanes, and the shared-memory
-// layout.  Prints one JSON object; tests/test_layout.py reads it.
+// layout: every region the path kernel addresses, at the largest index its
+// code forms, lies inside its own region and inside the dynamic LDS size the
+// host passes to the launch (smem_layout(...).total), for each layout.
+// Prints one JSON object; tests/test_layout.py reads it.
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 
 #include "../../compfinance_amd/csrc/cf_internal.h"
 
+using namespace cfk;
+
+// One region: [off, off + used) must lie in [off, next region's offset) and in
+// [0, total).  `used` is one past the largest byte the kernel's code addresses.
+static long g_out_of_region = 0, g_regions = 0;
+static void region(size_t off, size_t used, size_t next, size_t total)
+{
+    ++g_regions;
+    if (used == 0) return;
+    if (off + used > next || off + used > total) ++g_out_of_region;
+}
+
+// The kernel's largest index of each region (cf_path_kernel.h), for a block of
+// `waves` waves on n steps, m knots, ns samples, npay payoffs.
+static void kernel_extents(int n, int m, int ns, int npay, int waves, int rb, bool dup, bool eus, bool ad, bool lean,
+                           int nb, int book_bytes, int copies)
+{
+    const SmemLayout L = smem_layout(n, m, ns, npay, waves, rb, dup, eus, ad, lean, nb, book_bytes, copies);
+    const size_t T = L.total;
+    const int mp = lv_cols(m);
+    // stage loops: lo_tab[e], e < n * (lean ? 8 : 64); hi_all[wave * n + d]; hi_tail[d], d < n
+    region(L.lo, size_t(n) * (lean ? 8 : kWave) * 4, L.hi, T);
+    region(L.hi, size_t(waves - 1) * n * 4 + size_t(n) * 4, L.hi_tail, T);
+    region(L.hi_tail, size_t(n) * 4, L.step_sample, T);
+    region(L.step_sample, size_t(n + 1) * 4, L.invn, T);          // step_sample[e], e <= n
+    // FP32 FULL: the table itself, rows 0 .. kInvNormalRows - 1 of 16 bytes
+    region(L.invn, rb == 4 && !lean ? size_t(kInvNormalRows) * 16 : 0, L.knots, T);
+    // interval pairs kxi[2c], kxi[2c + 1], c <= m; knot search reads kxi[2 + 2 (m - 1)]
+    region(L.knots, dup ? size_t(2 * m + 2) * rb : 0, L.bucket, T);
+    region(L.bucket, dup && !lean ? size_t(nb) * (rb == 4 ? sizeof(BucketEntry<float>) : sizeof(BucketEntry<double>)) : 0,
+           L.iv, T);
+    // iv[i * mp + c + 1], i < n, c <= m
+    region(L.iv, dup && !lean ? (size_t(n - 1) * mp + m + 2) * rb : 0, L.drift, T);
+    region(L.drift, dup ? 0 : size_t(n) * rb, L.stdv, T);
+    region(L.stdv, dup ? 0 : size_t(n) * rb, L.samples, T);
+    region(L.samples, dup ? 0 : size_t(3 * ns) * 8, L.pay_first, T);
+    region(L.pay_first, eus ? size_t(ns + 1) * 4 : 0, L.pay_strike, T);
+    region(L.pay_strike, eus ? size_t(npay) * 8 : 0, L.book, T);
+    region(L.book, size_t(book_bytes), L.acc, T);
+    region(L.acc, size_t(waves) * out_stride(npay) * 8, L.ivbar, T);
+    // ivbar: copy c at c * ivbar_copy_stride, node (i, j + 1) with j <= m
+    // (padded column m + 1), and the end columns 0 and m + 1 on write-out
+    const size_t last = size_t(copies - 1) * ivbar_copy_stride(n, m) + size_t(n - 1) * mp + m + 1;
+    region(L.ivbar, ad && dup ? (last + 1) * 8 : 0, T, T);
+}
+
 int main()
 {
-    using namespace cfk;
     // merge_tail_blocks against brute force: paths [f, f + n) use Sobol indices
     // f + 1 .. f + n; the last aligned block of 64 can ride in the first one
     // when it differs from it and each of its lanes is idle in the first
@@ -44,12 +94,49 @@ int main()
             }
         }
     }
+    // region extents over layouts, precisions, models, products and adjoints
+    for (int rb = 4; rb <= 8; rb += 4)
+        for (int lean = 0; lean < 2; ++lean)
+            for (int ad = 0; ad < 2; ++ad)
+                for (int dup = 0; dup < 2; ++dup)
+                    for (int eus = 0; eus < 2; ++eus)
+                        for (int copies = 1; copies <= 2; ++copies)
+                            for (int n : {1, 2, 156, 261})
+                                for (int m : {1, 2, 30, 60, 255})
+                                    for (int waves : {1, 8, 16})
+                                        kernel_extents(n, m, n + 1, eus ? 31 : 2, waves, rb, dup != 0, eus != 0,
+                                                       ad != 0, lean != 0, 2 * m + 3, eus ? 0 : 1104, copies);
+    // FP32 inverse normal: the row the kernel reads, (bits(float(xs)) >> 18 &
+    // 0x3ffc) / 4 - kInvNormalRowBias, is inside the table for every xs in
+    // [1, 2^31] (each binade's ends and every row boundary, from both sides)
+    long bad_rows = 0, row_probes = 0;
+    {
+        auto row_of = [](uint32_t xs) {
+            const float f = float(xs);
+            uint32_t bits;
+            std::memcpy(&bits, &f, 4);
+            return int((bits >> 18) & 0x3ffcu) / 4 - (127 << 3);
+        };
+        for (int e = 0; e <= 31; ++e)
+            for (int q = 0; q < 8; ++q) {
+                const double b = (1.0 + q / 8.0) * double(1ull << e);
+                for (double d : {-1.0, 0.0, 1.0}) {
+                    const double v = b + d;
+                    if (v < 1.0 || v > 2147483648.0) continue;
+                    const int r = row_of(uint32_t(v));
+                    ++row_probes;
+                    if (r < 0 || r >= kInvNormalRows) ++bad_rows;
+                }
+            }
+    }
     fp32_c3 = smem_layout(156, 30, 157, 2, 16, 4, true, false, true, false, 64, 0, 2).total;
     fp64_c3 = smem_layout(156, 30, 157, 2, 16, 8, true, false, true, false, 64, 0, 1).total;
     lean_60x60 = smem_layout(261, 60, 262, 2, 16, 8, true, false, true, true, 64, 0, 1).total;
     copy_offset_mod128 = int((size_t(ivbar_copy_stride(156, 30)) * 8) % 128);
     std::printf("{\"cases\": %ld, \"wrong\": %ld, \"merged\": %ld, \"misaligned\": %ld, \"fp32_c3\": %zu, "
-                "\"fp64_c3\": %zu, \"lean_60x60_5y_fp64\": %zu, \"copy_offset_mod128\": %d}\n",
-                cases, wrong, merged, misaligned, fp32_c3, fp64_c3, lean_60x60, copy_offset_mod128);
+                "\"fp64_c3\": %zu, \"lean_60x60_5y_fp64\": %zu, \"copy_offset_mod128\": %d, \"regions\": %ld, "
+                "\"out_of_region\": %ld, \"row_probes\": %ld, \"bad_rows\": %ld}\n",
+                cases, wrong, merged, misaligned, fp32_c3, fp64_c3, lean_60x60, copy_offset_mod128, g_regions,
+                g_out_of_region, row_probes, bad_rows);
     return 0;
 }

@@ -78,6 +78,19 @@ def test_fp32_prices_within_tenth_of_standard_error(engine, mname, pname):
     assert np.all(np.abs(r32.means - r64.means)[ok] <= 0.1 * r64.std_errs[ok]), (r32.means, r64.means)
 
 
+@pytest.mark.parametrize("mname", ["lv30", "lv60"])
+def test_fp32_c2_price_at_full_size(engine, mname):
+    """The C2 pricing run at its configured size, 2^19 Sobol paths x 156
+    weekly steps: the FP32 barrier prices within 0.1 standard errors of FP64
+    on the same paths."""
+    mdl, prd, _, _ = make(MODELS[mname], PRODUCTS["barrier"])
+    n = 1 << 19
+    r64 = cf.mcParallelSimul(prd, mdl, n, per_path=False, precision=cf.FP64)
+    r32 = cf.mcParallelSimul(prd, mdl, n, per_path=False, precision=cf.FP32)
+    assert np.all(r64.std_errs > 0)
+    assert np.all(np.abs(r32.means - r64.means) <= 0.1 * r64.std_errs), (r32.means, r64.means, r64.std_errs)
+
+
 def test_one_hot_aggregate_selects_the_payoff(engine):
     mdl, prd, om, op = make(MODELS["lv30"], PRODUCTS["barrier"])
     n = 2048

@@ -38,6 +38,26 @@ def test_results_in_the_mix_equal_results_alone(engine):
             np.testing.assert_allclose(got["risks"], risks, rtol=1e-12, atol=1e-12 * np.abs(risks).max())
 
 
+def test_full_c5_stream_in_the_mix_equals_alone(engine):
+    """The whole 512-request serving stream (seed 20261204), submitted at
+    once: every request's prices equal its synchronous run bit for bit, its
+    aggregate and risks to 1e-12."""
+    reqs = serving.c5_requests()
+    assert len(reqs) == 512
+    with cf.Server(slots=4, hbmBudget=8 << 30) as srv:
+        ids = [srv.submit(r["product"], r["model"], r["num"], aad=r["aad"]) for r in reqs]
+        res = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["completed"] == 512 and st["failed"] == 0
+    assert st["peak_running"] >= 2
+    for r, got in zip(reqs, res):
+        means, risks, agg = _alone(r)
+        np.testing.assert_array_equal(got["means"], means)
+        if r["aad"]:
+            assert abs(got["agg_mean"] - agg) <= 1e-12 * max(abs(agg), 1e-300)
+            np.testing.assert_allclose(got["risks"], risks, rtol=1e-12, atol=1e-12 * np.abs(risks).max())
+
+
 def test_admission_waits_for_memory_and_switches_to_recompute(engine):
     """A budget of one checkpoint block plus one recompute block: the second
     AAD request runs the recompute variant, the next ones wait for memory;

@@ -101,3 +101,31 @@ def test_book_limits():
         cf.Book([cf.European(100.0, 1.0, 1.5)]).payoffLabels()
     with pytest.raises(cf.CfError):
         cf.Book([cf.Book([u])]).payoffLabels()
+
+
+@pytest.mark.parametrize("mk", [LV, BS])
+def test_asian_put_call_parity_per_path(mk):
+    """Per path, call - put = (average - K) / numeraire; the zero-strike call
+    is average / numeraire, so call - put - (zero-strike call) = -K / numeraire,
+    the same on every path (the numeraire is deterministic in both models
+    here).  The risks of that combination (weights 1, -1, -1) vanish: the put's
+    adjoint is -1{K > average} (AADExpr.h:571-583)."""
+    m = po.Model(**mk)
+    k = 100.0
+    parts = [dict(strike=k, maturity=3.0, fixingFreq=1 / 12, snapFreq=1 / 52),
+             dict(strike=k, maturity=3.0, fixingFreq=1 / 12, snapFreq=1 / 52, callPut=True),
+             dict(strike=0.0, maturity=3.0, fixingFreq=1 / 12, snapFreq=1 / 52)]
+    pay = po.value_paths(m, po.Product(parts=parts), 2000)
+    c, p, a0 = pay[:, 0], pay[:, 1], pay[:, 2]
+    assert (c > 0).sum() > 100 and (p > 0).sum() > 100           # both branches exercised
+    const = c - p - a0
+    disc = np.exp(-BS["rate"] * 3.0) if mk is BS else 1.0
+    np.testing.assert_allclose(const, -k * disc, rtol=1e-12, atol=1e-12)
+    _, _, risks = po.aad(m, po.Product(parts=parts), 2000, weights=[1.0, -1.0, -1.0])
+    _, _, scale = po.aad(m, po.Product(parts=parts), 2000, weights=[0.0, 1.0, 0.0])
+    if mk is LV:
+        assert np.abs(risks).max() < 1e-12 * np.abs(scale).max()
+    else:
+        # Black-Scholes parameters (spot, vol, rate, div): only the rate moves -K exp(-rT)
+        np.testing.assert_allclose(risks[[0, 1, 3]], 0.0, atol=1e-12 * np.abs(scale).max())
+        np.testing.assert_allclose(risks[2], 3.0 * k * disc, rtol=1e-10)
