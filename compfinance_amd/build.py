@@ -19,7 +19,7 @@ SOURCES = ([("cf_kern_inst.hip", "cf_kern_r%d_a%d_b%d" % (r, a, b),
              ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_NBUF=%d" % b])
             for r in (0, 1) for a in (0, 1, 2) for b in (0, 1)] +
            [(s, os.path.splitext(s)[0], []) for s in ("cf_kernels.hip", "cf_engine.cpp", "cf_store.cpp",
-                                                      "cf_server.cpp", "cf_calib.cpp")])
+                                                      "cf_server.cpp", "cf_calib.cpp", "cf_noalloc.hip")])
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = [

@@ -152,6 +152,21 @@ int cf_mc_simul_aad(const cf_model* mdl, const cf_product* prd, const cf_num* nu
                     const double* weights, double* payoff_means, double* agg_mean,
                     double* risks, double* per_path_payoffs, double* per_path_agg);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): cf_mc_simul_aad with the same arguments and results,
+ * with no device or pinned allocation and no freeing.  Every device buffer
+ * of the call comes from a module array of CF_NOALLOC_SCRATCH_MIB MiB per
+ * device, held until the process ends; a call that needs more returns
+ * CF_ERR_CAPACITY (use fewer paths, the recompute adjoint, or
+ * cf_mc_simul_aad).  The array keeps the last call's contents; the engine's
+ * lock runs the calls of one process on one device one after the other
+ * (calls that overlapped on one device would race on the array). */
+#define CF_NOALLOC_SCRATCH_MIB 256
+int cf_mc_simul_aad_noalloc(const cf_model* mdl, const cf_product* prd, const cf_num* num,
+                            const double* weights, double* payoff_means, double* agg_mean,
+                            double* risks, double* per_path_payoffs, double* per_path_agg);
+
 /* mcParallelSimulAADMulti (mcBase.h:858-989): the risk of every payoff
  * separately, risks[n_param][n_pay] = d E[payoff_k] / d parameter_i, on the
  * same paths; one adjoint sweep per payoff.  Returns n_param. */
