@@ -414,18 +414,24 @@ path_kernel(const LaunchParams P)
     R* invn = reinterpret_cast<R*>(smem + L.invn);   // FP32 inverse-normal table
 
     const int tid = threadIdx.x, nthr = blockDim.x;
+    // the inverse-normal table (FP32) and the knot search table are read from
+    // global memory through the L1 cache, not LDS, by the LEAN kernels and by
+    // the FP32 recompute kernels, whose re-simulation otherwise loads the LDS
+    // most (measured: C3 FP32 recompute 0.719 -> 0.643 ms; the checkpoint and
+    // pricing kernels are faster with both in LDS)
+    constexpr bool GTAB = LEAN || (RECOMP && sizeof(R) == 4);
 
     // ---- stage the block's tables in LDS
     sobol_stage_lo<LEAN>(lo_tab, P.dirs, n, tid, nthr);
     for (int e = tid; e <= n; e += nthr) step_sample[e] = P.step_sample[e];
-    if constexpr (sizeof(R) == 4 && !LEAN)
+    if constexpr (sizeof(R) == 4 && !GTAB)
         for (int e = tid; e < kInvNormalRows * 4; e += nthr) invn[e] = kInvNormalF32[e];
     if constexpr (DUP) {
         const R* gk = static_cast<const R*>(P.knots);
         const R* gi = static_cast<const R*>(P.inv_dx);
         const R* gv = static_cast<const R*>(P.iv);
         for (int e = tid; e <= m; e += nthr) { kxi[2 * e] = gk[e]; kxi[2 * e + 1] = gi[e]; }
-        if constexpr (!LEAN)
+        if constexpr (!GTAB)
             for (int e = tid; e < P.n_buckets; e += nthr) bucket[e] = static_cast<const BucketEntry<R>*>(P.bucket)[e];
         if constexpr (!LEAN)
             for (int e = tid; e < n * mp; e += nthr) iv_lds[e] = gv[e];
@@ -482,7 +488,7 @@ path_kernel(const LaunchParams P)
     KnotSearch<R> ks;
     ks.knots = kxi + 2;     // the knots themselves: interval starts from interval 1
     ks.stride = 2;
-    if constexpr (LEAN) ks.table = static_cast<const BucketEntry<R>*>(P.bucket); else ks.table = bucket;
+    if constexpr (GTAB) ks.table = static_cast<const BucketEntry<R>*>(P.bucket); else ks.table = bucket;
     ks.m = m;
     ks.fix = P.bucket_fix;
     ks.invh = R(P.bucket_invh);
@@ -548,7 +554,7 @@ path_kernel(const LaunchParams P)
         // the inverse-normal table (LDS, or for LEAN the constant array
         // itself) less kInvNormalRowBias rows
         const R* invn_tab;
-        if constexpr (LEAN && sizeof(R) == 4) invn_tab = reinterpret_cast<const R*>(kInvNormalF32);
+        if constexpr (GTAB && sizeof(R) == 4) invn_tab = reinterpret_cast<const R*>(kInvNormalF32);
         else invn_tab = invn;
         invn_tab -= 4 * kInvNormalRowBias;
         auto draw = [&](int i) -> R {
@@ -748,13 +754,15 @@ path_kernel(const LaunchParams P)
                 if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
                 x = x * rexp(drift[i] + stdv[i] * g);      // mcMdlBS.h:344-345
             }
-            const int s = __builtin_amdgcn_readfirstlane((step_sample + i0 + 1)[k]);
             // a wave-uniform test: the sample's code runs with all lanes (a lane
             // under the zone changes nothing there)
-            if (s >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(x > x_lo) != 0)) observe(s, DUP ? state_spot(x, s0) : x);
+            const int s = __builtin_amdgcn_readfirstlane((step_sample + i0 + 1)[k]);
+            if (s >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(x > x_lo) != 0))
+                observe(s, DUP ? state_spot(x, s0) : x);
         };
-        // Dupire adjoint kernels: groups of four steps
-        constexpr int UF = AD && DUP ? 4 : 1;
+        // FP32 Dupire adjoint kernels other than the book: groups of four steps
+        // (the book's sample code, four times over, and FP64 measured slower)
+        constexpr int UF = AD && DUP && !BOOK && sizeof(R) == 4 ? 4 : 1;
         int i = 0;
         if constexpr (UF > 1) {
             for (; i + UF <= n; i += UF) {
@@ -893,6 +901,23 @@ path_kernel(const LaunchParams P)
                 }
             };
 
+            // Dupire barrier, a sample before the last: S x d payoff / dF for spot S,
+            // = -alive_bar (product of the other factors) S / 2w in the zone.
+            // Without a zero factor the other factors' product is alive / f with
+            // f = (B + w - S) / 2w, so the adjoint is kz S / (B + w - S),
+            // kz = -alive_bar alive; with one, pnz at the zero factor
+            const R kz = -alive_bar * (ps.dead ? R(0) : ps.alive);
+            auto barrier_sf = [&](R S) -> R {
+                const bool zone = !ps.dead && S > R(P.minus_smooth);
+                const R d = R(P.bar_smooth) - S;
+                R sf = kz * recip_mul(S, d);
+                if (__builtin_expect(nzw, 0))
+                    sf = ps.nzero == 0 ? sf
+                                       : ((ps.nzero == 1 && d * R(P.inv_two_smooth) == R(0))
+                                              ? -alive_bar * ps.pnz * R(P.inv_two_smooth) * S : R(0));
+                return zone ? sf : R(0);
+            };
+
             if constexpr (DUP && RECOMP) {
                 // ---- reverse sweep, recompute variant: segments of kSeg steps
                 // from the last one back; each segment is first re-simulated
@@ -933,7 +958,10 @@ path_kernel(const LaunchParams P)
                         cv[k] = ws * b.wb;
                         cj[k] = b.j;
                         const int smp = __builtin_amdgcn_readfirstlane(step_sample[i]);
-                        if (smp >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)) {
+                        if constexpr (UNDER) {
+                            if (smp >= 0 && __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)
+                                cb[k] = barrier_sf(state_spot(xi, s0));
+                        } else if (smp >= 0) {
                             const R S = state_spot(xi, s0);
                             R Fb, nb, db;
                             sample_adj(smp, S, R(1), R(1), Fb, nb, db, false);
@@ -1010,21 +1038,27 @@ path_kernel(const LaunchParams P)
                         const Bracket<R> b = local_vol_at(kxi, iv + i * mp, xi, lo);
                         // d x_{i+1} / d vol = g - vol, recovered from the two states:
                         // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
-                        if (b.vol != R(0)) {
-                            ws[k] = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
-                        } else {
-                            ws[k] = draw(i);
-                        }
+                        // where a local vol is 0 (rare; a wave-uniform test) the
+                        // normal is drawn again
+                        ws[k] = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
+                        if (__builtin_amdgcn_ballot_w64(b.vol == R(0)) != 0)
+                            if (b.vol == R(0)) ws[k] = draw(i);
                         sl[k] = b.slope;
                         wa[k] = b.wa;
                         wb[k] = b.wb;
                         js[k] = b.j;
-                        const int smp = __builtin_amdgcn_readfirstlane(step_sample[i]);
-                        if (smp >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)) {
-                            const R S = state_spot(xi, s0);
-                            R Fb, nb, db;
-                            sample_adj(smp, S, R(1), R(1), Fb, nb, db, false);
-                            sf[k] = S * Fb;
+                        if constexpr (UNDER) {
+                            if (__builtin_amdgcn_readfirstlane(step_sample[i]) >= 0 &&
+                                __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)
+                                sf[k] = barrier_sf(state_spot(xi, s0));
+                        } else {
+                            const int smp = __builtin_amdgcn_readfirstlane(step_sample[i]);
+                            if (smp >= 0) {
+                                const R S = state_spot(xi, s0);
+                                R Fb, nb, db;
+                                sample_adj(smp, S, R(1), R(1), Fb, nb, db, false);
+                                sf[k] = S * Fb;
+                            }
                         }
                         xnext = xi;
                     }
