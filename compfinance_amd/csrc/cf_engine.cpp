@@ -468,16 +468,17 @@ BookTables build_book(const ProductPlan& d, double width0)
     L.n_bar = int(bar_col.size());
     L.n_grp = int(sched.size());
     L.n_asian = int(as_strike.size());
-    L.van_first = put(van_first.data(), van_first.size() * 4);
     L.van_strike = put(van_strike.data(), van_strike.size() * 8);
     L.van_col = put(van_col.data(), van_col.size() * 4);
-    // the four masks of a sample in one 16-byte row
+    // a sample's row, one 16-byte read: barriers observing (bits 0-15) and
+    // maturing (16-31), schedules fixing (bits 0-3) and ending (4-7), and its
+    // vanillas [van_first[s], van_first[s + 1])
     vector<uint32_t> masks(size_t(ns) * 4);
     for (int k = 0; k < ns; ++k) {
-        masks[4 * k] = bar_mask[k];
-        masks[4 * k + 1] = bar_last[k];
-        masks[4 * k + 2] = grp_mask[k];
-        masks[4 * k + 3] = grp_last[k];
+        masks[4 * k] = bar_mask[k] | (bar_last[k] << 16);
+        masks[4 * k + 1] = grp_mask[k] | (grp_last[k] << 4);
+        masks[4 * k + 2] = uint32_t(van_first[k]);
+        masks[4 * k + 3] = uint32_t(van_first[k + 1]);
     }
     L.masks = put(masks.data(), masks.size() * 4);
     L.bar_par = put(bar_par.data(), bar_par.size() * 8);

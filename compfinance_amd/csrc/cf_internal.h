@@ -45,11 +45,11 @@ constexpr int book_block_limit(int prd, int real_bytes, int adj)
 struct BookLayout {
     int n_van, n_bar, n_grp, n_asian;
     int bytes;
-    int van_first;    // [n_samples + 1] int: vanillas of sample s
     int van_strike;   // [n_van] double
     int van_col;      // [n_van] int: payoff column
-    int masks;        // [n_samples][4] uint32, 16-byte rows: barriers observing at s,
-                      // barriers maturing at s, schedules fixing at s, schedules ending at s
+    int masks;        // [n_samples][4] uint32, 16-byte rows: barriers observing at s (bits 0-15)
+                      // and maturing at s (16-31), schedules fixing at s (bits 0-3) and ending
+                      // at s (4-7), first vanilla of s, first vanilla of s + 1
     int bar_par;      // [n_bar][4] double: strike, B + w, B - w, 1 / 2w
     int bar_par_f;    // [n_bar][4] the same in float
     int bar_col;      // [n_bar] int: column of the barrier payoff (vanilla next)

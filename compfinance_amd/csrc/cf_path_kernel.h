@@ -506,15 +506,16 @@ path_kernel(const LaunchParams P)
 
     // book tables in LDS
     const unsigned char* bkb = smem + L.book;
-    const int* bk_van_first = reinterpret_cast<const int*>(bkb + P.bk.van_first);
     const double* bk_van_strike = reinterpret_cast<const double*>(bkb + P.bk.van_strike);
     const int* bk_van_col = reinterpret_cast<const int*>(bkb + P.bk.van_col);
-    // per sample: barriers observing, barriers maturing, schedules fixing,
-    // schedules ending (one aligned 16-byte LDS read)
-    struct alignas(16) SampleMasks {
-        uint32_t bar, bar_last, grp, grp_last;
+    // per sample, one aligned 16-byte LDS read: barriers observing (bits
+    // 0-15) and maturing (16-31), schedules fixing (bits 0-3) and ending
+    // (4-7), and the sample's vanillas [van_lo, van_hi)
+    struct alignas(16) SampleRow {
+        uint32_t bars, grps;
+        int van_lo, van_hi;
     };
-    const SampleMasks* bk_masks = reinterpret_cast<const SampleMasks*>(bkb + P.bk.masks);
+    const SampleRow* bk_rows = reinterpret_cast<const SampleRow*>(bkb + P.bk.masks);
     const int* bk_grp_first = reinterpret_cast<const int*>(bkb + P.bk.grp_first);
     const double* bk_as_strike = reinterpret_cast<const double*>(bkb + P.bk.as_strike);
     const int* bk_as_col = reinterpret_cast<const int*>(bkb + P.bk.as_col);
@@ -628,16 +629,16 @@ path_kernel(const LaunchParams P)
                 if (P.per_path && active) P.per_path[pidx * npay] = v;
                 if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[0]) * pay;
             } else if constexpr (PRD == PRD_BOOK) {
+                const SampleRow sm = bk_rows[s];
                 // vanilla calls at this sample (European / Europeans parts)
-                for (int k = bk_van_first[s]; k < bk_van_first[s + 1]; ++k) {
+                for (int k = sm.van_lo; k < sm.van_hi; ++k) {
                     const R kk = R(bk_van_strike[k]);
                     emit(bk_van_col[k], (F - kk > R(0) ? F - kk : R(0)) / numv);
                 }
                 // up-and-out barriers (mcPrd.h:256-287, one state per barrier),
                 // branch free: breach, smoothing zone, exactly-zero factor;
                 // no state changes while no lane is at or above the lowest zone
-                const SampleMasks sm = bk_masks[s];
-                const uint32_t bm0 = sm.bar, bl = sm.bar_last;
+                const uint32_t bm0 = sm.bars & 0xffffu, bl = sm.bars >> 16;
                 const uint32_t bm = bm0 && __any(F > bk_lo) ? bm0 : 0u;
                 if (bm | bl) {
 #pragma unroll
@@ -673,7 +674,7 @@ path_kernel(const LaunchParams P)
                     }
                 }
                 // arithmetic averages: running sums per fixing schedule
-                const uint32_t gm = sm.grp, gl = sm.grp_last;
+                const uint32_t gm = sm.grps & 0xfu, gl = (sm.grps >> 4) & 0xfu;
                 if (gm) {
 #pragma unroll
                     for (int g = 0; g < NGRP; ++g) {
@@ -834,7 +835,8 @@ path_kernel(const LaunchParams P)
                     db = w0 * intrinsic / numv;
                     nb = -w0 * intrinsic * discv / (numv * numv);
                 } else if constexpr (PRD == PRD_BOOK) {
-                    for (int k = bk_van_first[s]; k < bk_van_first[s + 1]; ++k) {
+                    const SampleRow sm = bk_rows[s];
+                    for (int k = sm.van_lo; k < sm.van_hi; ++k) {
                         const R kk = R(bk_van_strike[k]);
                         const R wk = R(P.weights[bk_van_col[k]]);
                         if (F > kk) {
@@ -843,8 +845,7 @@ path_kernel(const LaunchParams P)
                         }
                     }
                     // no lane at or above the lowest zone: no survival adjoint
-                    const SampleMasks sm = bk_masks[s];
-                    const uint32_t bm0 = sm.bar, bl = sm.bar_last;
+                    const uint32_t bm0 = sm.bars & 0xffffu, bl = sm.bars >> 16;
                     const uint32_t bm = bm0 && __any(F > bk_lo) ? bm0 : 0u;
                     if (bm | bl) {
 #pragma unroll
@@ -880,7 +881,7 @@ path_kernel(const LaunchParams P)
                             }
                         }
                     }
-                    const uint32_t gm = sm.grp, gl = sm.grp_last;
+                    const uint32_t gm = sm.grps & 0xfu, gl = (sm.grps >> 4) & 0xfu;
                     if (gm) {
 #pragma unroll
                         for (int g = 0; g < NGRP; ++g) {
