@@ -18,6 +18,8 @@ LIB = os.path.join(HERE, "libcfamd.so")
 SOURCES = ([("cf_kern_inst.hip", "cf_kern_r%d_a%d_b%d" % (r, a, b),
              ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_NBUF=%d" % b])
             for r in (0, 1) for a in (0, 1, 2) for b in (0, 1)] +
+           [("cf_kern_inst.hip", "cf_kern_r%d_a1_b2" % r, ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=1", "-DCF_INST_NBUF=2"])
+            for r in (0, 1)] +
            [(s, os.path.splitext(s)[0], []) for s in ("cf_kernels.hip", "cf_engine.cpp", "cf_store.cpp",
                                                       "cf_server.cpp", "cf_calib.cpp", "cf_noalloc.hip")])
 

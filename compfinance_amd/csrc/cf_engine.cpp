@@ -1108,9 +1108,10 @@ void make_sequence(const Prepared& PR, const CallShape& S, const CallBuffers& B,
 
 void ready_sequence(const Sequence& q)
 {
-    // the plan's kernel was set up at planning; the normals-buffer variant
-    // of it is set up on first use (cached by the launcher)
-    if (!q.cfg.normals) return;
+    // the plan's kernel was set up at planning; the normals-buffer and
+    // itemized-risk variants of it are set up on first use (cached by the
+    // launcher)
+    if (!q.cfg.normals && !q.cfg.item) return;
     int per_cu = 0;
     const int rc = cfk::path_kernel_occupancy(q.cfg, q.block, q.smem, &per_cu);
     if (rc != 0) hip_check(hipError_t(rc), "occupancy");
@@ -1429,11 +1430,103 @@ int cf_mc_simul_aad(const cf_model* mdl, const cf_product* prd, const cf_num* nu
     });
 }
 
+namespace cfh {
+
+// Itemized risk over shared paths (the reference's single sweep with vector
+// adjoints, mcBase.h:858-989): one forward-only launch keeps every block of 64
+// paths' checkpoints and end states, then one reverse-only launch per payoff
+// (one-hot weights) reads them.  Returns false, doing nothing, when the kept
+// checkpoints do not fit the device's free memory.
+bool run_multi(const cf_model* mdl, const cf_product* prd, const cf_num* num, double* payoff_means, double* risks)
+{
+    const bool fp64 = num->precision == CF_FP64;
+    Engine& E = engine();
+    std::lock_guard<std::mutex> lk(E.sync_mu);
+    hip_check(hipSetDevice(E.device), "hipSetDevice");
+    const std::shared_ptr<Prepared> PP = lookup(E, mdl, prd, fp64, cfk::ADJ_CHECKPOINT, true);
+    const Prepared& PR = *PP;
+    const CallShape S = shape_call(E, PR, num, true);
+    const CallBytes nb = call_bytes(S);
+    const int64_t q0 = (S.first + 1) >> 6, q1 = (S.first + S.n_path) >> 6;
+    const size_t blocks64 = size_t(q1 - q0 + 1 - (S.merge ? 1 : 0));
+    const size_t rb = fp64 ? 8 : 4;
+    const size_t ck_bytes = blocks64 * size_t(S.ck_rows) * cfk::kWave * rb;
+    const size_t st_bytes = blocks64 * size_t(cfk::state_rows(PR.cfg.product == cfk::PRD_BOOK)) * cfk::kWave * rb;
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (ck_bytes + st_bytes + (size_t(256) << 20) > free_b) return false;
+
+    struct Owned : DevBuf {
+        ~Owned() { if (p) (void)hipFree(p); }
+    } ckb, stb;
+    ckb.ensure(ck_bytes);
+    stb.ensure(st_bytes);
+    CallBuffers B;
+    B.tables = static_cast<const unsigned char*>(PR.tables.p);
+    B.block_out = static_cast<double*>(E.block_out.ensure(nb.block_out));
+    B.reduced = static_cast<double*>(E.reduced.ensure((size_t(S.stride) + S.ivn) * 8));
+    if (S.ivn) {
+        B.ivbar_blocks = static_cast<double*>(E.ivbar_blocks.ensure(nb.ivbar_blocks));
+        B.ivbar = B.reduced + S.stride;
+    }
+    B.weights = static_cast<double*>(E.weights.ensure(nb.weights));
+    E.weights_on_device = nullptr;   // this call rewrites the weights buffer
+    B.ckpt = ckb.p;
+    cfk::LaunchParams P = launch_params(E, PR, S, B);
+    P.state = stb.p;
+
+    const int npay = S.npay;
+    const int npar = PR.md.kind == CF_MODEL_DUPIRE ? 1 + PR.md.n_spots * PR.md.n_times : 4;
+    vector<double> w(static_cast<size_t>(npay), 0.0), red(static_cast<size_t>(S.stride) + S.ivn),
+        col(static_cast<size_t>(npar)), agg(1);
+    auto launch = [&](int mode, int k) {
+        std::fill(w.begin(), w.end(), 0.0);
+        w[size_t(k)] = 1.0;
+        hip_check(hipMemcpyAsync(B.weights, w.data(), size_t(npay) * 8, hipMemcpyHostToDevice, E.stream), "upload");
+        P.ck_mode = mode;
+        Sequence q;
+        make_sequence(PR, S, B, P, &q);
+        q.cfg.item = true;
+        ready_sequence(q);
+        enqueue_sequence(q, E.stream, nullptr, nullptr);
+        hip_check(hipMemcpyAsync(red.data(), B.reduced, red.size() * 8, hipMemcpyDeviceToHost, E.stream), "download");
+        hip_check(hipStreamSynchronize(E.stream), "kernel execution");
+    };
+    // the forward: payoff means (no adjoint results yet)
+    launch(1, 0);
+    {
+        CallShape S_fwd = S;
+        S_fwd.adjoint = false;
+        Call c;
+        c.num = num;
+        c.means = payoff_means;
+        collect_results(PR, S_fwd, red.data(), red.data() + S.stride, c);
+    }
+    // one reverse per payoff
+    for (int k = 0; k < npay; ++k) {
+        launch(2, k);
+        Call c;
+        c.adjoint = true;
+        c.num = num;
+        c.agg_mean = agg.data();
+        c.risks = col.data();
+        collect_results(PR, S, red.data(), red.data() + S.stride, c);
+        for (int i = 0; i < npar; ++i) risks[size_t(i) * npay + k] = col[size_t(i)];
+    }
+    return true;
+}
+
+}  // namespace cfh
+
 int cf_mc_simul_aad_multi(const cf_model* mdl, const cf_product* prd, const cf_num* num, double* payoff_means,
                           double* risks)
 {
     return guarded([&] {
         if (!risks) fail(CF_ERR_ARG, "null risks");
+        validate_num(num);
+        validate_model(mdl);
+        if (num->aad_mode == CF_AAD_CHECKPOINT && num->use_sobol && run_multi(mdl, prd, num, payoff_means, risks))
+            return mdl->kind == CF_MODEL_BLACK_SCHOLES ? 4 : 1 + mdl->n_spots * mdl->n_times;
         const int npay = plan_product(prd).n_pay;
         validate_model(mdl);
         const int npar = mdl->kind == CF_MODEL_BLACK_SCHOLES ? 4 : 1 + mdl->n_spots * mdl->n_times;

@@ -77,6 +77,12 @@ CF_HD inline int ckpt_rows(int n_steps, bool dupire, bool recompute, int real_by
     return (n_steps + seg - 1) / seg;
 }
 
+// Rows of 64 reals of a path's end state kept between the forward-only and
+// the reverse-only launches (ck_mode 1, 2): the state, the product state
+// (alive, pnz, nzero, dead, euro, final forward, final numeraire) and, per
+// book barrier and fixing schedule, two values (1 each for other products).
+CF_HD inline int state_rows(bool book) { return 8 + 2 * (book ? kMaxBookBarriers : 1) + 2 * (book ? kMaxBookGroups : 1); }
+
 // Per-block accumulator row: payoff sums, payoff sums of squares, aggregate
 // sum, aggregate sum of squares, then up to kMaxParamAdj parameter adjoints
 // accumulated in registers (Dupire: spot; Black-Scholes: spot, vol, rate, div).
@@ -148,6 +154,12 @@ struct LaunchParams {
     int ivbar_copies;            // LDS local-vol adjoint grids per block: even and odd lanes (2) or one
     double* ivbar_blocks;        // [gridDim][n_steps][n_spots] local-vol adjoints
     double* per_path_agg;        // [n_path] or null
+    // checkpoint adjoint shared across payoffs (itemized risk): 0 one call
+    // (forward and reverse, checkpoint slot per wave); 1 forward only, the
+    // checkpoint slot per block of 64 Sobol indices kept and each path's end
+    // state written to `state`; 2 reverse only, from those slots and states
+    int ck_mode;
+    void* state;                 // [blocks of 64 paths][state_rows(product)][64] reals
 };
 
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
@@ -343,6 +355,7 @@ struct KernelConfig {
     bool lean;       // LEAN shared-memory layout
     bool recompute;  // adjoint by re-simulation instead of per-step checkpoints
     bool normals;    // normals read from a buffer (mrg32k3a) instead of Sobol
+    bool item;       // checkpoint kernel with the itemized-risk modes (LaunchParams::ck_mode)
 };
 
 // Path kernel families (precision r: 0 FP64, 1 FP32; adjoint a: AdjointKind;
@@ -360,6 +373,8 @@ CF_PICK_DECL(1, 0)
 CF_PICK_DECL(1, 1)
 CF_PICK_DECL(1, 2)
 #undef CF_PICK_DECL
+const void* pick_r0_a1_b2(int model, int prd, bool lean);
+const void* pick_r1_a1_b2(int model, int prd, bool lean);
 
 // Returns 0 on success, else a hipError_t value.
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block,

@@ -1,12 +1,16 @@
 // One (precision, adjoint, normals source) family of path kernels,
 // instantiated for every model, product and LDS layout.  Built once per
 // family with -DCF_INST_REAL=0|1 (FP64, FP32), -DCF_INST_ADJ=0|1|2
-// (cfk::AdjointKind) and -DCF_INST_NBUF=0|1 (Sobol, normals buffer), so the
-// families compile in parallel.
+// (cfk::AdjointKind) and -DCF_INST_NBUF=0|1|2 (Sobol, normals buffer, Sobol
+// with the itemized-risk modes: checkpoint family only), so the families
+// compile in parallel.
 #include <type_traits>
 
 #if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ) || !defined(CF_INST_NBUF)
 #error "CF_INST_REAL, CF_INST_ADJ and CF_INST_NBUF must be defined"
+#endif
+#if CF_INST_NBUF == 2 && CF_INST_ADJ != 1
+#error "the itemized-risk kernels are checkpoint kernels"
 #endif
 
 // FP32 pricing kernels run two 1024-thread blocks per CU (LDS allows it for
@@ -23,13 +27,16 @@ namespace {
 
 using Real = std::conditional_t<CF_INST_REAL == 0, double, float>;
 constexpr int kAdj = CF_INST_ADJ;
-constexpr bool kNbuf = CF_INST_NBUF != 0;
+// CF_INST_NBUF: 0 Sobol, 1 normals buffer, 2 Sobol with the itemized-risk
+// modes (forward only keeping end states, reverse only from them)
+constexpr bool kNbuf = CF_INST_NBUF == 1;
+constexpr bool kItem = CF_INST_NBUF == 2;
 
 template <int MODEL, int PRD>
 const void* ptr(bool lean)
 {
-    return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true, kNbuf>)
-                : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false, kNbuf>);
+    return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true, kNbuf, kItem>)
+                : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false, kNbuf, kItem>);
 }
 
 template <int MODEL>

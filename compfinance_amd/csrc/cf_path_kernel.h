@@ -375,7 +375,7 @@ struct ProductState {
 
 // ------------------------------------------------------------------ kernel
 
-template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF>
+template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF, bool ITEM>
 #ifndef CF_PATH_KERNEL_ATTR
 #define CF_PATH_KERNEL_ATTR
 #endif
@@ -480,11 +480,18 @@ path_kernel(const LaunchParams P)
     const int ck_rows = ckpt_rows(n, DUP, RECOMP, int(sizeof(R)));
     // bracket bytes after the states: the knot count of x_i at each step
     constexpr bool CKB = DUP && ADJ == ADJ_CHECKPOINT;
-    // (the Dupire checkpoint slot starts with kCkptGuardRows guard rows)
+    // (the Dupire checkpoint slot starts with kCkptGuardRows guard rows);
+    // a wave's slot, or with ck_mode 1 and 2 the slot of each block of 64
+    // Sobol indices (set per block below)
+    // (only the ITEM checkpoint kernels read ck_mode)
+    const int ck_mode = ITEM && ADJ == ADJ_CHECKPOINT ? P.ck_mode : 0;
     R* ck = AD ? static_cast<R*>(P.ckpt) + (size_t(slot) * ck_rows + (CKB ? kCkptGuardRows : 0)) * kWave : nullptr;
     // (the FULL layout is only chosen for grids of at most kMaxStoredKnots knots)
     const bool ck_lo = CKB && (!LEAN || m <= kMaxStoredKnots);
     uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave) : nullptr;
+    constexpr int kStateRows = BOOK ? 8 + 2 * kMaxBookBarriers + 2 * kMaxBookGroups : 12;
+    static_assert(kStateRows == (BOOK ? 8 + 2 * kMaxBookBarriers + 2 * kMaxBookGroups : 8 + 2 + 2),
+                  "state rows as state_rows() counts them");
     KnotSearch<R> ks;
     ks.knots = kxi + 2;     // the knots themselves: interval starts from interval 1
     ks.stride = 2;
@@ -549,6 +556,14 @@ path_kernel(const LaunchParams P)
         const uint32_t* hl = sl.hl;
         const bool active = sl.active;
         const int64_t pidx = sl.pidx;
+        // itemized risk: the checkpoint slot and the end state of this block of 64 paths
+        R* stq = nullptr;
+        if (ck_mode != 0) {
+            const size_t sb = size_t(q - q0);
+            ck = static_cast<R*>(P.ckpt) + (sb * ck_rows + (CKB ? kCkptGuardRows : 0)) * kWave;
+            if constexpr (CKB) ckl = reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave);
+            stq = static_cast<R*>(P.state) + sb * kStateRows * kWave + lane;
+        }
         // normal of step i: from the Sobol point, or from the buffer of
         // pre-generated normals ([step][path] of the call's paths)
         const size_t ncol = size_t(active ? pidx : 0);
@@ -669,7 +684,10 @@ path_kernel(const LaunchParams P)
                             const int col = bk_bar_col[b];
                             emit(col, alive * euro);
                             emit(col + 1, euro);
-                            if constexpr (AD) babar[b] = R(P.weights[col]) * euro;
+                            // weight x vanilla: the adjoint of the survival
+                            // factor (the ITEM kernels keep the vanilla and
+                            // weigh it before the reverse sweep)
+                            if constexpr (AD) babar[b] = ITEM ? euro : R(P.weights[col]) * euro;
                         }
                     }
                 }
@@ -691,16 +709,20 @@ path_kernel(const LaunchParams P)
                                                         : (avg - k > R(0) ? avg - k : R(0));
                                 const int col = bk_as_col[a];
                                 emit(col, intrinsic / numv);
-                                if constexpr (AD) {
+                                if constexpr (AD && !ITEM) {
                                     const R w = R(P.weights[col]);
                                     const R d = put ? (k > avg ? R(-1) : R(0)) : (avg > k ? R(1) : R(0));
                                     dsum = dsum + w * d;
                                     qsum = qsum + w * intrinsic;
                                 }
                             }
+                            // the adjoint of each fixing and the weighted
+                            // intrinsic values (the numeraire adjoint); the ITEM
+                            // kernels keep the average and the numeraire and
+                            // form both before the reverse sweep
                             if constexpr (AD) {
-                                gsum[g] = dsum * inv_n / numv;   // adjoint of each fixing
-                                gq[g] = qsum;
+                                gsum[g] = ITEM ? avg : dsum * inv_n / numv;
+                                gq[g] = ITEM ? numv : qsum;
                             }
                         }
                     }
@@ -719,87 +741,160 @@ path_kernel(const LaunchParams P)
             }
         };
 
-        // ---- forward sweep
-        // adjoint kernels: waves in the forward (vector-ALU bound) take issue
-        // priority over waves in the reverse (bound by LDS atomics)
-        if constexpr (AD) __builtin_amdgcn_s_setprio(1);
         // Dupire barrier: a sample other than the last one changes nothing
         // while the spot is under the smoothing zone, so the spot is not
         // formed there (forward and reverse)
         constexpr bool UNDER = DUP && PRD == PRD_BARRIER;
         const R x_lo = UNDER ? R(P.bar_xlo) : R(0);
         R x;
-        if constexpr (DUP) x = R(P.x0); else x = R(s0);   // Dupire: log-spot state; BS: spot
-        {
-            const int s = step_sample[0];
-            if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
-        }
-        // step i0 + k, with every per-step address a fixed offset (k) from a
-        // base formed once per group of steps at i0
-        auto fwd_step = [&](int i0, int k) {
-            const int i = i0 + k;
-            R g;
-            if constexpr (NBUF || LEAN) g = draw(i);
-            else g = inv_normal(hl[i0 + k] ^ (lo_tab + i0 * kWave + lane)[k * kWave], invn_tab);
-            if constexpr (DUP) {
-                R* ckg = ck + i0 * kWave + lane;
-                if constexpr (ADJ == ADJ_CHECKPOINT) ckg[k * kWave] = x;
-                if constexpr (RECOMP)
-                    if ((i & (kSeg - 1)) == 0) ck[(i / kSeg) * kWave + lane] = x;
-                const int lo = count_le<LEAN>(ks, x);
-                if constexpr (CKB)
-                    if (ck_lo) (ckl + i0 * kWave + lane)[k * kWave] = uint8_t(lo);
-                const Bracket<R> b = local_vol_at(kxi, (iv + i0 * mp) + k * mp, x, lo);
-                x = euler_step(x, b.vol, g);
-            } else {
-                if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
-                x = x * rexp(drift[i] + stdv[i] * g);      // mcMdlBS.h:344-345
-            }
-            // a wave-uniform test: the sample's code runs with all lanes (a lane
-            // under the zone changes nothing there)
-            const int s = __builtin_amdgcn_readfirstlane((step_sample + i0 + 1)[k]);
-            if (s >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(x > x_lo) != 0))
-                observe(s, DUP ? state_spot(x, s0) : x);
-        };
-        // FP32 Dupire adjoint kernels other than the book: groups of four steps
-        // (the book's sample code, four times over, and FP64 measured slower)
-        constexpr int UF = AD && DUP && !BOOK && sizeof(R) == 4 ? 4 : 1;
-        int i = 0;
-        if constexpr (UF > 1) {
-            for (; i + UF <= n; i += UF) {
+        if (ck_mode == 2) {
+            // reverse only: the end state the forward-only launch kept
+            x = stq[0];
+            ps.alive = stq[1 * kWave];
+            ps.pnz = stq[2 * kWave];
+            ps.nzero = int(stq[3 * kWave]);
+            ps.dead = stq[4 * kWave] != R(0);
+            ps.euro = stq[5 * kWave];
+            ps.fin_f = stq[6 * kWave];
+            ps.fin_num = stq[7 * kWave];
 #pragma unroll
-                for (int k = 0; k < UF; ++k) fwd_step(i, k);
+            for (int b = 0; b < NBAR; ++b) {
+                bst[b] = stq[(8 + b) * kWave];
+                babar[b] = stq[(8 + NBAR + b) * kWave];
+            }
+#pragma unroll
+            for (int g = 0; g < NGRP; ++g) {
+                gsum[g] = stq[(8 + 2 * NBAR + g) * kWave];
+                gq[g] = stq[(8 + 2 * NBAR + NGRP + g) * kWave];
+            }
+            nzw = __builtin_amdgcn_ballot_w64(ps.nzero != 0) != 0;
+        } else {
+            // ---- forward sweep
+            // adjoint kernels: waves in the forward (vector-ALU bound) take issue
+            // priority over waves in the reverse (bound by LDS atomics)
+            if constexpr (AD) __builtin_amdgcn_s_setprio(1);
+            if constexpr (DUP) x = R(P.x0); else x = R(s0);   // Dupire: log-spot state; BS: spot
+            {
+                const int s = step_sample[0];
+                if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
+            }
+            // step i0 + k, with every per-step address a fixed offset (k) from a
+            // base formed once per group of steps at i0
+            auto fwd_step = [&](int i0, int k) {
+                const int i = i0 + k;
+                R g;
+                if constexpr (NBUF || LEAN) g = draw(i);
+                else g = inv_normal(hl[i0 + k] ^ (lo_tab + i0 * kWave + lane)[k * kWave], invn_tab);
+                if constexpr (DUP) {
+                    R* ckg = ck + i0 * kWave + lane;
+                    if constexpr (ADJ == ADJ_CHECKPOINT) ckg[k * kWave] = x;
+                    if constexpr (RECOMP)
+                        if ((i & (kSeg - 1)) == 0) ck[(i / kSeg) * kWave + lane] = x;
+                    const int lo = count_le<LEAN>(ks, x);
+                    if constexpr (CKB)
+                        if (ck_lo) (ckl + i0 * kWave + lane)[k * kWave] = uint8_t(lo);
+                    const Bracket<R> b = local_vol_at(kxi, (iv + i0 * mp) + k * mp, x, lo);
+                    x = euler_step(x, b.vol, g);
+                } else {
+                    if constexpr (ADJ == ADJ_CHECKPOINT) ck[i * kWave + lane] = g;
+                    x = x * rexp(drift[i] + stdv[i] * g);      // mcMdlBS.h:344-345
+                }
+                // a wave-uniform test: the sample's code runs with all lanes (a lane
+                // under the zone changes nothing there)
+                const int s = __builtin_amdgcn_readfirstlane((step_sample + i0 + 1)[k]);
+                if (s >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(x > x_lo) != 0))
+                    observe(s, DUP ? state_spot(x, s0) : x);
+            };
+            // FP32 Dupire adjoint kernels other than the book: groups of four steps
+            // (the book's sample code, four times over, and FP64 measured slower)
+            constexpr int UF = AD && DUP && !BOOK && sizeof(R) == 4 ? 4 : 1;
+            int i = 0;
+            if constexpr (UF > 1) {
+                for (; i + UF <= n; i += UF) {
+    #pragma unroll
+                    for (int k = 0; k < UF; ++k) fwd_step(i, k);
+                }
+            }
+            for (; i < n; ++i) fwd_step(i, 0);
+
+            // ---- payoffs of the barrier (known at the end of the path)
+            if constexpr (PRD == PRD_BARRIER) {
+                if (last >= 0) {   // mcPrd.h:281-287, at the last sample (timeline point n)
+                    const R S = DUP ? state_spot(x, s0) : x;
+                    const R F = DUP ? S : S * R(s_ff[last]);
+                    const R numv = DUP ? R(1) : (P.spot_measure ? R(s_num[last]) * S : R(s_num[last]));
+                    const R k = R(P.strike);
+                    const R intrinsic = P.put ? (k - F > R(0) ? k - F : R(0)) : (F - k > R(0) ? F - k : R(0));
+                    ps.euro = intrinsic / numv;
+                    ps.fin_f = F;
+                    ps.fin_num = numv;
+                }
+                const R p0 = ps.dead ? R(0) : ps.alive * ps.euro;
+                const R p1 = ps.euro;
+                const double v0 = active ? double(p0) : 0.0, v1 = active ? double(p1) : 0.0;
+                const double a0 = wave_sum(v0), a1 = wave_sum(v1), b0 = wave_sum(v0 * v0), b1 = wave_sum(v1 * v1);
+                if (lane == kSumLane) { accw[0] += a0; accw[1] += a1; accw[npay] += b0; accw[npay + 1] += b1; }
+                if (P.per_path && active) { P.per_path[pidx * 2] = v0; P.per_path[pidx * 2 + 1] = v1; }
+                if constexpr (AD) ps.aggp = R(P.weights[0]) * p0 + R(P.weights[1]) * p1;
             }
         }
-        for (; i < n; ++i) fwd_step(i, 0);
-
-        // ---- payoffs of the barrier (known at the end of the path)
-        if constexpr (PRD == PRD_BARRIER) {
-            if (last >= 0) {   // mcPrd.h:281-287, at the last sample (timeline point n)
-                const R S = DUP ? state_spot(x, s0) : x;
-                const R F = DUP ? S : S * R(s_ff[last]);
-                const R numv = DUP ? R(1) : (P.spot_measure ? R(s_num[last]) * S : R(s_num[last]));
-                const R k = R(P.strike);
-                const R intrinsic = P.put ? (k - F > R(0) ? k - F : R(0)) : (F - k > R(0) ? F - k : R(0));
-                ps.euro = intrinsic / numv;
-                ps.fin_f = F;
-                ps.fin_num = numv;
-            }
-            const R p0 = ps.dead ? R(0) : ps.alive * ps.euro;
-            const R p1 = ps.euro;
-            const double v0 = active ? double(p0) : 0.0, v1 = active ? double(p1) : 0.0;
-            const double a0 = wave_sum(v0), a1 = wave_sum(v1), b0 = wave_sum(v0 * v0), b1 = wave_sum(v1 * v1);
-            if (lane == kSumLane) { accw[0] += a0; accw[1] += a1; accw[npay] += b0; accw[npay + 1] += b1; }
-            if (P.per_path && active) { P.per_path[pidx * 2] = v0; P.per_path[pidx * 2 + 1] = v1; }
-            if constexpr (AD) ps.aggp = R(P.weights[0]) * p0 + R(P.weights[1]) * p1;
-        }
-
         if constexpr (AD) {
             __builtin_amdgcn_s_setprio(0);
-            const double agg = active ? double(ps.aggp) : 0.0;
-            const double as = wave_sum(agg), aq = wave_sum(agg * agg);
-            if (lane == kSumLane) { accw[2 * npay] += as; accw[2 * npay + 1] += aq; }
-            if (P.per_path_agg && active) P.per_path_agg[pidx] = agg;
+            if (ck_mode != 2) {
+                const double agg = active ? double(ps.aggp) : 0.0;
+                const double as = wave_sum(agg), aq = wave_sum(agg * agg);
+                if (lane == kSumLane) { accw[2 * npay] += as; accw[2 * npay + 1] += aq; }
+                if (P.per_path_agg && active) P.per_path_agg[pidx] = agg;
+            }
+            if (ck_mode == 1) {
+                // forward only: keep the end state for the reverse-only launches
+                stq[0] = x;
+                stq[1 * kWave] = ps.alive;
+                stq[2 * kWave] = ps.pnz;
+                stq[3 * kWave] = R(ps.nzero);
+                stq[4 * kWave] = ps.dead ? R(1) : R(0);
+                stq[5 * kWave] = ps.euro;
+                stq[6 * kWave] = ps.fin_f;
+                stq[7 * kWave] = ps.fin_num;
+#pragma unroll
+                for (int b = 0; b < NBAR; ++b) {
+                    stq[(8 + b) * kWave] = bst[b];
+                    stq[(8 + NBAR + b) * kWave] = babar[b];
+                }
+#pragma unroll
+                for (int g = 0; g < NGRP; ++g) {
+                    stq[(8 + 2 * NBAR + g) * kWave] = gsum[g];
+                    stq[(8 + 2 * NBAR + NGRP + g) * kWave] = gq[g];
+                }
+                continue;
+            }
+            if constexpr (BOOK && ITEM) {
+                // the weighted adjoints: each barrier's weight x vanilla (the
+                // adjoint of its survival factor); per fixing schedule, from
+                // the average and the numeraire, the adjoint of each fixing
+                // sum_a w_a d max(.)/d avg / n / numeraire and the weighted
+                // intrinsic values (the numeraire adjoint)
+#pragma unroll
+                for (int b = 0; b < NBAR; ++b)
+                    if (b < P.bk.n_bar) babar[b] = R(P.weights[bk_bar_col[b]]) * babar[b];
+#pragma unroll
+                for (int g = 0; g < NGRP; ++g) {
+                    if (g >= P.bk.n_grp) continue;
+                    const R avg = gsum[g], numv = gq[g], inv_n = bk_ginv[g];
+                    R dsum = R(0), qsum = R(0);
+                    for (int a = bk_grp_first[g]; a < bk_grp_first[g + 1]; ++a) {
+                        const R k = R(bk_as_strike[a]);
+                        const bool put = bk_as_put[a] != 0;
+                        const R intrinsic = put ? (k - avg > R(0) ? k - avg : R(0)) : (avg - k > R(0) ? avg - k : R(0));
+                        const R w = R(P.weights[bk_as_col[a]]);
+                        const R d = put ? (k > avg ? R(-1) : R(0)) : (avg > k ? R(1) : R(0));
+                        dsum = dsum + w * d;
+                        qsum = qsum + w * intrinsic;
+                    }
+                    gsum[g] = dsum * inv_n / numv;
+                    gq[g] = qsum;
+                }
+            }
 
             // adjoints of the forward, numeraire and discount at sample s
             const R w0 = R(P.weights[0]);

@@ -127,3 +127,41 @@ def test_book_risk_is_the_weighted_sum_of_its_parts(engine):
         tot = tot + cf.mcParallelSimulAAD(p, mdl, n, weights=w[col:col + npk]).risks
         col += npk
     np.testing.assert_allclose(rb.risks, tot, rtol=1e-9, atol=1e-12 * np.abs(tot).max())
+
+
+@pytest.mark.parametrize("prec", [cf.FP64, cf.FP32])
+@pytest.mark.parametrize("mname", ["lv30", "bs"])
+def test_itemized_book_risks_are_the_one_hot_runs(engine, mname, prec):
+    """mcParallelSimulAADMulti on a book (one forward kept, one reverse per
+    payoff): column k equals the single AAD run with one-hot weights on
+    payoff k (the same kernels on the same paths, to the rounding of the LDS
+    atomic sums), and the means equal those runs' bit for bit."""
+    mdl, _ = model(mname)
+    prd, _ = book(SMALL)
+    n = 3000                                   # ragged: not a multiple of 64
+    num = cf.NumericalParam(numPath=n, precision=prec)
+    means, risks = cf.mcParallelSimulAADMulti(prd, mdl, num=num)
+    npay = len(prd.payoffLabels())
+    assert risks.shape == (mdl.numParams(), npay)
+    for k in range(npay):
+        w = np.zeros(npay)
+        w[k] = 1.0
+        one = cf.mcParallelSimulAAD(prd, mdl, weights=w, num=num)
+        np.testing.assert_array_equal(means, one.payoff_means)
+        ref = one.risks
+        tol = (1e-12 if prec == cf.FP64 else 1e-6) * max(np.abs(ref).max(), 1e-300)
+        assert np.abs(risks[:, k] - ref).max() <= tol, (k, np.abs(risks[:, k] - ref).max())
+
+
+def test_itemized_c4_book_risks_sum_to_the_aggregate(engine):
+    """The C4 book (55 payoffs, 2^20 paths, FP32): the notional-weighted sum
+    of the itemized risks is the aggregated risk of the same paths."""
+    mdl = cf.Dupire(**W.dupire_args(30))
+    prd = cf.Book([CLS[k](**kw) for k, kw in W.c4_parts()])
+    w = W.c4_notionals(len(prd.payoffLabels()))
+    num = cf.NumericalParam(numPath=W.C4_PATHS, precision=cf.FP32)
+    means, risks = cf.mcParallelSimulAADMulti(prd, mdl, num=num)
+    agg = cf.mcParallelSimulAAD(prd, mdl, weights=w, num=num)
+    np.testing.assert_allclose(means, agg.payoff_means, rtol=1e-12, atol=0)
+    combined = risks @ w
+    assert np.abs(combined - agg.risks).max() <= 1e-5 * np.abs(agg.risks).max()
