@@ -76,7 +76,7 @@ class cf_kernel_stats(ctypes.Structure):
         ("main_kernel_ms", ctypes.c_double), ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
         ("grid_blocks", ctypes.c_int), ("block_threads", ctypes.c_int), ("lds_bytes", ctypes.c_int),
         ("path_steps", ctypes.c_int64), ("setup_ms", ctypes.c_double), ("host_ms", ctypes.c_double),
-        ("lean_layout", ctypes.c_int),
+        ("lean_layout", ctypes.c_int), ("enqueue_ms", ctypes.c_double), ("wait_ms", ctypes.c_double),
     ]
 
 

@@ -40,16 +40,14 @@
 
 namespace cfh {
 
-constexpr int kDefaultBlockThreads = 1024;
-
 // Launch-shape overrides for tuning sweeps: CF_BLOCK_THREADS (64..1024, a
 // multiple of 64) and CF_LAYOUT (full, lean).  Unset in normal use.
-int block_threads()
+int forced_block_threads()   // 0: by path count
 {
     static const int v = [] {
         const char* e = std::getenv("CF_BLOCK_THREADS");
-        const int b = e ? std::atoi(e) : kDefaultBlockThreads;
-        return (b >= 64 && b <= 1024 && b % 64 == 0) ? b : kDefaultBlockThreads;
+        const int b = e ? std::atoi(e) : 0;
+        return (b >= 64 && b <= 1024 && b % 64 == 0) ? b : 0;
     }();
     return v;
 }
@@ -702,7 +700,7 @@ void validate_num(const cf_num* num)
 
 // Byte image of everything a prepared plan depends on.
 // adj: cfk::AdjointKind
-string plan_key(const cf_model* m, const cf_product* p, int precision, int adj)
+string plan_key(const cf_model* m, const cf_product* p, int precision, int adj, int block)
 {
     string k;
     auto put = [&k](const void* d, size_t n) { k.append(static_cast<const char*>(d), n); };
@@ -710,6 +708,7 @@ string plan_key(const cf_model* m, const cf_product* p, int precision, int adj)
     auto puti = [&](int x) { put(&x, sizeof x); };
     puti(precision);
     puti(adj);
+    puti(block);
     puti(m->kind);
     putd(m->spot);
     if (m->kind == CF_MODEL_BLACK_SCHOLES) {
@@ -788,7 +787,22 @@ vector<uint64_t> mrg_jumps(int dim)
 
 // Plans, tables and layout of a model and product; FULL layout when it fits,
 // LEAN otherwise.
-std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj)
+int call_block(const Engine& E, const cf_num* num, bool fp64)
+{
+    if (const int f = forced_block_threads()) return f;
+    // measured on MI355X (C3 grid, 2^12..2^19 paths): 512-thread blocks up to
+    // 8 waves per compute unit cut the FP32 AAD kernel from 0.16 to 0.12-0.13
+    // ms and FP32 pricing from 0.076 to 0.067; FP64 AAD runs 0.20-0.21 ms on
+    // 256 threads up to 4 waves per compute unit, against 0.34-0.36 on 1024
+    const int64_t waves = (std::max<int64_t>(num->num_path, 1) + cfk::kWave - 1) / cfk::kWave;
+    const int64_t per_cu = (waves + std::max(E.num_cus, 1) - 1) / std::max(E.num_cus, 1);
+    int w = fp64 ? 4 : 8;
+    while (w < 16 && w < per_cu) w *= 2;
+    return w * cfk::kWave;
+}
+
+std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj,
+                                  int block)
 {
     const bool adjoint = adj != cfk::ADJ_NONE;
     auto P = std::make_shared<Prepared>();
@@ -817,7 +831,7 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
     const bool book = kernel_product(pd) == cfk::PRD_BOOK;
     if (book) P->book = build_book(pd, md.s_t0);
     P->cfg = cfk::KernelConfig{fp64 ? 0 : 1, md.kind, kernel_product(pd), adjoint, false, adj == cfk::ADJ_RECOMPUTE};
-    P->block = std::min(block_threads(), cfk::book_block_limit(P->cfg.product, fp64 ? 8 : 4, adj));
+    P->block = std::min(block, cfk::book_block_limit(P->cfg.product, fp64 ? 8 : 4, adj));
     const int waves = P->block / cfk::kWave;
     const int forced = forced_layout();
     auto lds_bytes = [&](int copies) {
@@ -926,11 +940,11 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
 }
 
 std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj,
-                                 bool with_tables)
+                                 bool with_tables, int block)
 {
     if (!mdl) fail(CF_ERR_ARG, "null model");
     if (!prd) fail(CF_ERR_ARG, "null product");
-    const string key = plan_key(mdl, prd, fp64 ? CF_FP64 : CF_FP32, adj);
+    const string key = plan_key(mdl, prd, fp64 ? CF_FP64 : CF_FP32, adj, block);
     std::lock_guard<std::mutex> lk(E.plan_mu);
     std::shared_ptr<Prepared> P;
     for (auto it = E.plans.begin(); it != E.plans.end(); ++it) {
@@ -941,7 +955,7 @@ std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_produc
         }
     }
     if (!P) {
-        P = prepare(E, mdl, prd, fp64, adj);
+        P = prepare(E, mdl, prd, fp64, adj, block);
         E.plans.emplace_front(key, P);
         if (E.plans.size() > kMaxPlans) E.plans.pop_back();
     }
@@ -1196,16 +1210,19 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     Engine& E = engine();
     std::lock_guard<std::mutex> lk(E.sync_mu);
     hip_check(hipSetDevice(E.device), "hipSetDevice");
-    const std::shared_ptr<Prepared> PP = lookup(E, mdl, prd, fp64, adjoint_kind(c), true);
+    const std::shared_ptr<Prepared> PP = lookup(E, mdl, prd, fp64, adjoint_kind(c), true, call_block(E, c.num, fp64));
     const Prepared& PR = *PP;
     const CallShape S = shape_call(E, PR, c.num, c.adjoint);
     const CallBytes nb = call_bytes(S);
 
+    double* h_in = static_cast<double*>(E.pin_in.ensure(size_t(S.npay) * 8));
+    double* h_out = static_cast<double*>(E.pin_out.ensure((size_t(S.stride) + S.ivn) * 8));
     CallBuffers B;
     B.tables = static_cast<const unsigned char*>(PR.tables.p);
     B.block_out = static_cast<double*>(E.block_out.ensure(nb.block_out));
-    // the reduced sums and the local-vol adjoints side by side, one download
-    B.reduced = static_cast<double*>(E.reduced.ensure((size_t(S.stride) + S.ivn) * 8));
+    // the reduced sums and the local-vol adjoints side by side, written by the
+    // reduction straight into mapped host memory
+    B.reduced = static_cast<double*>(E.pin_out.dev);
     if (S.ivn) {
         B.ivbar_blocks = static_cast<double*>(E.ivbar_blocks.ensure(nb.ivbar_blocks));
         B.ivbar = B.reduced + S.stride;
@@ -1217,8 +1234,6 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     if (S.mrg) B.normals = E.normals.ensure(nb.normals);
     if (c.per_path) B.per_path = static_cast<double*>(E.per_path.ensure(size_t(S.n_path) * S.npay * 8));
     if (c.per_path_agg) B.per_path_agg = static_cast<double*>(E.per_path_agg.ensure(size_t(S.n_path) * 8));
-    double* h_in = static_cast<double*>(E.pin_in.ensure(size_t(S.npay) * 8));
-    double* h_out = static_cast<double*>(E.pin_out.ensure((size_t(S.stride) + S.ivn) * 8));
 
     hip_check(hipEventRecord(E.ev[0], E.stream), "hipEventRecord");
     if (c.adjoint) {
@@ -1239,15 +1254,15 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     enqueue_call(PR, S, B, P, E.stream, E.ev[1], E.ev[2]);
     hip_check(hipEventRecord(E.ev[3], E.stream), "hipEventRecord");
 
-    hip_check(hipMemcpyAsync(h_out, B.reduced, (size_t(S.stride) + S.ivn) * 8, hipMemcpyDeviceToHost, E.stream),
-              "download");
     if (B.per_path)
         hip_check(hipMemcpyAsync(c.per_path, B.per_path, size_t(S.n_path) * S.npay * 8, hipMemcpyDeviceToHost,
                                  E.stream), "download");
     if (B.per_path_agg)
         hip_check(hipMemcpyAsync(c.per_path_agg, B.per_path_agg, size_t(S.n_path) * 8, hipMemcpyDeviceToHost,
                                  E.stream), "download");
+    const double queued_ms = ms_since(t_start);
     hip_check(hipStreamSynchronize(E.stream), "kernel execution");
+    const double waited_ms = ms_since(t_start);
 
     float t_main = 0, t_red = 0, t_all = 0;
     hip_check(hipEventElapsedTime(&t_main, E.ev[1], E.ev[2]), "event");
@@ -1266,6 +1281,8 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
     g_stats.setup_ms = setup_ms;
     g_stats.host_ms = ms_since(t_start);
     g_stats.lean_layout = PR.cfg.lean ? 1 : 0;
+    g_stats.enqueue_ms = queued_ms - setup_ms;
+    g_stats.wait_ms = waited_ms - queued_ms;
     return CF_OK;
 }
 
@@ -1443,7 +1460,8 @@ bool run_multi(const cf_model* mdl, const cf_product* prd, const cf_num* num, do
     Engine& E = engine();
     std::lock_guard<std::mutex> lk(E.sync_mu);
     hip_check(hipSetDevice(E.device), "hipSetDevice");
-    const std::shared_ptr<Prepared> PP = lookup(E, mdl, prd, fp64, cfk::ADJ_CHECKPOINT, true);
+    const std::shared_ptr<Prepared> PP =
+        lookup(E, mdl, prd, fp64, cfk::ADJ_CHECKPOINT, true, call_block(E, num, fp64));
     const Prepared& PR = *PP;
     const CallShape S = shape_call(E, PR, num, true);
     const CallBytes nb = call_bytes(S);
@@ -1557,7 +1575,7 @@ int64_t cf_aad_footprint(const cf_model* mdl, const cf_product* prd, const cf_nu
         Engine& E = engine();
         const bool fp64 = num->precision == CF_FP64;
         const int adj = num->aad_mode == CF_AAD_RECOMPUTE ? cfk::ADJ_RECOMPUTE : cfk::ADJ_CHECKPOINT;
-        const std::shared_ptr<Prepared> P = lookup(E, mdl, prd, fp64, adj, false);
+        const std::shared_ptr<Prepared> P = lookup(E, mdl, prd, fp64, adj, false, call_block(E, num, fp64));
         bytes = int64_t(call_bytes(shape_call(E, *P, num, true)).total() + P->staged.size());
         return CF_OK;
     });

@@ -150,18 +150,23 @@ struct DevBuf {
 };
 
 // Grow-only pinned host buffer (uploads and downloads without staging copies).
+// Pinned host memory, coherent and mapped: kernels write results straight
+// into it (dev is its device address), so a call ends with its last kernel
+// rather than with a copy.
 struct PinnedBuf {
     void* p = nullptr;
+    void* dev = nullptr;
     size_t cap = 0;
     void* ensure(size_t bytes)
     {
         if (bytes <= cap && p) return p;
         if (p) hip_check(hipHostFree(p), "hipHostFree");
-        p = nullptr;
+        p = dev = nullptr;
         cap = 0;
         const size_t want = std::max<size_t>(bytes, 4096);
-        hip_check(hipHostMalloc(&p, want, hipHostMallocDefault), "hipHostMalloc");
+        hip_check(hipHostMalloc(&p, want, hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
         cap = want;
+        hip_check(hipHostGetDevicePointer(&dev, p, 0), "hipHostGetDevicePointer");
         return p;
     }
 };
@@ -257,14 +262,24 @@ vector<string> payoff_labels(const ProductPlan& d);
 
 // adj: cfk::AdjointKind.  The host plan of a model and product (tables
 // staged on the host, none on the device) and its cache key.
-std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj);
-string plan_key(const cf_model* m, const cf_product* p, int precision, int adj);
+// block: threads per block of the path kernel (call_block), capped for books.
+std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj,
+                                  int block);
+string plan_key(const cf_model* m, const cf_product* p, int precision, int adj, int block);
 int adjoint_kind(const Call& c);
+
+// Threads per block of the path kernel for a call of num->num_path paths: the
+// fewest waves per block that still give every compute unit its share of the
+// paths' waves, at least 8 (FP32) or 4 (FP64) and at most 16.  Fewer waves on
+// a compute unit run each path's dependent steps faster, so a call too small
+// to fill the device finishes sooner; from 16 waves per compute unit on, the
+// 1024-thread blocks are the fastest.
+int call_block(const Engine& E, const cf_num* num, bool fp64);
 
 // adj: cfk::AdjointKind.  Host plan from the engine's cache (prepared on a
 // miss); with_tables also makes its tables resident on the device.
 std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64, int adj,
-                                 bool with_tables);
+                                 bool with_tables, int block);
 
 // Launch geometry of one call: a persistent grid of at most the resident
 // blocks of the device, enough waves for the 64-path Sobol blocks.

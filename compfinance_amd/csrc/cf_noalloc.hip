@@ -53,11 +53,11 @@ std::mutex g_plans_mu;
 std::list<std::pair<std::string, std::shared_ptr<cfh::Prepared>>> g_plans;
 
 std::shared_ptr<cfh::Prepared> host_plan(cfh::Engine& E, const cf_model* mdl, const cf_product* prd, bool fp64,
-                                         int adj)
+                                         int adj, int block)
 {
     if (!mdl) cfh::fail(CF_ERR_ARG, "null model");
     if (!prd) cfh::fail(CF_ERR_ARG, "null product");
-    const std::string key = cfh::plan_key(mdl, prd, fp64 ? CF_FP64 : CF_FP32, adj);
+    const std::string key = cfh::plan_key(mdl, prd, fp64 ? CF_FP64 : CF_FP32, adj, block);
     std::lock_guard<std::mutex> lk(g_plans_mu);
     for (auto it = g_plans.begin(); it != g_plans.end(); ++it) {
         if (it->first == key) {
@@ -65,7 +65,7 @@ std::shared_ptr<cfh::Prepared> host_plan(cfh::Engine& E, const cf_model* mdl, co
             return g_plans.front().second;
         }
     }
-    auto P = cfh::prepare(E, mdl, prd, fp64, adj);
+    auto P = cfh::prepare(E, mdl, prd, fp64, adj, block);
     g_plans.emplace_front(key, P);
     if (g_plans.size() > kMaxNoAllocPlans) g_plans.pop_back();
     return P;
@@ -84,7 +84,7 @@ int run_noalloc(const cf_model* mdl, const cf_product* prd, cfh::Call& c)
     std::lock_guard<std::mutex> lk(E.sync_mu);
     hip_check(hipSetDevice(E.device), "hipSetDevice");
     unsigned char* base = scratch_on(E.device);
-    const std::shared_ptr<Prepared> PP = host_plan(E, mdl, prd, fp64, adjoint_kind(c));
+    const std::shared_ptr<Prepared> PP = host_plan(E, mdl, prd, fp64, adjoint_kind(c), call_block(E, c.num, fp64));
     const Prepared& PR = *PP;
     const CallShape S = shape_call(E, PR, c.num, true);
     const CallBytes nb = call_bytes(S);

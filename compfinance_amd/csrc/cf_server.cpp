@@ -233,7 +233,8 @@ size_t block_bytes(const Prepared& PR, const CallShape& S)
     return ((PR.staged.size() + 255) & ~size_t(255)) + call_bytes(S).total();
 }
 
-// Host-device copies of a request around its sequence.
+// Host-to-device copies of a request before its sequence (the results come
+// back through the slot's mapped host memory, written by the reduction).
 struct Copies {
     void* tables_dst;
     const void* tables_src;
@@ -241,12 +242,6 @@ struct Copies {
     void* weights_dst;
     const void* weights_src;
     size_t weights_bytes;
-    void* out_dst;
-    const void* out_src;
-    size_t out_bytes;
-    void* ivbar_dst;
-    const void* ivbar_src;
-    size_t ivbar_bytes;
 };
 
 void enqueue_request(const Sequence& q, const Copies& c, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
@@ -255,9 +250,6 @@ void enqueue_request(const Sequence& q, const Copies& c, hipStream_t st, hipEven
     if (c.weights_bytes)
         hip_check(hipMemcpyAsync(c.weights_dst, c.weights_src, c.weights_bytes, hipMemcpyHostToDevice, st), "upload");
     enqueue_sequence(q, st, ev0, ev1);
-    hip_check(hipMemcpyAsync(c.out_dst, c.out_src, c.out_bytes, hipMemcpyDeviceToHost, st), "download");
-    if (c.ivbar_bytes)
-        hip_check(hipMemcpyAsync(c.ivbar_dst, c.ivbar_src, c.ivbar_bytes, hipMemcpyDeviceToHost, st), "download");
 }
 
 // The slot's graph of (q, c): found by its bytes; else, on the second
@@ -320,13 +312,19 @@ void launch(Server& S, Slot& s, Request& r)
         p += (bytes + 255) & ~size_t(255);
         return q;
     };
+    // the reduction writes the sums and the local-vol adjoints straight into
+    // the slot's mapped host memory (their arena space stays reserved)
+    s.pin_out.ensure((size_t(SH.stride) + SH.ivn) * 8);
+    double* hout_dev = static_cast<double*>(s.pin_out.dev);
     CallBuffers B;
     unsigned char* tables = take(PR.staged.size());
     B.tables = tables;
     B.block_out = reinterpret_cast<double*>(take(nb.block_out));
-    B.reduced = reinterpret_cast<double*>(take(nb.reduced));
+    (void)take(nb.reduced);
+    B.reduced = hout_dev;
     B.ivbar_blocks = reinterpret_cast<double*>(take(nb.ivbar_blocks));
-    B.ivbar = reinterpret_cast<double*>(take(nb.ivbar));
+    (void)take(nb.ivbar);
+    B.ivbar = SH.ivn ? hout_dev + SH.stride : nullptr;
     B.weights = reinterpret_cast<double*>(take(nb.weights));
     B.ckpt = take(nb.ckpt);
     B.normals = take(nb.normals);
@@ -335,7 +333,6 @@ void launch(Server& S, Slot& s, Request& r)
     unsigned char* hin = static_cast<unsigned char*>(s.pin_in.ensure(tb + nb.weights + 8));
     std::memcpy(hin, PR.staged.data(), tb);
     if (r.aad) std::memcpy(hin + tb, r.weights.data(), nb.weights);
-    double* hout = static_cast<double*>(s.pin_out.ensure((size_t(SH.stride) + SH.ivn) * 8));
     Copies c;
     std::memset(&c, 0, sizeof c);
     c.tables_dst = tables;
@@ -345,14 +342,6 @@ void launch(Server& S, Slot& s, Request& r)
         c.weights_dst = B.weights;
         c.weights_src = hin + tb;
         c.weights_bytes = nb.weights;
-    }
-    c.out_dst = hout;
-    c.out_src = B.reduced;
-    c.out_bytes = size_t(SH.stride) * 8;
-    if (SH.ivn) {
-        c.ivbar_dst = hout + SH.stride;
-        c.ivbar_src = B.ivbar;
-        c.ivbar_bytes = SH.ivn * 8;
     }
     const cfk::LaunchParams P = launch_params(*S.E, PR, SH, B);
     Sequence q;
@@ -569,7 +558,8 @@ int cf_server_submit(cf_server* srv, const cf_request* req, int64_t* id)
         const bool fp64 = r->num.precision == CF_FP64;
         int adj = cfk::ADJ_NONE;
         if (r->aad) adj = r->num.aad_mode == CF_AAD_RECOMPUTE ? cfk::ADJ_RECOMPUTE : cfk::ADJ_CHECKPOINT;
-        r->plan = lookup(*S.E, &r->model, &r->prd.d, fp64, adj, false);
+        const int block = call_block(*S.E, &r->num, fp64);
+        r->plan = lookup(*S.E, &r->model, &r->prd.d, fp64, adj, false, block);
         r->shape = shape_call(*S.E, *r->plan, &r->num, r->aad);
         r->bytes = block_bytes(*r->plan, r->shape);
         if (r->aad) {
@@ -579,7 +569,7 @@ int cf_server_submit(cf_server* srv, const cf_request* req, int64_t* id)
             else r->weights[0] = 1.0;
         }
         if (adj == cfk::ADJ_CHECKPOINT) {
-            r->plan_rc = lookup(*S.E, &r->model, &r->prd.d, fp64, cfk::ADJ_RECOMPUTE, false);
+            r->plan_rc = lookup(*S.E, &r->model, &r->prd.d, fp64, cfk::ADJ_RECOMPUTE, false, block);
             r->shape_rc = shape_call(*S.E, *r->plan_rc, &r->num, true);
             r->bytes_rc = block_bytes(*r->plan_rc, r->shape_rc);
         }

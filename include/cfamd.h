@@ -369,6 +369,8 @@ typedef struct cf_kernel_stats {
     double setup_ms;         /* host time from the call to the kernel launch */
     double host_ms;          /* host time of the whole call */
     int lean_layout;         /* 1 when the LEAN shared-memory layout was used */
+    double enqueue_ms;       /* host time queueing the call's copies and kernels */
+    double wait_ms;          /* host time waiting for the device after queueing */
 } cf_kernel_stats;
 int cf_last_kernel_stats(cf_kernel_stats* out);
 
