@@ -693,7 +693,7 @@ def dupireAADRisk(modelId, productId, notionals, num):
 
 class cf_server_opts(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("hbm_budget", ctypes.c_int64), ("auto_recompute", ctypes.c_int),
-                ("graphs", ctypes.c_int)]
+                ("graphs", ctypes.c_int), ("max_pack", ctypes.c_int)]
 
 
 class cf_request(ctypes.Structure):
@@ -708,14 +708,16 @@ class cf_result(ctypes.Structure):
         ("status", ctypes.c_int), ("error", ctypes.c_char * 256), ("n_pay", ctypes.c_int), ("n_param", ctypes.c_int),
         ("agg_mean", ctypes.c_double), ("submit_ms", ctypes.c_double), ("queue_ms", ctypes.c_double),
         ("latency_ms", ctypes.c_double), ("kernel_ms", ctypes.c_double), ("device_bytes", ctypes.c_int64),
-        ("recomputed", ctypes.c_int),
+        ("recomputed", ctypes.c_int), ("packed", ctypes.c_int), ("plan_ms", ctypes.c_double),
+        ("launch_ms", ctypes.c_double), ("device_ms", ctypes.c_double),
     ]
 
 
 class cf_server_stats(ctypes.Structure):
     _fields_ = [(f, ctypes.c_int64) for f in (
         "submitted", "completed", "failed", "queued", "running", "peak_running", "admitted_busy", "memory_waits",
-        "recompute_switches", "arena_bytes", "peak_arena_bytes", "graph_captures", "graph_replays")]
+        "recompute_switches", "arena_bytes", "peak_arena_bytes", "graph_captures", "graph_replays",
+        "packed_launches", "packed_requests")]
 
 
 def _server_lib():
@@ -741,11 +743,12 @@ class Server:
     once, device blocks from an arena of `hbmBudget` bytes; AAD requests that
     do not fit switch to the recompute variant when `autoRecompute`; with
     `graphs`, repeated requests replay a captured HIP graph of their launch
-    sequence."""
+    sequence; queued requests of one path kernel too small to fill the device
+    run up to `maxPack` to a launch (1: no packing)."""
 
-    def __init__(self, slots=4, hbmBudget=16 << 30, autoRecompute=True, graphs=True):
+    def __init__(self, slots=4, hbmBudget=16 << 30, autoRecompute=True, graphs=True, maxPack=8):
         L = _server_lib()
-        o = cf_server_opts(int(slots), int(hbmBudget), int(bool(autoRecompute)), int(bool(graphs)))
+        o = cf_server_opts(int(slots), int(hbmBudget), int(bool(autoRecompute)), int(bool(graphs)), int(maxPack))
         h = ctypes.c_void_p()
         _check(L.cf_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -785,7 +788,9 @@ class Server:
             raise CfError(rc, out.error.decode() or lib().cf_last_error().decode())
         return OrderedDict(means=means, std_errs=ses, agg_mean=out.agg_mean, risks=risks[:npar],
                            submit_ms=out.submit_ms, queue_ms=out.queue_ms, latency_ms=out.latency_ms,
-                           kernel_ms=out.kernel_ms, device_bytes=out.device_bytes, recomputed=bool(out.recomputed))
+                           kernel_ms=out.kernel_ms, device_bytes=out.device_bytes, recomputed=bool(out.recomputed),
+                           packed=out.packed, plan_ms=out.plan_ms, launch_ms=out.launch_ms,
+                           device_ms=out.device_ms)
 
     def stats(self):
         s = cf_server_stats()

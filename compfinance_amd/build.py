@@ -20,10 +20,18 @@ SOURCES = ([("cf_kern_inst.hip", "cf_kern_r%d_a%d_b%d" % (r, a, b),
             for r in (0, 1) for a in (0, 1, 2) for b in (0, 1)] +
            [("cf_kern_inst.hip", "cf_kern_r%d_a1_b2" % r, ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=1", "-DCF_INST_NBUF=2"])
             for r in (0, 1)] +
+           [("cf_kern_inst.hip", "cf_kern_r%d_a%d_b3" % (r, a),
+             ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_NBUF=3"])
+            for r in (0, 1) for a in (0, 1, 2)] +
+
            [(s, os.path.splitext(s)[0], []) for s in ("cf_kernels.hip", "cf_engine.cpp", "cf_store.cpp",
                                                       "cf_server.cpp", "cf_calib.cpp", "cf_noalloc.hip")])
 
-# Packed FP32 instructions are disabled for all device code.
+# Packed FP32 instructions are disabled for all device code.  Multiply-adds
+# are fused only within one source expression (-ffp-contract=on), so every
+# kernel instantiation rounds a shared function (the Euler step, the local-vol
+# interpolation, the payoffs) the same way: a call gives the same bits in the
+# checkpoint and recompute kernels, alone and packed.
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -31,6 +39,7 @@ HIP_FLAGS = [
     "-fPIC",
     "-Wall",
     "-Wno-unused-function",
+    "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 

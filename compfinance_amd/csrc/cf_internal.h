@@ -162,6 +162,29 @@ struct LaunchParams {
     void* state;                 // [blocks of 64 paths][state_rows(product)][64] reals
 };
 
+// Calls packed into one launch of a path kernel: call r runs on blocks
+// [first[r], first[r + 1]).  Passed by value (kernel arguments stay under
+// 4 KiB).
+constexpr int kMaxPack = 8;
+struct PackArgs {
+    LaunchParams p[kMaxPack];
+    int first[kMaxPack + 1];
+    int n;
+};
+static_assert(sizeof(PackArgs) <= 3584, "packed kernel arguments");
+
+// The cross-block reductions of the calls of a packed launch, one launch:
+// call r's column sums run on blocks [first[r], first[r + 1]).
+struct ReducePack {
+    const double* in0[kMaxPack];
+    double* out0[kMaxPack];
+    const double* in1[kMaxPack];
+    double* out1[kMaxPack];
+    int cols0[kMaxPack], cols1[kMaxPack], rows[kMaxPack];
+    int first[kMaxPack + 1];
+    int n;
+};
+
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
     size_t lo, hi, hi_tail, step_sample, invn, knots, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book, acc,
@@ -356,11 +379,13 @@ struct KernelConfig {
     bool recompute;  // adjoint by re-simulation instead of per-step checkpoints
     bool normals;    // normals read from a buffer (mrg32k3a) instead of Sobol
     bool item;       // checkpoint kernel with the itemized-risk modes (LaunchParams::ck_mode)
+    bool packed;     // path_kernel_packed (several calls, PackArgs)
 };
 
 // Path kernel families (precision r: 0 FP64, 1 FP32; adjoint a: AdjointKind;
-// normals b: 0 Sobol, 1 buffer), one per build of cf_kern_inst.hip: kernel
-// for a model, product and layout.
+// b: 0 Sobol, 1 normals buffer, 2 itemized-risk checkpoint kernels, 3 Sobol
+// kernels of packed launches), one per build of cf_kern_inst.hip: kernel for
+// a model, product and layout.
 #define CF_PICK_CAT(r, a, b) pick_r##r##_a##a##_b##b
 #define CF_PICK_NAME(r, a, b) CF_PICK_CAT(r, a, b)
 #define CF_PICK_DECL(r, a)                                                                                            \
@@ -375,6 +400,13 @@ CF_PICK_DECL(1, 2)
 #undef CF_PICK_DECL
 const void* pick_r0_a1_b2(int model, int prd, bool lean);
 const void* pick_r1_a1_b2(int model, int prd, bool lean);
+#define CF_PICK_DECL_PACKED(r)                                                                                       \
+    const void* pick_r##r##_a0_b3(int model, int prd, bool lean);                                                  \
+    const void* pick_r##r##_a1_b3(int model, int prd, bool lean);                                                  \
+    const void* pick_r##r##_a2_b3(int model, int prd, bool lean);
+CF_PICK_DECL_PACKED(0)
+CF_PICK_DECL_PACKED(1)
+#undef CF_PICK_DECL_PACKED
 
 // Returns 0 on success, else a hipError_t value.
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block,
@@ -383,6 +415,10 @@ int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid,
 // (cols1 may be 0), one launch.
 int launch_reduce_rows(const double* in0, int cols0, double* out0, const double* in1, int cols1, double* out1,
                        int rows, void* stream);
+// A packed launch (cfg.packed: Sobol, not itemized) of A.n calls over grid =
+// A.first[A.n] blocks, and the calls' reductions in one launch.
+int launch_path_kernel_packed(const KernelConfig& cfg, const PackArgs& A, int block, size_t smem, void* stream);
+int launch_reduce_packed(const ReducePack& R, void* stream);
 int path_kernel_occupancy(const KernelConfig& cfg, int block, size_t smem, int* blocks_per_cu);
 
 // Copies the direction numbers [32][kSobolDims] into the device's module

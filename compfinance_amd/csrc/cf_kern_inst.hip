@@ -1,9 +1,9 @@
 // One (precision, adjoint, normals source) family of path kernels,
 // instantiated for every model, product and LDS layout.  Built once per
 // family with -DCF_INST_REAL=0|1 (FP64, FP32), -DCF_INST_ADJ=0|1|2
-// (cfk::AdjointKind) and -DCF_INST_NBUF=0|1|2 (Sobol, normals buffer, Sobol
-// with the itemized-risk modes: checkpoint family only), so the families
-// compile in parallel.
+// (cfk::AdjointKind) and -DCF_INST_NBUF=0|1|2|3 (Sobol, normals buffer, Sobol
+// with the itemized-risk modes: checkpoint family only, Sobol kernels of
+// packed launches), so the families compile in parallel.
 #include <type_traits>
 
 #if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ) || !defined(CF_INST_NBUF)
@@ -28,15 +28,23 @@ namespace {
 using Real = std::conditional_t<CF_INST_REAL == 0, double, float>;
 constexpr int kAdj = CF_INST_ADJ;
 // CF_INST_NBUF: 0 Sobol, 1 normals buffer, 2 Sobol with the itemized-risk
-// modes (forward only keeping end states, reverse only from them)
+// modes (forward only keeping end states, reverse only from them), 3 Sobol
+// kernels of packed launches (path_kernel_packed: the same body; with
+// multiply-adds fused only within an expression it rounds as the single
+// kernel does)
 constexpr bool kNbuf = CF_INST_NBUF == 1;
 constexpr bool kItem = CF_INST_NBUF == 2;
+constexpr bool kPacked = CF_INST_NBUF == 3;
 
 template <int MODEL, int PRD>
 const void* ptr(bool lean)
 {
-    return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true, kNbuf, kItem>)
-                : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false, kNbuf, kItem>);
+    if constexpr (kPacked)
+        return lean ? reinterpret_cast<const void*>(&path_kernel_packed<Real, MODEL, PRD, kAdj, true>)
+                    : reinterpret_cast<const void*>(&path_kernel_packed<Real, MODEL, PRD, kAdj, false>);
+    else
+        return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true, kNbuf, kItem>)
+                    : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false, kNbuf, kItem>);
 }
 
 template <int MODEL>

@@ -98,17 +98,17 @@ int launch_mrg_normals(const uint64_t* jump, uint64_t seed1, uint64_t seed2, int
 // [0, ceil(cols0 / 64)) take the first, the rest the second.  A block takes 64
 // columns; wave w sums rows w, w + 16, ... in order and the 16 partial sums
 // are added in wave order, so the result does not depend on scheduling.
-__global__ void __launch_bounds__(1024) reduce_rows_kernel(const double* in0, int cols0, double* out0,
-                                                           const double* in1, int cols1, double* out1, int rows)
+__device__ __forceinline__ void reduce_rows_block(const double* in0, int cols0, double* out0, const double* in1,
+                                                  int cols1, double* out1, int rows, int bid)
 {
     __shared__ double part[16][kWave];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int b0 = (cols0 + kWave - 1) / kWave;
-    const bool first = int(blockIdx.x) < b0;
+    const bool first = bid < b0;
     const double* in = first ? in0 : in1;
     double* out = first ? out0 : out1;
     const int cols = first ? cols0 : cols1;
-    const int c = (first ? int(blockIdx.x) : int(blockIdx.x) - b0) * kWave + lane;
+    const int c = (first ? bid : bid - b0) * kWave + lane;
     double s = 0.0;
     if (c < cols)
         for (int r = w; r < rows; r += 16) s += in[size_t(r) * cols + c];
@@ -120,6 +120,23 @@ __global__ void __launch_bounds__(1024) reduce_rows_kernel(const double* in0, in
         for (int k = 0; k < 16; ++k) t += part[k][lane];
         out[c] = t;
     }
+}
+
+__global__ void __launch_bounds__(1024) reduce_rows_kernel(const double* in0, int cols0, double* out0,
+                                                           const double* in1, int cols1, double* out1, int rows)
+{
+    reduce_rows_block(in0, cols0, out0, in1, cols1, out1, rows, int(blockIdx.x));
+}
+
+// The reductions of a packed launch: blocks [first[r], first[r + 1]) sum call
+// r's rows exactly as reduce_rows_kernel sums them alone.
+__global__ void __launch_bounds__(1024) reduce_packed_kernel(const ReducePack R)
+{
+    const int b = int(blockIdx.x);
+    int r = 0;
+    while (r + 1 < R.n && b >= R.first[r + 1]) ++r;
+    r = __builtin_amdgcn_readfirstlane(r);
+    reduce_rows_block(R.in0[r], R.cols0[r], R.out0[r], R.in1[r], R.cols1[r], R.out1[r], R.rows[r], b - R.first[r]);
 }
 
 // ------------------------------------------------------------------ Sobol directions
@@ -227,19 +244,43 @@ static const void* pick(const KernelConfig& c)
         {{pick_r1_a0_b0, pick_r1_a0_b1}, {pick_r1_a1_b0, pick_r1_a1_b1}, {pick_r1_a2_b0, pick_r1_a2_b1}}};
     const int adj = !c.adjoint ? ADJ_NONE : c.recompute ? ADJ_RECOMPUTE : ADJ_CHECKPOINT;
     if (c.item) {
-        if (adj != ADJ_CHECKPOINT || c.normals) return nullptr;
+        if (adj != ADJ_CHECKPOINT || c.normals || c.packed) return nullptr;
         return (c.precision == 0 ? pick_r0_a1_b2 : pick_r1_a1_b2)(c.model, c.product, c.lean);
+    }
+    if (c.packed) {
+        if (c.normals) return nullptr;
+        static const Pick packed[2][3] = {{pick_r0_a0_b3, pick_r0_a1_b3, pick_r0_a2_b3},
+                                          {pick_r1_a0_b3, pick_r1_a1_b3, pick_r1_a2_b3}};
+        return packed[c.precision == 0 ? 0 : 1][adj](c.model, c.product, c.lean);
     }
     return table[c.precision == 0 ? 0 : 1][adj][c.normals ? 1 : 0](c.model, c.product, c.lean);
 }
 
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block, size_t smem, void* stream)
 {
+    if (cfg.packed) return int(hipErrorInvalidValue);
     const void* f = pick(cfg);
     if (!f) return int(hipErrorInvalidDeviceFunction);
     void* args[] = {const_cast<LaunchParams*>(&p)};
     hipError_t e = hipLaunchKernel(f, dim3(grid), dim3(block), args, smem, static_cast<hipStream_t>(stream));
     return int(e);
+}
+
+int launch_path_kernel_packed(const KernelConfig& cfg, const PackArgs& A, int block, size_t smem, void* stream)
+{
+    if (!cfg.packed || cfg.item || cfg.normals || A.n < 1 || A.n > kMaxPack) return int(hipErrorInvalidValue);
+    const void* f = pick(cfg);
+    if (!f) return int(hipErrorInvalidDeviceFunction);
+    void* args[] = {const_cast<PackArgs*>(&A)};
+    return int(hipLaunchKernel(f, dim3(A.first[A.n]), dim3(block), args, smem, static_cast<hipStream_t>(stream)));
+}
+
+int launch_reduce_packed(const ReducePack& R, void* stream)
+{
+    if (R.n < 1 || R.n > kMaxPack) return int(hipErrorInvalidValue);
+    if (R.first[R.n] == 0) return 0;
+    hipLaunchKernelGGL(reduce_packed_kernel, dim3(R.first[R.n]), dim3(1024), 0, static_cast<hipStream_t>(stream), R);
+    return int(hipGetLastError());
 }
 
 int launch_reduce_rows(const double* in0, int cols0, double* out0, const double* in1, int cols1, double* out1,

@@ -375,12 +375,10 @@ struct ProductState {
 
 // ------------------------------------------------------------------ kernel
 
+// The kernel of one call, run by block bid of the call's nblk blocks (the
+// whole grid, or the call's share of a packed launch).
 template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF, bool ITEM>
-#ifndef CF_PATH_KERNEL_ATTR
-#define CF_PATH_KERNEL_ATTR
-#endif
-__global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) CF_PATH_KERNEL_ATTR
-path_kernel(const LaunchParams P)
+__device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, const int nblk)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr bool DUP = MODEL == MODEL_DUPIRE;
@@ -474,7 +472,7 @@ path_kernel(const LaunchParams P)
     const bool merge = P.merge_tail != 0;
     const int64_t qe = merge ? q1 - 1 : q1;
     uint32_t* hi_tail = reinterpret_cast<uint32_t*>(smem + L.hi_tail);
-    const int slot = blockIdx.x * waves + wave;
+    const int slot = bid * waves + wave;
     // checkpoint slot of the wave: [n][64] states (Dupire) or normals
     // (Black-Scholes); recompute variant: [ceil(n / kSeg)][64] Dupire states
     const int ck_rows = ckpt_rows(n, DUP, RECOMP, int(sizeof(R)));
@@ -549,7 +547,7 @@ path_kernel(const LaunchParams P)
     //   v = -4        breached (survival 0, no derivative)
     constexpr float kTwoZeros = -2.0f, kBreached = -4.0f;
 
-    for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= qe; q += int64_t(gridDim.x) * waves) {
+    for (int64_t q = q0 + int64_t(bid) * waves + wave; q <= qe; q += int64_t(nblk) * waves) {
         // ---- high Sobol words of this block of 64 indices (and of the
         // merged last block); this lane's row, index and path
         const SobolLane sl = sobol_block(hi, hi_tail, P.dirs, n, q, q0, q1, nlo, nhi, merge, lane, !NBUF);
@@ -1254,10 +1252,10 @@ path_kernel(const LaunchParams P)
     for (int c = tid; c < stride; c += nthr) {
         double s = 0.0;
         for (int w = 0; w < waves; ++w) s += acc[w * stride + c];
-        P.block_out[size_t(blockIdx.x) * stride + c] = s;
+        P.block_out[size_t(bid) * stride + c] = s;
     }
     if constexpr (AD && DUP) {
-        double* dst = P.ivbar_blocks + size_t(blockIdx.x) * n * m;
+        double* dst = P.ivbar_blocks + size_t(bid) * n * m;
         // padded columns back onto the grid nodes: column j + 1 is node j,
         // the end columns 0 and m + 1 belong to the first and last node
         for (int e = tid; e < n * m; e += nthr) {
@@ -1272,6 +1270,31 @@ path_kernel(const LaunchParams P)
             dst[e] = v;
         }
     }
+}
+
+#ifndef CF_PATH_KERNEL_ATTR
+#define CF_PATH_KERNEL_ATTR
+#endif
+
+template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF, bool ITEM>
+__global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) CF_PATH_KERNEL_ATTR
+path_kernel(const LaunchParams P)
+{
+    path_body<R, MODEL, PRD, ADJ, LEAN, NBUF, ITEM>(P, int(blockIdx.x), int(gridDim.x));
+}
+
+// Several calls of one kernel in one launch: blocks [first[r], first[r + 1])
+// run call r (Sobol normals, no itemized modes).  The calls' parameters sit in
+// the kernel arguments, read with scalar loads like the single call's.
+template <typename R, int MODEL, int PRD, int ADJ, bool LEAN>
+__global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) CF_PATH_KERNEL_ATTR
+path_kernel_packed(const PackArgs A)
+{
+    const int b = int(blockIdx.x);
+    int r = 0;
+    while (r + 1 < A.n && b >= A.first[r + 1]) ++r;
+    r = __builtin_amdgcn_readfirstlane(r);
+    path_body<R, MODEL, PRD, ADJ, LEAN, false, false>(A.p[r], b - A.first[r], A.first[r + 1] - A.first[r]);
 }
 
 }  // namespace cfk

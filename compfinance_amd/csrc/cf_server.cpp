@@ -93,6 +93,11 @@ class Arena {
         }
         free_[o] = b;
     }
+    size_t size_of(size_t off) const
+    {
+        auto it = live_.find(off);
+        return it == live_.end() ? 0 : it->second;
+    }
     unsigned char* base() const { return static_cast<unsigned char*>(base_); }
     size_t capacity() const { return cap_; }
     size_t used() const { return used_; }
@@ -159,9 +164,11 @@ struct Request {
     int status = CF_OK;
     string error;
     bool done = false;
-    Clock::time_point t_submit, t_start, t_done;
-    float kernel_ms = 0.f;
-    size_t off = 0;
+    Clock::time_point t_submit, t_planned, t_start, t_launched, t_done;
+    float kernel_ms = 0.f, device_ms = 0.f;
+    size_t off = 0;       // arena block (alone)
+    size_t out_off = 0;   // first result in the slot's mapped memory (doubles)
+    int packed = 1;       // requests in its launch
 };
 
 struct Server;
@@ -180,9 +187,11 @@ constexpr size_t kSeenPerSlot = 32;
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;   // around the path kernel
+    hipEvent_t ev_a = nullptr, ev_b = nullptr; // around the whole sequence
     PinnedBuf pin_in, pin_out;
-    Request* req = nullptr;
+    vector<Request*> batch;   // the requests of its launch (empty: idle)
+    size_t off = 0;           // their arena block
     Clock::time_point t_end;
     Server* srv = nullptr;
     int index = 0;
@@ -200,6 +209,7 @@ struct Server {
     int device = 0;
     bool auto_recompute = true;
     bool graphs = true;
+    int max_pack = cfk::kMaxPack;
     Arena arena;
     vector<std::unique_ptr<Slot>> slots;
     std::mutex mu;
@@ -252,14 +262,21 @@ void enqueue_request(const Sequence& q, const Copies& c, hipStream_t st, hipEven
     enqueue_sequence(q, st, ev0, ev1);
 }
 
-// The slot's graph of (q, c): found by its bytes; else, on the second
-// sighting of the sequence, captured from the same enqueue as a direct
-// launch and instantiated (*captured set); else null: launch directly.
-hipGraphExec_t slot_graph(Slot& s, const Sequence& q, const Copies& c, bool* captured)
+// Appends the bytes of x to a graph key.
+template <typename T>
+void key_add(vector<unsigned char>& key, const T& x)
 {
-    vector<unsigned char> key(sizeof q + sizeof c);
-    std::memcpy(key.data(), &q, sizeof q);
-    std::memcpy(key.data() + sizeof q, &c, sizeof c);
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&x);
+    key.insert(key.end(), b, b + sizeof x);
+}
+
+// The slot's graph of the launch sequence named by `key` (the bytes of every
+// operation's parameters): found by its bytes; else, on the second sighting
+// of the key, captured from `enqueue` (the same calls as a direct launch) and
+// instantiated (*captured set); else null: launch directly.
+template <typename Enqueue>
+hipGraphExec_t slot_graph(Slot& s, vector<unsigned char> key, const Enqueue& enqueue, bool* captured)
+{
     ++s.tick;
     *captured = false;
     for (GraphEntry& g : s.graphs)
@@ -277,7 +294,7 @@ hipGraphExec_t slot_graph(Slot& s, const Sequence& q, const Copies& c, bool* cap
     hipGraph_t graph = nullptr;
     hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     try {
-        enqueue_request(q, c, s.stream, nullptr, nullptr);
+        enqueue();
     } catch (...) {
         if (hipStreamEndCapture(s.stream, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
         throw;
@@ -348,7 +365,14 @@ void launch(Server& S, Slot& s, Request& r)
     make_sequence(PR, SH, B, P, &q);
     ready_sequence(q);
     bool captured = false;
-    hipGraphExec_t exec = S.graphs ? slot_graph(s, q, c, &captured) : nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (S.graphs) {
+        vector<unsigned char> key;
+        key_add(key, q);
+        key_add(key, c);
+        exec = slot_graph(s, std::move(key), [&] { enqueue_request(q, c, s.stream, nullptr, nullptr); }, &captured);
+    }
+    hip_check(hipEventRecord(s.ev_a, s.stream), "hipEventRecord");
     if (exec) {
         {
             std::lock_guard<std::mutex> lk(S.mu);
@@ -360,15 +384,153 @@ void launch(Server& S, Slot& s, Request& r)
     } else {
         enqueue_request(q, c, s.stream, s.ev0, s.ev1);
     }
+    hip_check(hipEventRecord(s.ev_b, s.stream), "hipEventRecord");
     hip_check(hipLaunchHostFunc(s.stream, on_slot_done, &s), "hipLaunchHostFunc");
+    r.t_launched = Clock::now();
 }
 
-// Results of the request that ran in slot s (dispatcher thread, no lock).
+// A request that may share a launch: Sobol normals, its primary plan, and
+// too few blocks to fill the device alone.
+bool packable(const Server& S, const Request& r)
+{
+    return !r.use_rc && !r.shape.mrg && !r.plan->cfg.item && r.shape.grid < S.E->num_cus * r.plan->per_cu;
+}
+
+// Requests b can join a launch of a: the same path kernel and block size.
+bool pack_compatible(const Request& a, const Request& b)
+{
+    const cfk::KernelConfig &x = a.plan->cfg, &y = b.plan->cfg;
+    return x.precision == y.precision && x.model == y.model && x.product == y.product && x.adjoint == y.adjoint &&
+           x.lean == y.lean && x.recompute == y.recompute && a.plan->block == b.plan->block && a.aad == b.aad;
+}
+
+// Launches the requests of a pack in slot s from the slot's arena block, as
+// one packed path-kernel launch and one packed reduction (dispatcher thread,
+// no lock held).  Block layout: every request's tables and weights first
+// (one upload from the same layout in pinned memory), then each request's
+// call buffers.  Each request runs on its own blocks with the launch
+// parameters, grid and LDS layout it has alone, so its results are those of
+// its launch alone.
+void launch_pack(Server& S, Slot& s)
+{
+    const int n = int(s.batch.size());
+    unsigned char* base = S.arena.base() + s.off;
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t staged = 0, outs = 0;
+    for (Request* r : s.batch) {
+        staged += al(r->plan->staged.size()) + al(call_bytes(r->shape).weights);
+        outs += (size_t(r->shape.stride) + r->shape.ivn + 7) & ~size_t(7);
+    }
+    unsigned char* hin = static_cast<unsigned char*>(s.pin_in.ensure(staged));
+    s.pin_out.ensure(outs * 8);
+    double* hout_dev = static_cast<double*>(s.pin_out.dev);
+
+    cfk::PackArgs A;
+    cfk::ReducePack RP;
+    std::memset(&A, 0, sizeof A);   // padding too: keys compare byte for byte
+    std::memset(&RP, 0, sizeof RP);
+    A.n = RP.n = n;
+    size_t a_off = 0, b_off = staged, o_off = 0, smem = 0;
+    for (int i = 0; i < n; ++i) {
+        Request& r = *s.batch[size_t(i)];
+        const Prepared& PR = *r.plan;
+        const CallShape& SH = r.shape;
+        const CallBytes nb = call_bytes(SH);
+        CallBuffers B;
+        B.tables = base + a_off;
+        std::memcpy(hin + a_off, PR.staged.data(), PR.staged.size());
+        a_off += al(PR.staged.size());
+        if (nb.weights) {
+            B.weights = reinterpret_cast<double*>(base + a_off);
+            std::memcpy(hin + a_off, r.weights.data(), nb.weights);
+            a_off += al(nb.weights);
+        }
+        auto take = [&](size_t bytes) {
+            unsigned char* q = bytes ? base + b_off : nullptr;
+            b_off += al(bytes);
+            return q;
+        };
+        B.block_out = reinterpret_cast<double*>(take(nb.block_out));
+        (void)take(nb.reduced);
+        B.ivbar_blocks = reinterpret_cast<double*>(take(nb.ivbar_blocks));
+        (void)take(nb.ivbar);
+        B.ckpt = take(nb.ckpt);
+        B.normals = nullptr;
+        r.out_off = o_off;
+        B.reduced = hout_dev + o_off;
+        B.ivbar = SH.ivn ? B.reduced + SH.stride : nullptr;
+        o_off += (size_t(SH.stride) + SH.ivn + 7) & ~size_t(7);
+        A.p[i] = launch_params(*S.E, PR, SH, B);
+        A.first[i + 1] = A.first[i] + SH.grid;
+        smem = std::max(smem, PR.smem);
+        RP.in0[i] = B.block_out;
+        RP.cols0[i] = SH.stride;
+        RP.out0[i] = B.reduced;
+        RP.in1[i] = B.ivbar_blocks;
+        RP.cols1[i] = int(SH.ivn);
+        RP.out1[i] = B.ivbar;
+        RP.rows[i] = SH.grid;
+        RP.first[i + 1] = RP.first[i] + (SH.stride + cfk::kWave - 1) / cfk::kWave +
+                          (int(SH.ivn) + cfk::kWave - 1) / cfk::kWave;
+        r.packed = n;
+    }
+    if (b_off > S.arena.size_of(s.off)) fail(CF_ERR_CAPACITY, "packed block exceeds its arena allocation");
+    cfk::KernelConfig cfg = s.batch[0]->plan->cfg;
+    cfg.packed = true;
+    const int block = s.batch[0]->plan->block;
+    int per_cu = 0;
+    int rc = cfk::path_kernel_occupancy(cfg, block, smem, &per_cu);   // sets the LDS limit once per size
+    if (rc != 0) hip_check(hipError_t(rc), "occupancy");
+    if (per_cu < 1) fail(CF_ERR_CAPACITY, "packed kernel does not fit on a compute unit");
+
+    auto enqueue = [&](hipEvent_t e0, hipEvent_t e1) {
+        hip_check(hipMemcpyAsync(base, hin, staged, hipMemcpyHostToDevice, s.stream), "upload");
+        if (e0) hip_check(hipEventRecord(e0, s.stream), "hipEventRecord");
+        int e = cfk::launch_path_kernel_packed(cfg, A, block, smem, s.stream);
+        if (e != 0) hip_check(hipError_t(e), "packed path kernel launch");
+        if (e1) hip_check(hipEventRecord(e1, s.stream), "hipEventRecord");
+        e = cfk::launch_reduce_packed(RP, s.stream);
+        if (e != 0) hip_check(hipError_t(e), "packed reduce launch");
+    };
+    bool captured = false;
+    hipGraphExec_t exec = nullptr;
+    if (S.graphs) {
+        vector<unsigned char> key;
+        key_add(key, A);
+        key_add(key, RP);
+        key_add(key, cfg);
+        key_add(key, block);
+        key_add(key, smem);
+        key_add(key, staged);
+        key_add(key, hin);
+        exec = slot_graph(s, std::move(key), [&] { enqueue(nullptr, nullptr); }, &captured);
+    }
+    {
+        std::lock_guard<std::mutex> lk(S.mu);
+        ++S.st.packed_launches;
+        S.st.packed_requests += n;
+        if (exec) ++(captured ? S.st.graph_captures : S.st.graph_replays);
+    }
+    hip_check(hipEventRecord(s.ev_a, s.stream), "hipEventRecord");
+    if (exec) {
+        hip_check(hipEventRecord(s.ev0, s.stream), "hipEventRecord");
+        hip_check(hipGraphLaunch(exec, s.stream), "hipGraphLaunch");
+        hip_check(hipEventRecord(s.ev1, s.stream), "hipEventRecord");
+    } else {
+        enqueue(s.ev0, s.ev1);
+    }
+    hip_check(hipEventRecord(s.ev_b, s.stream), "hipEventRecord");
+    hip_check(hipLaunchHostFunc(s.stream, on_slot_done, &s), "hipLaunchHostFunc");
+    const Clock::time_point t = Clock::now();
+    for (Request* r : s.batch) r->t_launched = t;
+}
+
+// Results of request r that ran in slot s (dispatcher thread, no lock).
 void finish(Server& S, Slot& s, Request& r)
 {
     const Prepared& PR = r.use_rc ? *r.plan_rc : *r.plan;
     const CallShape& SH = r.use_rc ? r.shape_rc : r.shape;
-    const double* hout = static_cast<const double*>(s.pin_out.p);
+    const double* hout = static_cast<const double*>(s.pin_out.p) + r.out_off;
     r.means.assign(size_t(SH.npay), 0.0);
     r.std_errs.assign(size_t(SH.npay), 0.0);
     Call c;
@@ -383,6 +545,7 @@ void finish(Server& S, Slot& s, Request& r)
     collect_results(PR, SH, hout, hout + SH.stride, c);
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, s.ev0, s.ev1) == hipSuccess) r.kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, s.ev_a, s.ev_b) == hipSuccess) r.device_ms = ms;
     (void)S;
 }
 
@@ -397,10 +560,11 @@ void dispatcher(Server* Sp)
             const int si = S.finished.back();
             S.finished.pop_back();
             Slot& s = *S.slots[si];
-            Request* r = s.req;
+            const vector<Request*> batch = s.batch;
             const Clock::time_point t_end = s.t_end;
             lk.unlock();
-            if (r->status == CF_OK) {
+            for (Request* r : batch) {
+                if (r->status != CF_OK) continue;
                 try {
                     finish(S, s, *r);
                 } catch (const CfError& e) {
@@ -412,21 +576,22 @@ void dispatcher(Server* Sp)
                 }
             }
             lk.lock();
-            const int code = r->status;
-            r->t_done = t_end;
-            r->done = true;
-            S.arena.release(r->off);
-            s.req = nullptr;
-            --S.running;
-            ++S.st.completed;
-            if (code != CF_OK) ++S.st.failed;
+            for (Request* r : batch) {
+                r->t_done = t_end;
+                r->done = true;
+                --S.running;
+                ++S.st.completed;
+                if (r->status != CF_OK) ++S.st.failed;
+            }
+            S.arena.release(s.off);
+            s.batch.clear();
             S.cv_done.notify_all();
         }
         // ---- admissions: free slots, oldest requests first; a request that
         // does not fit is passed over at most kMaxWaits times
-        vector<std::pair<Slot*, Request*>> start;
+        vector<Slot*> start;
         for (auto& sp : S.slots) {
-            if (sp->req || S.queue.empty()) continue;
+            if (!sp->batch.empty() || S.queue.empty()) continue;
             Request* pick = nullptr;
             for (auto it = S.queue.begin(); it != S.queue.end(); ++it) {
                 Request* r = *it;
@@ -447,30 +612,70 @@ void dispatcher(Server* Sp)
                 if (++r->waits > kMaxWaits) break;
             }
             if (!pick) break;
-            if (S.running > 0) ++S.st.admitted_busy;
-            ++S.running;
+            vector<Request*> batch{pick};
+            size_t off = pick->off;
+            // queued requests of the same kernel that fit beside it on the
+            // device join its launch, in one arena block
+            if (S.max_pack > 1 && packable(S, *pick)) {
+                int cap = S.E->num_cus * pick->plan->per_cu, grid = pick->shape.grid;
+                size_t bytes = pick->bytes;
+                for (Request* r : S.queue) {
+                    if (int(batch.size()) >= S.max_pack) break;
+                    if (!packable(S, *r) || !pack_compatible(*pick, *r)) continue;
+                    const int cap_r = std::min(cap, S.E->num_cus * r->plan->per_cu);
+                    if (grid + r->shape.grid > cap_r) continue;
+                    cap = cap_r;
+                    grid += r->shape.grid;
+                    bytes += r->bytes;
+                    batch.push_back(r);
+                }
+                if (batch.size() > 1) {
+                    S.arena.release(pick->off);
+                    if (S.arena.alloc(bytes, &off)) {
+                        for (size_t i = 1; i < batch.size(); ++i)
+                            S.queue.erase(std::find(S.queue.begin(), S.queue.end(), batch[i]));
+                    } else {
+                        // no room for the pack: the first request alone, in the
+                        // room its own block had a moment ago
+                        if (!S.arena.alloc(pick->bytes, &pick->off)) {
+                            S.queue.push_front(pick);
+                            break;
+                        }
+                        off = pick->off;
+                        batch.resize(1);
+                    }
+                }
+            }
+            for (Request* r : batch) {
+                if (S.running > 0) ++S.st.admitted_busy;
+                ++S.running;
+                r->t_start = Clock::now();
+            }
             S.st.peak_running = std::max<int64_t>(S.st.peak_running, S.running);
             S.st.peak_arena_bytes = std::max<int64_t>(S.st.peak_arena_bytes, int64_t(S.arena.used()));
-            sp->req = pick;
-            pick->t_start = Clock::now();
-            start.push_back({sp.get(), pick});
+            sp->batch = std::move(batch);
+            sp->off = off;
+            start.push_back(sp.get());
         }
         if (!start.empty()) {
             lk.unlock();
-            for (auto& sr : start) {
+            for (Slot* sl : start) {
                 try {
-                    launch(S, *sr.first, *sr.second);
+                    if (sl->batch.size() == 1) launch(S, *sl, *sl->batch[0]);
+                    else launch_pack(S, *sl);
                 } catch (const std::exception& e) {
-                    // nothing of this request is queued after the failing call
-                    // (the completion signal is queued last); complete it
-                    // through the same path with the error
-                    (void)hipStreamSynchronize(sr.first->stream);
+                    // nothing of these requests is queued after the failing
+                    // call (the completion signal is queued last); complete
+                    // them through the same path with the error
+                    (void)hipStreamSynchronize(sl->stream);
                     std::lock_guard<std::mutex> g(S.mu);
                     const CfError* ce = dynamic_cast<const CfError*>(&e);
-                    sr.second->status = ce ? ce->code : CF_ERR_DEVICE;
-                    sr.second->error = e.what();
-                    sr.first->t_end = Clock::now();
-                    S.finished.push_back(sr.first->index);
+                    for (Request* r : sl->batch) {
+                        r->status = ce ? ce->code : CF_ERR_DEVICE;
+                        r->error = e.what();
+                    }
+                    sl->t_end = Clock::now();
+                    S.finished.push_back(sl->index);
                 }
             }
             lk.lock();
@@ -513,6 +718,9 @@ int cf_server_create(const cf_server_opts* opts, cf_server** out)
         S->device = S->E->device;
         S->auto_recompute = opts ? opts->auto_recompute != 0 : true;
         S->graphs = opts ? opts->graphs != 0 : true;
+        const int mp = opts ? opts->max_pack : 0;
+        if (mp < 0 || mp > cfk::kMaxPack) fail(CF_ERR_ARG, "max_pack must be in 0..8");
+        S->max_pack = mp == 0 ? cfk::kMaxPack : mp;
         hip_check(hipSetDevice(S->device), "hipSetDevice");
         S->arena.init(size_t(budget));
         for (int i = 0; i < n_slots; ++i) {
@@ -520,6 +728,8 @@ int cf_server_create(const cf_server_opts* opts, cf_server** out)
             hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
             hip_check(hipEventCreate(&s->ev0), "hipEventCreate");
             hip_check(hipEventCreate(&s->ev1), "hipEventCreate");
+            hip_check(hipEventCreate(&s->ev_a), "hipEventCreate");
+            hip_check(hipEventCreate(&s->ev_b), "hipEventCreate");
             s->srv = S.get();
             s->index = i;
             s->pin_in.ensure(size_t(1) << 20);   // staging sized for the common requests up front
@@ -577,6 +787,7 @@ int cf_server_submit(cf_server* srv, const cf_request* req, int64_t* id)
         if (least > S.arena.capacity())
             fail(CF_ERR_CAPACITY, "request needs " + std::to_string(least) + " device bytes, server holds " +
                                       std::to_string(S.arena.capacity()));
+        r->t_planned = Clock::now();
         {
             std::lock_guard<std::mutex> lk(S.mu);
             if (S.stop) fail(CF_ERR_ARG, "server is shutting down");
@@ -631,6 +842,10 @@ int cf_server_wait(cf_server* srv, int64_t id, cf_result* out)
         out->submit_ms = ms_between(S.t0, r->t_submit);
         out->device_bytes = int64_t(r->use_rc ? r->bytes_rc : r->bytes);
         out->recomputed = r->use_rc ? 1 : 0;
+        out->packed = r->packed;
+        out->plan_ms = ms_between(r->t_submit, r->t_planned);
+        out->launch_ms = r->t_launched > r->t_start ? ms_between(r->t_start, r->t_launched) : 0.0;
+        out->device_ms = r->device_ms;
         if (r->status != CF_OK) {
             g_err = r->error;
             return r->status;
@@ -670,6 +885,8 @@ int cf_server_destroy(cf_server* srv)
             s->graphs.clear();
             (void)hipEventDestroy(s->ev0);
             (void)hipEventDestroy(s->ev1);
+            (void)hipEventDestroy(s->ev_a);
+            (void)hipEventDestroy(s->ev_b);
             (void)hipStreamDestroy(s->stream);
             if (s->pin_in.p) (void)hipHostFree(s->pin_in.p);
             if (s->pin_out.p) (void)hipHostFree(s->pin_out.p);

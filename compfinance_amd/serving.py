@@ -117,21 +117,31 @@ def summarize(reqs, results, wall, stats):
         "recomputed_requests": int(sum(r["recomputed"] for r in results)),
         "graph_captures": int(stats["graph_captures"]),
         "graph_replays": int(stats["graph_replays"]),
+        "packed_launches": int(stats["packed_launches"]),
+        "packed_requests": int(stats["packed_requests"]),
     }
 
 
-def run_c5(n=512, rate=200.0, slots=4, budget=4 << 30, seed=20261204, warmup=16):
+def run_c5(n=512, rate=200.0, slots=4, budget=4 << 30, seed=20261204, warmup=16, max_pack=8):
+    """The C5 replay: n requests of the seeded mix at `rate` per second."""
     reqs = c5_requests(n, rate, seed)
-    with cf.Server(slots=slots, hbmBudget=budget) as srv:
+    with cf.Server(slots=slots, hbmBudget=budget, maxPack=max_pack) as srv:
         # warm-up: kernel loading and first-touch of the arena, not timed
         for r in reqs[:warmup]:
             srv.wait(srv.submit(r["product"], r["model"], r["num"], aad=r["aad"]))
         base = srv.stats()
         results, wall = replay(reqs, srv)
         st = srv.stats()
-        for k in ("admitted_busy", "memory_waits", "recompute_switches", "graph_captures", "graph_replays"):
+        for k in ("admitted_busy", "memory_waits", "recompute_switches", "graph_captures", "graph_replays",
+                  "packed_launches", "packed_requests"):
             st[k] -= base[k]
     return summarize(reqs, results, wall, st)
+
+
+def run_burst(n=512, slots=4, budget=4 << 30, seed=20261204, max_pack=8):
+    """The same mix submitted at once (saturation): throughput, latencies and
+    how many requests shared packed launches."""
+    return run_c5(n, 1e9, slots, budget, seed, max_pack=max_pack)
 
 
 def main():
@@ -140,8 +150,9 @@ def main():
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gib", type=float, default=4.0)
+    ap.add_argument("--max-pack", type=int, default=8)
     a = ap.parse_args()
-    print(json.dumps(run_c5(a.requests, a.rate, a.slots, int(a.budget_gib * (1 << 30)))))
+    print(json.dumps(run_c5(a.requests, a.rate, a.slots, int(a.budget_gib * (1 << 30)), max_pack=a.max_pack)))
 
 
 if __name__ == "__main__":

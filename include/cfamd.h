@@ -292,6 +292,10 @@ typedef struct cf_server_opts {
                               * addresses and sizes) captures it in a HIP graph
                               * and replays the graph for later repeats;
                               * 0: direct launches only */
+    int max_pack;            /* queued requests of one path kernel that a slot
+                              * runs in one packed launch, 1..8 (0: 8; 1: no
+                              * packing).  Only requests too small to fill the
+                              * device alone are packed. */
 } cf_server_opts;
 
 typedef struct cf_request {
@@ -321,6 +325,10 @@ typedef struct cf_result {
                               * request's whole device sequence */
     int64_t device_bytes;    /* arena bytes held while on the device */
     int recomputed;          /* 1 if admitted as the recompute variant */
+    int packed;              /* requests in the launch that ran it (1: alone) */
+    double plan_ms;          /* submission: validation and planning (part of queue_ms) */
+    double launch_ms;        /* host time queueing its copies and kernels */
+    double device_ms;        /* its launch's whole device sequence (HIP events) */
 } cf_result;
 
 typedef struct cf_server_stats {
@@ -334,10 +342,12 @@ typedef struct cf_server_stats {
     int64_t graph_captures;       /* launch sequences captured and instantiated
                                    * (and launched) */
     int64_t graph_replays;        /* launches that replayed a captured graph */
+    int64_t packed_launches;      /* launches that ran two or more requests */
+    int64_t packed_requests;      /* requests run in those launches */
 } cf_server_stats;
 
-/* opts NULL: 4 slots, 16 GiB, auto_recompute on, graphs on.  Uses the
- * current device. */
+/* opts NULL: 4 slots, 16 GiB, auto_recompute on, graphs on, packs of up to 8.
+ * Uses the current device. */
 int cf_server_create(const cf_server_opts* opts, cf_server** out);
 /* Validates and plans the request on the calling thread, then queues it. */
 int cf_server_submit(cf_server* srv, const cf_request* req, int64_t* id);

@@ -58,6 +58,62 @@ def test_full_c5_stream_in_the_mix_equals_alone(engine):
             np.testing.assert_allclose(got["risks"], risks, rtol=1e-12, atol=1e-12 * np.abs(risks).max())
 
 
+def _small_mix():
+    """Small requests of two kernels: FP32 Dupire barrier AAD on grids of
+    20..40 spots (different LDS sizes) and Black-Scholes European prices."""
+    rng = np.random.default_rng(5)
+    out = []
+    for k in range(16):
+        if k % 2 == 0:
+            ns = 20 + 2 * k
+            spots = 50.0 + 150.0 * np.arange(ns) / (ns - 1)
+            times = (np.arange(24) + 1) / 12.0
+            vols = 0.15 + 0.10 * np.log(spots / 100.0)[:, None] ** 2 + 0.05 * np.exp(-times)[None, :]
+            mdl = cf.Dupire(100.0, spots, times, vols, 1 / 52)
+            prd = cf.UOC(float(rng.uniform(90, 120)), float(rng.uniform(140, 180)), 1.5, 1 / 52, 0.05)
+            out.append((prd, mdl, cf.NumericalParam(numPath=int(2 ** rng.uniform(12, 15)), precision=cf.FP32), True))
+        else:
+            mdl = cf.BlackScholes(100.0, float(rng.uniform(0.1, 0.4)))
+            prd = cf.European(float(rng.uniform(80, 130)), float(rng.uniform(0.5, 3.0)))
+            out.append((prd, mdl, cf.NumericalParam(numPath=int(2 ** rng.uniform(12, 15)), precision=cf.FP32), False))
+    return out
+
+
+def test_small_requests_share_packed_launches(engine):
+    """Sixteen small requests of two kernels queued behind a large one in a
+    one-slot server run in packed launches (one path-kernel and one
+    reduction launch for several requests of a kernel); each request's
+    prices equal its synchronous run bit for bit, its risks to the rounding
+    of the LDS-atomic sums.  With maxPack=1 nothing is packed."""
+    mix = _small_mix()
+    big = (cf.UOC(**W.barrier_args()), cf.Dupire(**W.dupire_args(30)),   # fills the device for ~15 ms
+           cf.NumericalParam(numPath=1 << 23, precision=cf.FP64, aadMode=cf.AAD_RECOMPUTE), True)
+    # the synchronous runs first: they also load the kernels, whose first use
+    # can hold a submission until the device is idle
+    refs = [cf.mcParallelSimulAAD(p, m, num=n) if a else cf.mcParallelSimul(p, m, num=n, per_path=False)
+            for p, m, n, a in mix]
+    for max_pack in (8, 1):
+        with cf.Server(slots=1, hbmBudget=4 << 30, maxPack=max_pack) as srv:
+            first = srv.submit(big[0], big[1], big[2], aad=big[3])
+            ids = [srv.submit(p, m, n, aad=a) for p, m, n, a in mix]
+            srv.wait(first)
+            res = [srv.wait(i) for i in ids]
+            st = srv.stats()
+        assert st["completed"] == 17 and st["failed"] == 0
+        if max_pack == 8:
+            assert st["packed_launches"] >= 1 and st["packed_requests"] >= 2, (st, [g["queue_ms"] for g in res])
+            assert max(g["packed"] for g in res) >= 2
+        else:
+            assert st["packed_launches"] == 0 and all(g["packed"] == 1 for g in res), st
+        for (p, m, n, a), ref, got in zip(mix, refs, res):
+            if a:
+                np.testing.assert_array_equal(got["means"], ref.payoff_means)
+                assert abs(got["agg_mean"] - ref.agg_mean) <= 1e-12 * max(abs(ref.agg_mean), 1e-300)
+                np.testing.assert_allclose(got["risks"], ref.risks, rtol=1e-12, atol=1e-12 * np.abs(ref.risks).max())
+            else:
+                np.testing.assert_array_equal(got["means"], ref.means)
+
+
 def test_admission_waits_for_memory_and_switches_to_recompute(engine):
     """A budget of one checkpoint block plus one recompute block: the second
     AAD request runs the recompute variant, the next ones wait for memory;
