@@ -892,6 +892,9 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
             P->o_knots = st.add(kx);
             P->o_invdx = st.add(ih);
             P->o_iv = st.add(v);
+            // every node at least 1e-30: an interpolated vol is never 0, so
+            // the reverse sweep never draws a normal again
+            P->iv_pos = std::all_of(v.begin(), v.end(), [](T y) { return std::isfinite(double(y)) && y >= T(1e-30); });
         };
         if (fp64) {
             tables(double(), md.log_spots);
@@ -1086,6 +1089,7 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     P.per_path_agg = B.per_path_agg;
     P.normals = S.mrg ? B.normals : nullptr;
     P.ivbar_copies = PR.ivbar_copies;
+    P.iv_pos = PR.iv_pos ? 1 : 0;
     return P;
 }
 

@@ -199,6 +199,7 @@ struct Prepared {
     int per_cu = 0;
     int block = 0;
     int ivbar_copies = 1;   // LDS local-vol adjoint grids per block
+    bool iv_pos = false;    // Dupire: every local-vol table entry positive (at least 1e-30)
     BookTables book;
     vector<unsigned char> staged;   // the tables as uploaded
     DevBuf tables;                  // device copy for the synchronous calls

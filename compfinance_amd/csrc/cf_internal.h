@@ -67,11 +67,18 @@ struct BookLayout {
 // state's bracket (grids of up to kMaxStoredKnots knots).
 // The Dupire checkpoint slot starts with kCkptGuardRows rows that the reverse
 // sweep's prefetch may read below step 0 (values never used).
+// FP32 Dupire checkpoints are interleaved by groups of four steps: the state
+// of step i of lane l is element ((i >> 2) * 64 + l) * 4 + (i & 3) (a lane's
+// four states in one 16-byte word), its bracket byte the same element of the
+// byte rows that follow the states; the steps are padded to a multiple of four.
+// FP64 keeps one row of 64 per step.
 constexpr int kMaxStoredKnots = 255;
 constexpr int kCkptGuardRows = 4;
+CF_HD inline int ckpt_steps(int n_steps, int real_bytes) { return real_bytes == 4 ? (n_steps + 3) & ~3 : n_steps; }
 CF_HD inline int ckpt_rows(int n_steps, bool dupire, bool recompute, int real_bytes)
 {
-    if (!recompute) return dupire ? kCkptGuardRows + n_steps + (n_steps + real_bytes - 1) / real_bytes : n_steps;
+    const int ns = ckpt_steps(n_steps, real_bytes);
+    if (!recompute) return dupire ? kCkptGuardRows + ns + (ns + real_bytes - 1) / real_bytes : n_steps;
     if (!dupire) return 0;
     const int seg = real_bytes == 4 ? kSegF32 : kSegF64;
     return (n_steps + seg - 1) / seg;
@@ -152,6 +159,7 @@ struct LaunchParams {
     void* ckpt;                  // [slots][ckpt_rows][64] path states or normals,
                                  // then (Dupire) [n_steps][64] bracket bytes
     int ivbar_copies;            // LDS local-vol adjoint grids per block: even and odd lanes (2) or one
+    int iv_pos;                  // 1: every local-vol table entry is at least 1e-30 (no zero vol)
     double* ivbar_blocks;        // [gridDim][n_steps][n_spots] local-vol adjoints
     double* per_path_agg;        // [n_path] or null
     // checkpoint adjoint shared across payoffs (itemized risk): 0 one call

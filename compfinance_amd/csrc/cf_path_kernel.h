@@ -486,7 +486,13 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     R* ck = AD ? static_cast<R*>(P.ckpt) + (size_t(slot) * ck_rows + (CKB ? kCkptGuardRows : 0)) * kWave : nullptr;
     // (the FULL layout is only chosen for grids of at most kMaxStoredKnots knots)
     const bool ck_lo = CKB && (!LEAN || m <= kMaxStoredKnots);
-    uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave) : nullptr;
+    // FP32: states and bracket bytes interleaved by groups of four steps (a
+    // lane's four in one 16-byte / 4-byte word); FP64: one row per step
+    constexpr bool IL = CKB && sizeof(R) == 4;
+    const int nck = ckpt_steps(n, int(sizeof(R)));
+    uint8_t* ckl = CKB ? reinterpret_cast<uint8_t*>(ck + size_t(nck) * kWave) : nullptr;
+    // element of step i of this lane in the states (and bracket bytes)
+    auto ck_el = [&](int i) -> int { return IL ? ((i >> 2) << 8) + (lane << 2) + (i & 3) : i * kWave + lane; };
     constexpr int kStateRows = BOOK ? 8 + 2 * kMaxBookBarriers + 2 * kMaxBookGroups : 12;
     static_assert(kStateRows == (BOOK ? 8 + 2 * kMaxBookBarriers + 2 * kMaxBookGroups : 8 + 2 + 2),
                   "state rows as state_rows() counts them");
@@ -538,6 +544,13 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     };
     auto bar_par = [&](int b) { return reinterpret_cast<const BarPar*>(bk_par)[b]; };
     const R bk_lo = R(P.book_lo);   // the lowest B - w (rounding keeps it the lowest)
+    // Barrier smoothing factors (B + w - F) / 2w: FP64 tracks exactly-zero
+    // factors (F = B + w) as the tape does.  FP32 keeps B + w - F at least
+    // tiny_d instead, so a factor is never 0 and the product of the others,
+    // the tape's derivative at a zero factor, stays the product over the
+    // factor; the price moves by at most 1e-30 of the payoff there.
+    constexpr bool ZT = sizeof(R) == 8;
+    const R tiny_d = ZT ? R(0) : max(R(P.two_smooth) * R(1e-30), R(1e-37));
     constexpr int NBAR = BOOK ? kMaxBookBarriers : 1, NGRP = BOOK ? kMaxBookGroups : 1;
     // Book barrier state in one real per barrier:
     //   v in [0, 1]   no exactly-zero smoothing factor yet, v = product of the
@@ -559,7 +572,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
         if (ck_mode != 0) {
             const size_t sb = size_t(q - q0);
             ck = static_cast<R*>(P.ckpt) + (sb * ck_rows + (CKB ? kCkptGuardRows : 0)) * kWave;
-            if constexpr (CKB) ckl = reinterpret_cast<uint8_t*>(ck + size_t(n) * kWave);
+            if constexpr (CKB) ckl = reinterpret_cast<uint8_t*>(ck + size_t(nck) * kWave);
             stq = static_cast<R*>(P.state) + sb * kStateRows * kWave + lane;
         }
         // normal of step i: from the Sobol point, or from the buffer of
@@ -614,21 +627,26 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 const bool live = !ps.dead;
                 const bool brk = live && F > R(P.bar_smooth);
                 const bool zone = live && !brk && F > R(P.minus_smooth);
-                const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
                 const R a_prev = ps.alive;
                 ps.dead = ps.dead || brk;
-                if (zone) ps.alive = a_prev * f;
-                // exactly-zero factors (F = B + w), rare: from the first one on,
-                // pnz holds the product of the non-zero factors
-                const bool fz = zone && f == R(0);
-                nzw = nzw | (__builtin_amdgcn_ballot_w64(fz) != 0);
-                if (nzw) {
-                    if (fz) {
-                        if (ps.nzero == 0) ps.pnz = a_prev;
-                        ps.nzero++;
-                    } else if (zone && ps.nzero > 0) {
-                        ps.pnz = ps.pnz * f;
+                if constexpr (ZT) {
+                    const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
+                    if (zone) ps.alive = a_prev * f;
+                    // exactly-zero factors (F = B + w), rare: from the first one on,
+                    // pnz holds the product of the non-zero factors
+                    const bool fz = zone && f == R(0);
+                    nzw = nzw | (__builtin_amdgcn_ballot_w64(fz) != 0);
+                    if (nzw) {
+                        if (fz) {
+                            if (ps.nzero == 0) ps.pnz = a_prev;
+                            ps.nzero++;
+                        } else if (zone && ps.nzero > 0) {
+                            ps.pnz = ps.pnz * f;
+                        }
                     }
+                } else {
+                    const R f = max(R(P.bar_smooth) - F, tiny_d) * R(P.inv_two_smooth);
+                    if (zone) ps.alive = a_prev * f;
                 }
                 // the vanilla of the last sample: after the path (the last
                 // sample is the last timeline point)
@@ -777,20 +795,27 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
             }
             // step i0 + k, with every per-step address a fixed offset (k) from a
-            // base formed once per group of steps at i0
-            auto fwd_step = [&](int i0, int k) {
+            // base formed once per group of steps at i0.  GRP: a step of an
+            // aligned group of four whose state and bracket the caller stores
+            // at once (interleaved FP32 checkpoints), else stored here.
+            R gx[4];
+            uint32_t glo = 0;
+            auto fwd_step = [&](int i0, int k, auto grp) {
+                constexpr bool GRP = decltype(grp)::value;
                 const int i = i0 + k;
                 R g;
                 if constexpr (NBUF || LEAN) g = draw(i);
                 else g = inv_normal(hl[i0 + k] ^ (lo_tab + i0 * kWave + lane)[k * kWave], invn_tab);
                 if constexpr (DUP) {
-                    R* ckg = ck + i0 * kWave + lane;
-                    if constexpr (ADJ == ADJ_CHECKPOINT) ckg[k * kWave] = x;
+                    if constexpr (GRP) gx[k] = x;
+                    else if constexpr (IL) ck[ck_el(i)] = x;
+                    else if constexpr (ADJ == ADJ_CHECKPOINT) (ck + i0 * kWave + lane)[k * kWave] = x;
                     if constexpr (RECOMP)
                         if ((i & (kSeg - 1)) == 0) ck[(i / kSeg) * kWave + lane] = x;
                     const int lo = count_le<LEAN>(ks, x);
-                    if constexpr (CKB)
-                        if (ck_lo) (ckl + i0 * kWave + lane)[k * kWave] = uint8_t(lo);
+                    if constexpr (GRP) glo |= uint32_t(lo) << (8 * k);
+                    else if constexpr (IL) { if (ck_lo) ckl[ck_el(i)] = uint8_t(lo); }
+                    else if constexpr (CKB) { if (ck_lo) (ckl + i0 * kWave + lane)[k * kWave] = uint8_t(lo); }
                     const Bracket<R> b = local_vol_at(kxi, (iv + i0 * mp) + k * mp, x, lo);
                     x = euler_step(x, b.vol, g);
                 } else {
@@ -808,12 +833,20 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
             constexpr int UF = AD && DUP && !BOOK && sizeof(R) == 4 ? 4 : 1;
             int i = 0;
             if constexpr (UF > 1) {
+                // (UF = 4 kernels with interleaved checkpoints: one 16-byte
+                // store of the four states and one word of brackets per group)
+                constexpr bool GRP = IL && UF == 4;
                 for (; i + UF <= n; i += UF) {
+                    glo = 0;
     #pragma unroll
-                    for (int k = 0; k < UF; ++k) fwd_step(i, k);
+                    for (int k = 0; k < UF; ++k) fwd_step(i, k, std::integral_constant<bool, GRP>{});
+                    if constexpr (GRP) {
+                        *reinterpret_cast<float4*>(ck + ck_el(i)) = make_float4(gx[0], gx[1], gx[2], gx[3]);
+                        if (ck_lo) *reinterpret_cast<uint32_t*>(ckl + ck_el(i)) = glo;
+                    }
                 }
             }
-            for (; i < n; ++i) fwd_step(i, 0);
+            for (; i < n; ++i) fwd_step(i, 0, std::false_type{});
 
             // ---- payoffs of the barrier (known at the end of the path)
             if constexpr (PRD == PRD_BARRIER) {
@@ -909,9 +942,9 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     // without a zero factor; with one, pnz for the zero factor
                     // and 0 for the others; 0 with two or more
                     const bool zone = !ps.dead && F > R(P.minus_smooth);
-                    const R f = (R(P.bar_smooth) - F) * R(P.inv_two_smooth);
+                    const R f = (ZT ? R(P.bar_smooth) - F : max(R(P.bar_smooth) - F, tiny_d)) * R(P.inv_two_smooth);
                     R others = recip_mul(ps.alive, f);
-                    if (__builtin_expect(nzw, 0))
+                    if (ZT && __builtin_expect(nzw, 0))
                         others = ps.nzero == 0 ? others : ((ps.nzero == 1 && f == R(0)) ? ps.pnz : R(0));
                     if (zone) Fb -= alive_bar * others * R(P.inv_two_smooth);
                     if (at_last) {
@@ -1003,9 +1036,9 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
             const R kz = -alive_bar * (ps.dead ? R(0) : ps.alive);
             auto barrier_sf = [&](R S) -> R {
                 const bool zone = !ps.dead && S > R(P.minus_smooth);
-                const R d = R(P.bar_smooth) - S;
+                const R d = ZT ? R(P.bar_smooth) - S : max(R(P.bar_smooth) - S, tiny_d);
                 R sf = kz * recip_mul(S, d);
-                if (__builtin_expect(nzw, 0))
+                if (ZT && __builtin_expect(nzw, 0))
                     sf = ps.nzero == 0 ? sf
                                        : ((ps.nzero == 1 && d * R(P.inv_two_smooth) == R(0))
                                               ? -alive_bar * ps.pnz * R(P.inv_two_smooth) * S : R(0));
@@ -1089,44 +1122,71 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     }
                 }
                 R xnext = x;     // state at point i + 1
-                // groups of U steps, i0 down to i0 - U + 1; the checkpoint
+                // every local-vol node positive: no vol is 0, no normal drawn again
+                const bool iv_pos = P.iv_pos != 0;
+                // groups of U steps, top down to top - U + 1; the checkpoint
                 // loads of the next group are issued before the current group
                 // is worked (FP64 and the book hold more registers: U = 1).
-                // Rows below 0 read the slot's guard rows (never used).
+                // FP32 groups of four are the aligned groups of the interleaved
+                // checkpoints (steps 4g .. 4g + 3), one 16-byte load of states
+                // and one word of brackets each; the top group may be partial
+                // (steps from n on are skipped).  The prefetch below step 0
+                // reads the slot's guard rows (never used).
                 constexpr int U = sizeof(R) == 8 ? 1 : (BOOK ? 1 : 4);
+                constexpr bool VG = IL && U == 4;
+                static_assert(U == 1 || VG, "groups of four steps read interleaved checkpoints");
                 static_assert(U <= kCkptGuardRows, "the prefetch reads at most kCkptGuardRows rows below 0");
                 R buf[U];
                 int lbuf[U];
-                // states and brackets of rows top, top - 1, ..., top - U + 1
+                // states and brackets of steps top, top - 1, ..., top - U + 1
                 auto load_group = [&](int top, R* xb, int* lb) {
-                    const R* src = ck + (top - (U - 1)) * kWave + lane;
-                    const uint8_t* srcl = ckl + (top - (U - 1)) * kWave + lane;
+                    if constexpr (VG) {
+                        const int e = ck_el(top - 3);   // a 16-byte aligned group
+                        const float4 v = *reinterpret_cast<const float4*>(ck + e);
+                        xb[0] = v.w;
+                        xb[1] = v.z;
+                        xb[2] = v.y;
+                        xb[3] = v.x;
+                        const uint32_t w = ck_lo ? *reinterpret_cast<const uint32_t*>(ckl + e) : 0u;
 #pragma unroll
-                    for (int k = 0; k < U; ++k) {
-                        xb[k] = src[(U - 1 - k) * kWave];
-                        lb[k] = ck_lo ? int(srcl[(U - 1 - k) * kWave]) : 0;
+                        for (int k = 0; k < 4; ++k) lb[k] = int((w >> (8 * (3 - k))) & 0xffu);
+                    } else {
+                        const int e = ck_el(top);
+                        xb[0] = ck[e];
+                        lb[0] = ck_lo ? int(ckl[e]) : 0;
                     }
                 };
-                if (n > 0) load_group(n - 1, buf, lbuf);
-                auto group = [&](int i0, auto partial) {
+                auto group = [&](int top, auto partial) {
                     constexpr bool PART = decltype(partial)::value;
                     R nbuf[U];
                     int nlbuf[U];
-                    if constexpr (!PART) load_group(i0 - U, nbuf, nlbuf);
+                    load_group(top - U, nbuf, nlbuf);
                     // phase 1, independent of the adjoint chain: brackets, the
                     // derivative of each step to its vol, the sample adjoints
                     R ws[U], sl[U], wa[U], wb[U], sf[U];
                     int js[U];
+                    // the sample index of each step (aligned groups: one 16-byte read)
+                    int smps[U];
+                    if constexpr (VG) {
+                        const int4 v = *reinterpret_cast<const int4*>(step_sample + (top - 3));
+                        smps[0] = v.w;
+                        smps[1] = v.z;
+                        smps[2] = v.y;
+                        smps[3] = v.x;
+                    } else {
+                        smps[0] = step_sample[top];
+                    }
 #pragma unroll
                     for (int k = 0; k < U; ++k) {
-                        const int i = i0 - k;
+                        const int i = top - k;
+                        const int smp = __builtin_amdgcn_readfirstlane(smps[k]);
                         ws[k] = R(0);
                         sl[k] = R(0);
                         wa[k] = R(0);
                         wb[k] = R(0);
                         sf[k] = R(0);
                         js[k] = 0;
-                        if (PART && i < 0) continue;
+                        if (PART && i >= n) continue;
                         const R xi = buf[k];
                         const int lo = ck_lo ? lbuf[k] : count_le<LEAN>(ks, xi);
                         const Bracket<R> b = local_vol_at(kxi, iv + i * mp, xi, lo);
@@ -1135,18 +1195,16 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         // where a local vol is 0 (rare; a wave-uniform test) the
                         // normal is drawn again
                         ws[k] = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
-                        if (__builtin_amdgcn_ballot_w64(b.vol == R(0)) != 0)
+                        if (!iv_pos && __builtin_amdgcn_ballot_w64(b.vol == R(0)) != 0)
                             if (b.vol == R(0)) ws[k] = draw(i);
                         sl[k] = b.slope;
                         wa[k] = b.wa;
                         wb[k] = b.wb;
                         js[k] = b.j;
                         if constexpr (UNDER) {
-                            if (__builtin_amdgcn_readfirstlane(step_sample[i]) >= 0 &&
-                                __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)
+                            if (smp >= 0 && __builtin_amdgcn_ballot_w64(xi > x_lo) != 0)
                                 sf[k] = barrier_sf(state_spot(xi, s0));
                         } else {
-                            const int smp = __builtin_amdgcn_readfirstlane(step_sample[i]);
                             if (smp >= 0) {
                                 const R S = state_spot(xi, s0);
                                 R Fb, nb, db;
@@ -1165,25 +1223,29 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         abar = abar + sf[k];
                     }
                     // phase 3: scatter onto the two bracketing local-vol nodes
-                    // (rows i0 - U + 1 .. i0 at fixed offsets from one base)
+                    // (rows top - U + 1 .. top at fixed offsets from one base)
                     if (active) {
-                        double* rows = ivbl + (i0 - (U - 1)) * mp;
+                        double* rows = ivbl + (top - (U - 1)) * mp;
 #pragma unroll
                         for (int k = 0; k < U; ++k) {
-                            if (PART && i0 - k < 0) break;
+                            if (PART && top - k >= n) continue;
                             double* row = rows + (U - 1 - k) * mp;
                             atomicAdd(row + js[k], double(vb[k] * wa[k]));
                             atomicAdd(row + js[k] + 1, double(vb[k] * wb[k]));
                         }
                     }
-                    if constexpr (!PART) {
 #pragma unroll
-                        for (int k = 0; k < U; ++k) { buf[k] = nbuf[k]; lbuf[k] = nlbuf[k]; }
-                    }
+                    for (int k = 0; k < U; ++k) { buf[k] = nbuf[k]; lbuf[k] = nlbuf[k]; }
                 };
-                int i0 = n - 1;
-                for (; i0 >= U - 1; i0 -= U) group(i0, std::false_type{});
-                if (i0 >= 0) group(i0, std::true_type{});
+                if (n > 0) {
+                    int top = VG ? (((n - 1) >> 2) << 2) + 3 : n - 1;
+                    load_group(top, buf, lbuf);
+                    if (VG && top >= n) {
+                        group(top, std::true_type{});
+                        top -= U;
+                    }
+                    for (; top >= U - 1; top -= U) group(top, std::false_type{});
+                }
                 if (active) padj[0] += double(abar) / s0;      // d log(S0) / d S0
             } else {
                 // ---- Black-Scholes: forward tangents of the spot with respect

@@ -5,6 +5,7 @@
 // code forms, lies inside its own region and inside the dynamic LDS size the
 // host passes to the launch (smem_layout(...).total), for each layout.
 // Prints one JSON object; tests/test_layout.py reads it.
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -35,7 +36,10 @@ static void kernel_extents(int n, int m, int ns, int npay, int waves, int rb, bo
     region(L.lo, size_t(n) * (lean ? 8 : kWave) * 4, L.hi, T);
     region(L.hi, size_t(waves - 1) * n * 4 + size_t(n) * 4, L.hi_tail, T);
     region(L.hi_tail, size_t(n) * 4, L.step_sample, T);
-    region(L.step_sample, size_t(n + 1) * 4, L.invn, T);          // step_sample[e], e <= n
+    // step_sample[e], e <= n; the FP32 checkpoint reverse sweep reads the
+    // aligned group of four holding step n - 1, up to e = 4 floor((n - 1) / 4) + 3
+    const size_t ss_used = size_t(rb == 4 && ad && dup && n > 0 ? std::max(n, 4 * ((n - 1) / 4) + 3) + 1 : n + 1);
+    region(L.step_sample, ss_used * 4, L.invn, T);
     // FP32 FULL: the table itself, rows 0 .. kInvNormalRows - 1 of 16 bytes
     region(L.invn, rb == 4 && !lean ? size_t(kInvNormalRows) * 16 : 0, L.knots, T);
     // interval pairs kxi[2c], kxi[2c + 1], c <= m; knot search reads kxi[2 + 2 (m - 1)]
