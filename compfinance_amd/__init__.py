@@ -447,8 +447,9 @@ def sobol_points(first_path, n_paths, dim, precision=FP64, layout=-1):
     """Sobol points of paths first_path .. first_path + n_paths - 1 (path p
     uses index p + 1: skipTo and next, sobol.h:77-151) in `dim` dimensions, as
     the path kernels draw them on the device: (words [n_paths][dim] uint32,
-    normals [n_paths][dim] float64 or float32).  layout: 0 FULL, 1 LEAN, -1
-    FULL when it fits."""
+    normals [n_paths][dim] float64 or float32).  layout: 0 FULL (rows), 1
+    LEAN, 2 FULL interleaved by four dimensions and read four at a time (the
+    FP32 adjoint kernels' form), -1 FULL when it fits."""
     words = np.zeros((int(n_paths), int(dim)), dtype=np.uint32)
     normals = np.zeros((int(n_paths), int(dim)), dtype=np.float64 if precision == FP64 else np.float32)
     _check(lib().cf_sobol_points(int(first_path), int(n_paths), int(dim), int(precision), int(layout),

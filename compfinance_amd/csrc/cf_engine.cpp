@@ -1312,11 +1312,11 @@ int cf_sobol_points(int64_t first_path, int64_t n_paths, int dim, int precision,
             fail(CF_ERR_ARG, "Sobol path range out of range");
         if (dim < 1 || dim > cfk::kSobolDims) fail(CF_ERR_ARG, "Sobol dimension out of range");
         if (precision != CF_FP64 && precision != CF_FP32) fail(CF_ERR_ARG, "unknown precision");
-        if (layout < -1 || layout > 1) fail(CF_ERR_ARG, "unknown layout");
+        if (layout < -1 || layout > 2) fail(CF_ERR_ARG, "unknown layout");
         Engine& E = engine();
         const int rb = precision == CF_FP64 ? 8 : 4;
-        bool lean = layout == 1;
-        if (layout == -1) lean = cfk::sobol_points_smem(rb, false, dim, 256) > size_t(E.max_lds);
+        if (layout == -1) layout = cfk::sobol_points_smem(rb, false, dim, 256) > size_t(E.max_lds) ? 1 : 0;
+        const bool lean = layout == 1;
         if (cfk::sobol_points_smem(rb, lean, dim, 256) > size_t(E.max_lds))
             fail(CF_ERR_CAPACITY, "Sobol tables do not fit the block's shared memory");
         const size_t n = size_t(n_paths) * size_t(dim);
@@ -1326,7 +1326,7 @@ int cf_sobol_points(int64_t first_path, int64_t n_paths, int dim, int precision,
         } dw, dn;
         if (words) dw.ensure(n * 4);
         if (normals) dn.ensure(n * size_t(rb));
-        const int rc = cfk::launch_sobol_points(rb, lean, E.d_dirs, dim, first_path, n_paths,
+        const int rc = cfk::launch_sobol_points(rb, layout, E.d_dirs, dim, first_path, n_paths,
                                                 static_cast<uint32_t*>(dw.p), dn.p, E.stream);
         if (rc != 0) hip_check(hipError_t(rc), "Sobol points launch");
         if (words) hip_check(hipMemcpyAsync(words, dw.p, n * 4, hipMemcpyDeviceToHost, E.stream), "copy");

@@ -201,6 +201,18 @@ struct SmemLayout {
 
 CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+// Sobol words in LDS.  LO, the low words of the 64 lanes of a block of 64
+// Sobol indices, is interleaved by groups of four dimensions: dimension d of
+// lane l is word ((d >> 2) * 64 + l) * 4 + (d & 3), so a lane reads four
+// consecutive dimensions in one 16-byte word (dimensions padded to a multiple
+// of four).  The LEAN layout keeps only direction numbers 0..5 of each
+// dimension (8 words).  A wave's row of high words holds the dimensions at a
+// stride of a multiple of four words, 16-byte aligned.
+CF_HD inline int sobol_dims4(int n) { return (n + 3) & ~3; }
+CF_HD inline size_t sobol_lo_words(int n, bool lean) { return size_t(lean ? n * 8 : sobol_dims4(n) * kWave); }
+CF_HD inline int sobol_hi_stride(int n) { return sobol_dims4(n); }
+CF_HD inline int sobol_lo_el(int d, int lane) { return ((d >> 2) << 8) + (lane << 2) + (d & 3); }
+
 // Paths [first, first + n) use Sobol indices [first + 1, first + n], taken in
 // blocks of 64 aligned indices.  When the last block's indices (lanes 0..(hi
 // mod 64)) all fall below the first block's first lane (lo mod 64), the last
@@ -351,9 +363,9 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
 {
     SmemLayout L{};
     size_t o = 0;
-    L.lo = o;          o = cf_align16(o + size_t(n_steps) * (lean ? 8 : kWave) * 4);
-    L.hi = o;          o = cf_align16(o + size_t(waves) * n_steps * 4);
-    L.hi_tail = o;     o = cf_align16(o + size_t(n_steps) * 4);
+    L.lo = o;          o = cf_align16(o + sobol_lo_words(n_steps, lean) * 4);
+    L.hi = o;          o = cf_align16(o + size_t(waves) * sobol_hi_stride(n_steps) * 4);
+    L.hi_tail = o;     o = cf_align16(o + size_t(sobol_hi_stride(n_steps)) * 4);
     L.step_sample = o; o = cf_align16(o + size_t(n_steps + 1) * 4);
     // LEAN reads the inverse-normal table and the bucket entries, like the
     // local-vol table, from global memory through L1
@@ -437,7 +449,8 @@ int upload_sobol_dirs(const uint32_t* host, uint32_t** device);
 // type, nullable) of paths [first, first + npath), as the path kernels form
 // them, in the FULL or LEAN form.
 size_t sobol_points_smem(int real_bytes, bool lean, int dim, int block);
-int launch_sobol_points(int real_bytes, bool lean, const uint32_t* dirs, int dim, int64_t first, int64_t npath,
+// layout: 0 FULL rows, 1 LEAN, 2 FULL quads (SobolLoLayout)
+int launch_sobol_points(int real_bytes, int layout, const uint32_t* dirs, int dim, int64_t first, int64_t npath,
                         uint32_t* words, void* normals, void* stream);
 
 // mrg32k3a normals of paths [first_path, first_path + n_path), dim steps,

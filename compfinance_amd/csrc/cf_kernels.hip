@@ -160,24 +160,27 @@ int upload_sobol_dirs(const uint32_t* host, uint32_t** device)
 
 // Words and normals of paths [first, first + npath) in dim dimensions, formed
 // exactly as the path kernels form them (sobol_stage_lo, sobol_block,
-// sobol_low_word, inv_normal); written [path][dim].  256-thread blocks; LDS:
-// LO, one high row per wave, the tail row, and (FP32 FULL) the inverse-normal
-// table.
-template <typename R, bool LEAN>
+// sobol_low_word / sobol_words4, inv_normal); written [path][dim].
+// 256-thread blocks; LDS: LO (rows, quads or LEAN), one high row per wave,
+// the tail row, and (FP32 FULL) the inverse-normal table.  The quads layout
+// reads dimensions four at a time up to the last whole group, then one at a
+// time, as the FP32 adjoint kernels' forward sweep does.
+template <typename R, int LO>
 __global__ void __launch_bounds__(256) sobol_points_kernel(const uint32_t* dirs, int dim, int64_t first, int64_t npath,
                                                            int merge_tail, uint32_t* words, R* normals)
 {
+    constexpr bool LEAN = LO == kLoLean;
     extern __shared__ __align__(16) unsigned char smem[];
     const int waves = blockDim.x >> 6, tid = threadIdx.x, nthr = blockDim.x;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const size_t o_hi = cf_align16(size_t(dim) * (LEAN ? 8 : kWave) * 4);
-    const size_t o_tail = cf_align16(o_hi + size_t(waves) * dim * 4);
-    const size_t o_invn = cf_align16(o_tail + size_t(dim) * 4);
+    const size_t o_hi = cf_align16(sobol_lo_words(dim, LEAN) * 4);
+    const size_t o_tail = cf_align16(o_hi + size_t(waves) * sobol_hi_stride(dim) * 4);
+    const size_t o_invn = cf_align16(o_tail + size_t(sobol_hi_stride(dim)) * 4);
     uint32_t* lo_tab = reinterpret_cast<uint32_t*>(smem);
-    uint32_t* hi = reinterpret_cast<uint32_t*>(smem + o_hi) + wave * dim;
+    uint32_t* hi = reinterpret_cast<uint32_t*>(smem + o_hi) + wave * sobol_hi_stride(dim);
     uint32_t* hi_tail = reinterpret_cast<uint32_t*>(smem + o_tail);
     float* invn = reinterpret_cast<float*>(smem + o_invn);
-    sobol_stage_lo<LEAN>(lo_tab, dirs, dim, tid, nthr);
+    sobol_stage_lo<LO>(lo_tab, dirs, dim, tid, nthr);
     if constexpr (sizeof(R) == 4 && !LEAN)
         for (int e = tid; e < kInvNormalRows * 4; e += nthr) invn[e] = kInvNormalF32[e];
     __syncthreads();
@@ -193,11 +196,19 @@ __global__ void __launch_bounds__(256) sobol_points_kernel(const uint32_t* dirs,
     for (int64_t q = q0 + int64_t(blockIdx.x) * waves + wave; q <= qe; q += int64_t(gridDim.x) * waves) {
         const SobolLane sl = sobol_block(hi, hi_tail, dirs, dim, q, q0, q1, nlo, nhi, merge, lane, true);
         if (sl.active) {
-            for (int i = 0; i < dim; ++i) {
-                const uint32_t w = sl.hl[i] ^ sobol_low_word<LEAN>(lo_tab, lm, lane, i);
+            auto put = [&](int i, uint32_t w) {
                 if (words) words[size_t(sl.pidx) * dim + i] = w;
                 if (normals) normals[size_t(sl.pidx) * dim + i] = inv_normal(w, tab);
+            };
+            int i = 0;
+            if constexpr (LO == kLoQuads) {
+                for (; i + 4 <= dim; i += 4) {
+                    uint32_t w4[4];
+                    sobol_words4(lo_tab, sl.hl, lane, i, w4);
+                    for (int k = 0; k < 4; ++k) put(i + k, w4[k]);
+                }
             }
+            for (; i < dim; ++i) put(i, sl.hl[i] ^ sobol_low_word<LO>(lo_tab, lm, lane, i));
         }
         // the rows are rewritten for the wave's next block
         wave_sync_lds();
@@ -206,25 +217,29 @@ __global__ void __launch_bounds__(256) sobol_points_kernel(const uint32_t* dirs,
 
 size_t sobol_points_smem(int real_bytes, bool lean, int dim, int block)
 {
-    const size_t o_hi = cf_align16(size_t(dim) * (lean ? 8 : kWave) * 4);
-    const size_t o_tail = cf_align16(o_hi + size_t(block / kWave) * dim * 4);
-    const size_t o_invn = cf_align16(o_tail + size_t(dim) * 4);
+    const size_t o_hi = cf_align16(sobol_lo_words(dim, lean) * 4);
+    const size_t o_tail = cf_align16(o_hi + size_t(block / kWave) * sobol_hi_stride(dim) * 4);
+    const size_t o_invn = cf_align16(o_tail + size_t(sobol_hi_stride(dim)) * 4);
     return o_invn + (real_bytes == 4 && !lean ? size_t(kInvNormalRows) * 16 : 0);
 }
 
-int launch_sobol_points(int real_bytes, bool lean, const uint32_t* dirs, int dim, int64_t first, int64_t npath,
+int launch_sobol_points(int real_bytes, int layout, const uint32_t* dirs, int dim, int64_t first, int64_t npath,
                         uint32_t* words, void* normals, void* stream)
 {
     const int block = 256;
+    const bool lean = layout == kLoLean;
     const size_t smem = sobol_points_smem(real_bytes, lean, dim, block);
     const bool merge = merge_tail_blocks(first, npath);
     const int64_t blocks64 = ((first + npath) >> 6) - ((first + 1) >> 6) + 1 - (merge ? 1 : 0);
     const int grid = int(std::min<int64_t>((blocks64 + block / kWave - 1) / (block / kWave), 4096));
     const void* f;
-    if (real_bytes == 4) f = lean ? reinterpret_cast<const void*>(&sobol_points_kernel<float, true>)
-                                  : reinterpret_cast<const void*>(&sobol_points_kernel<float, false>);
-    else f = lean ? reinterpret_cast<const void*>(&sobol_points_kernel<double, true>)
-                  : reinterpret_cast<const void*>(&sobol_points_kernel<double, false>);
+    auto pick = [&](auto real) -> const void* {
+        using T = decltype(real);
+        if (layout == kLoLean) return reinterpret_cast<const void*>(&sobol_points_kernel<T, kLoLean>);
+        if (layout == kLoQuads) return reinterpret_cast<const void*>(&sobol_points_kernel<T, kLoQuads>);
+        return reinterpret_cast<const void*>(&sobol_points_kernel<T, kLoRows>);
+    };
+    f = real_bytes == 4 ? pick(0.0f) : pick(0.0);
     if (smem > 65536) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(smem));
         if (e != hipSuccess) return int(e);

@@ -254,20 +254,30 @@ __device__ __forceinline__ R euler_step(R x, R vol, R g)
 // in dimension d is LO[d][r] ^ HI_q[d].  The path kernels and the points
 // kernel share these functions.
 
-// LO staged in LDS by a block: FULL keeps the 64 lanes' low words of each
-// dimension, LEAN only direction numbers 0..5 (the low word is formed in
-// registers from the lane's masks).
-template <bool LEAN>
+// LO staged in LDS by a block, in one of three layouts: rows (the 64 lanes'
+// low words of each dimension, [d][lane]), quads (the same words interleaved
+// by groups of four dimensions, [d / 4][lane][d % 4], for kernels that read a
+// lane's four consecutive dimensions in one 16-byte word; a lane reading one
+// word at a time would stride 16 bytes and conflict on the LDS banks), or
+// LEAN (only direction numbers 0..5: the low word is formed in registers from
+// the lane's masks).
+enum SobolLoLayout { kLoRows = 0, kLoLean = 1, kLoQuads = 2 };
+
+template <int LO>
 __device__ __forceinline__ void sobol_stage_lo(uint32_t* lo_tab, const uint32_t* dirs, int n, int tid, int nthr)
 {
-    if constexpr (LEAN) {
+    if constexpr (LO == kLoLean) {
         for (int e = tid; e < n * 8; e += nthr) {
             const int d = e >> 3, k = e & 7;
             lo_tab[e] = k < 6 ? dirs[k * kSobolDims + d] : 0u;
         }
     } else {
-        for (int e = tid; e < n * kWave; e += nthr) {
-            const int d = e >> 6, r = e & 63;
+        // word e: rows dimension e >> 6, lane e & 63; quads dimension
+        // 4 (e >> 8) + (e & 3), lane (e >> 2) & 63
+        for (int e = tid; e < sobol_dims4(n) * kWave; e += nthr) {
+            const int d = LO == kLoQuads ? ((e >> 8) << 2) + (e & 3) : e >> 6;
+            const int r = LO == kLoQuads ? (e >> 2) & 63 : e & 63;
+            if (d >= n) continue;
             const uint32_t g = uint32_t(r ^ (r >> 1));
             uint32_t x = 0;
 #pragma unroll
@@ -292,16 +302,31 @@ __device__ __forceinline__ SobolLaneMasks sobol_lane_masks(int lane)
 }
 
 // low word of dimension i for this lane
-template <bool LEAN>
+template <int LO>
 __device__ __forceinline__ uint32_t sobol_low_word(const uint32_t* lo_tab, const SobolLaneMasks& lm, int lane, int i)
 {
-    if constexpr (LEAN) {
+    if constexpr (LO == kLoLean) {
         const uint32_t* v = lo_tab + i * 8;
         return (lm.m[0] & v[0]) ^ (lm.m[1] & v[1]) ^ (lm.m[2] & v[2]) ^ (lm.m[3] & v[3]) ^ (lm.m[4] & v[4]) ^
                (lm.m[5] & v[5]);
+    } else if constexpr (LO == kLoQuads) {
+        return lo_tab[sobol_lo_el(i, lane)];
     } else {
         return lo_tab[i * kWave + lane];
     }
+}
+
+// Quads layout: this lane's words of dimensions i .. i + 3 (i a multiple of
+// four, below the padded dimension count), one 16-byte read of LO and one of
+// the lane's high row (16-byte aligned rows)
+__device__ __forceinline__ void sobol_words4(const uint32_t* lo_tab, const uint32_t* hl, int lane, int i, uint32_t* w)
+{
+    const uint4 lw = *reinterpret_cast<const uint4*>(lo_tab + sobol_lo_el(i, lane));
+    const uint4 hw = *reinterpret_cast<const uint4*>(hl + i);
+    w[0] = lw.x ^ hw.x;
+    w[1] = lw.y ^ hw.y;
+    w[2] = lw.z ^ hw.z;
+    w[3] = lw.w ^ hw.w;
 }
 
 // HI_qb of the n dimensions into a wave's LDS row, formed by the wave's lanes
@@ -415,14 +440,22 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     // the inverse-normal table (FP32) and the knot search table are read from
     // global memory through the L1 cache, not LDS, by the LEAN kernels and by
     // the FP32 recompute kernels, whose re-simulation otherwise loads the LDS
-    // most (measured: C3 FP32 recompute 0.719 -> 0.643 ms; the checkpoint and
-    // pricing kernels are faster with both in LDS)
+    // most (measured: C3 FP32 recompute 0.719 -> 0.643 ms; the pricing kernels
+    // are faster with both in LDS).  The FP32 Dupire checkpoint kernels other
+    // than the book, bound by the LDS pipe, read the inverse-normal table
+    // through L1 (C3 0.587 -> 0.576 ms; with the knot search table as well
+    // they lose: the vector memory path saturates instead)
     constexpr bool GTAB = LEAN || (RECOMP && sizeof(R) == 4);
+    constexpr bool GINV = GTAB || (sizeof(R) == 4 && ADJ == ADJ_CHECKPOINT && DUP && !BOOK);
 
+    // Sobol LO layout: the FP32 Dupire adjoint kernels other than the book
+    // read four dimensions at once (quads); the others one at a time (rows)
+    constexpr bool QUADS = sizeof(R) == 4 && AD && DUP && !BOOK && !LEAN && !NBUF;
+    constexpr int LOL = LEAN ? kLoLean : (QUADS ? kLoQuads : kLoRows);
     // ---- stage the block's tables in LDS
-    sobol_stage_lo<LEAN>(lo_tab, P.dirs, n, tid, nthr);
+    sobol_stage_lo<LOL>(lo_tab, P.dirs, n, tid, nthr);
     for (int e = tid; e <= n; e += nthr) step_sample[e] = P.step_sample[e];
-    if constexpr (sizeof(R) == 4 && !GTAB)
+    if constexpr (sizeof(R) == 4 && !GINV)
         for (int e = tid; e < kInvNormalRows * 4; e += nthr) invn[e] = kInvNormalF32[e];
     if constexpr (DUP) {
         const R* gk = static_cast<const R*>(P.knots);
@@ -460,8 +493,8 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     const R* iv;
     if constexpr (LEAN) iv = static_cast<const R*>(P.iv); else iv = iv_lds;
     const SobolLaneMasks lmask = sobol_lane_masks(lane);
-    auto low_word = [&](int i) -> uint32_t { return sobol_low_word<LEAN>(lo_tab, lmask, lane, i); };
-    uint32_t* hi = hi_all + wave * n;
+    auto low_word = [&](int i) -> uint32_t { return sobol_low_word<LOL>(lo_tab, lmask, lane, i); };
+    uint32_t* hi = hi_all + wave * sobol_hi_stride(n);
     double* accw = acc + wave * stride;
     const int64_t nlo = P.first_path + 1, nhi = P.first_path + P.n_path;   // Sobol indices
     const int64_t q0 = nlo >> 6, q1 = nhi >> 6;
@@ -581,7 +614,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
         // the inverse-normal table (LDS, or for LEAN the constant array
         // itself) less kInvNormalRowBias rows
         const R* invn_tab;
-        if constexpr (GTAB && sizeof(R) == 4) invn_tab = reinterpret_cast<const R*>(kInvNormalF32);
+        if constexpr (GINV && sizeof(R) == 4) invn_tab = reinterpret_cast<const R*>(kInvNormalF32);
         else invn_tab = invn;
         invn_tab -= 4 * kInvNormalRowBias;
         auto draw = [&](int i) -> R {
@@ -800,12 +833,14 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
             // at once (interleaved FP32 checkpoints), else stored here.
             R gx[4];
             uint32_t glo = 0;
-            auto fwd_step = [&](int i0, int k, auto grp) {
-                constexpr bool GRP = decltype(grp)::value;
+            uint32_t gw[4];   // QW: the group's four Sobol words, read at once
+            auto fwd_step = [&](int i0, int k, auto grp, auto qw) {
+                constexpr bool GRP = decltype(grp)::value, QW = decltype(qw)::value;
                 const int i = i0 + k;
                 R g;
-                if constexpr (NBUF || LEAN) g = draw(i);
-                else g = inv_normal(hl[i0 + k] ^ (lo_tab + i0 * kWave + lane)[k * kWave], invn_tab);
+                if constexpr (QW) g = inv_normal(gw[k], invn_tab);
+                else if constexpr (NBUF || LEAN) g = draw(i);
+                else g = inv_normal(hl[i] ^ low_word(i), invn_tab);
                 if constexpr (DUP) {
                     if constexpr (GRP) gx[k] = x;
                     else if constexpr (IL) ck[ck_el(i)] = x;
@@ -836,17 +871,21 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 // (UF = 4 kernels with interleaved checkpoints: one 16-byte
                 // store of the four states and one word of brackets per group)
                 constexpr bool GRP = IL && UF == 4;
+                // (the quads layout: the group's four Sobol words in two 16-byte reads)
+                constexpr bool QW = QUADS && UF == 4;
                 for (; i + UF <= n; i += UF) {
                     glo = 0;
+                    if constexpr (QW) sobol_words4(lo_tab, hl, lane, i, gw);
     #pragma unroll
-                    for (int k = 0; k < UF; ++k) fwd_step(i, k, std::integral_constant<bool, GRP>{});
+                    for (int k = 0; k < UF; ++k)
+                        fwd_step(i, k, std::integral_constant<bool, GRP>{}, std::integral_constant<bool, QW>{});
                     if constexpr (GRP) {
                         *reinterpret_cast<float4*>(ck + ck_el(i)) = make_float4(gx[0], gx[1], gx[2], gx[3]);
                         if (ck_lo) *reinterpret_cast<uint32_t*>(ckl + ck_el(i)) = glo;
                     }
                 }
             }
-            for (; i < n; ++i) fwd_step(i, 0, std::false_type{});
+            for (; i < n; ++i) fwd_step(i, 0, std::false_type{}, std::false_type{});
 
             // ---- payoffs of the barrier (known at the end of the path)
             if constexpr (PRD == PRD_BARRIER) {
@@ -1068,6 +1107,15 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     R xi = ck[sg * kWave + lane];
                     R ca[kSeg], cb[kSeg], cw[kSeg], cv[kSeg];
                     int cj[kSeg];
+                    // quads layout: the segment's Sobol words, four at a time
+                    // (a group from step n on is not read)
+                    uint32_t sw[kSeg];
+                    if constexpr (QUADS) {
+                        static_assert(kSeg % 4 == 0, "segments of whole groups of four");
+#pragma unroll
+                        for (int q4 = 0; q4 < kSeg; q4 += 4)
+                            if (i0 + q4 < n) sobol_words4(lo_tab, hl, lane, i0 + q4, sw + q4);
+                    }
 #pragma unroll
                     for (int k = 0; k < kSeg; ++k) {
                         const int i = i0 + k;
@@ -1077,7 +1125,9 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         cv[k] = R(0);
                         cj[k] = 0;
                         if (i >= n) continue;
-                        const R g = draw(i);
+                        R g;
+                        if constexpr (QUADS) g = inv_normal(sw[k], invn_tab);
+                        else g = draw(i);
                         const Bracket<R> b = local_vol<LEAN>(ks, kxi, iv + i * mp, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol
                         ca[k] = R(1) + ws * b.slope;

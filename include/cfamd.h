@@ -395,8 +395,10 @@ int cf_device_synchronize(void);
  * normals: [n_paths][dim] Gaussians in the precision's real type, the
  * reference's nextG (sobol.h:110-116, invNormalCdf gaussians.h:47-87) in
  * FP64, the kernels' table form in FP32 (nullable).  layout: 0 the kernels'
- * FULL form (low words staged per lane), 1 LEAN (formed from six direction
- * numbers), -1 FULL when its tables fit the block's shared memory. */
+ * FULL form (low words staged per lane, [dimension][lane]), 1 LEAN (formed
+ * from six direction numbers), 2 FULL interleaved by groups of four
+ * dimensions and read four at a time (the FP32 adjoint kernels' form), -1
+ * FULL when its tables fit the block's shared memory. */
 int cf_sobol_points(int64_t first_path, int64_t n_paths, int dim, int precision, int layout, uint32_t* words,
                     void* normals);
 

@@ -32,9 +32,12 @@ static void kernel_extents(int n, int m, int ns, int npay, int waves, int rb, bo
     const SmemLayout L = smem_layout(n, m, ns, npay, waves, rb, dup, eus, ad, lean, nb, book_bytes, copies);
     const size_t T = L.total;
     const int mp = lv_cols(m);
-    // stage loops: lo_tab[e], e < n * (lean ? 8 : 64); hi_all[wave * n + d]; hi_tail[d], d < n
-    region(L.lo, size_t(n) * (lean ? 8 : kWave) * 4, L.hi, T);
-    region(L.hi, size_t(waves - 1) * n * 4 + size_t(n) * 4, L.hi_tail, T);
+    // stage loops: lo_tab[e], e < n * 8 (LEAN) or e < 4 ceil(n / 4) * 64 (FULL,
+    // interleaved by four dimensions, read 16 bytes at a time);
+    // hi_all[wave * stride + d], hi_tail[d], d < n (16-byte reads of d < 4 floor(n / 4))
+    const size_t hs = size_t(sobol_hi_stride(n));
+    region(L.lo, (lean ? size_t(n) * 8 : size_t((n + 3) / 4) * 4 * kWave) * 4, L.hi, T);
+    region(L.hi, (size_t(waves - 1) * hs + size_t(n)) * 4, L.hi_tail, T);
     region(L.hi_tail, size_t(n) * 4, L.step_sample, T);
     // step_sample[e], e <= n; the FP32 checkpoint reverse sweep reads the
     // aligned group of four holding step n - 1, up to e = 4 floor((n - 1) / 4) + 3
@@ -133,14 +136,29 @@ int main()
                 }
             }
     }
+    // interleaved tables: the Sobol LO word of (dimension d, lane r) and the
+    // FP32 checkpoint element of (step i, lane r) lie inside their tables, and
+    // the kernel's staging order (word e -> dimension 4 (e >> 8) + (e & 3),
+    // lane (e >> 2) & 63) inverts the read index
+    long il_probes = 0, il_bad = 0;
+    for (int n = 1; n <= 300; ++n)
+        for (int d = 0; d < n; ++d)
+            for (int r = 0; r < kWave; ++r) {
+                const int e = sobol_lo_el(d, r);
+                ++il_probes;
+                if (e < 0 || size_t(e) >= sobol_lo_words(n, false)) ++il_bad;
+                if (((e >> 8) << 2) + (e & 3) != d || ((e >> 2) & 63) != r) ++il_bad;
+                if (e >= ckpt_steps(n, 4) * kWave) ++il_bad;
+            }
     fp32_c3 = smem_layout(156, 30, 157, 2, 16, 4, true, false, true, false, 64, 0, 2).total;
     fp64_c3 = smem_layout(156, 30, 157, 2, 16, 8, true, false, true, false, 64, 0, 1).total;
     lean_60x60 = smem_layout(261, 60, 262, 2, 16, 8, true, false, true, true, 64, 0, 1).total;
     copy_offset_mod128 = int((size_t(ivbar_copy_stride(156, 30)) * 8) % 128);
     std::printf("{\"cases\": %ld, \"wrong\": %ld, \"merged\": %ld, \"misaligned\": %ld, \"fp32_c3\": %zu, "
                 "\"fp64_c3\": %zu, \"lean_60x60_5y_fp64\": %zu, \"copy_offset_mod128\": %d, \"regions\": %ld, "
-                "\"out_of_region\": %ld, \"row_probes\": %ld, \"bad_rows\": %ld}\n",
+                "\"out_of_region\": %ld, \"row_probes\": %ld, \"bad_rows\": %ld, \"il_probes\": %ld, "
+                "\"il_bad\": %ld}\n",
                 cases, wrong, merged, misaligned, fp32_c3, fp64_c3, lean_60x60, copy_offset_mod128, g_regions,
-                g_out_of_region, row_probes, bad_rows);
+                g_out_of_region, row_probes, bad_rows, il_probes, il_bad);
     return 0;
 }

@@ -2,7 +2,8 @@
 (cf_sobol_points: sobol_stage_lo, sobol_block, sobol_low_word and
 inv_normal, shared with the path kernels) against the reference's points
 (tests/golden/sobol_points.npz, written by the reference build: skipTo and
-next, sobol.h:77-151), bit for bit, in the FULL and LEAN forms; ranges with
+next, sobol.h:77-151), bit for bit, in the FULL (rows and quads) and LEAN
+forms; ranges with
 and without the merge of the last block of 64 indices against the oracle; the
 normals against the oracle's Moro function (FP64, gaussians.h:47-87) and the
 FP32 table's accuracy bound."""
@@ -21,7 +22,7 @@ ONEOVER2POW32 = 2.3283064365387E-10   # sobol.h:26
 
 
 def _layouts(dim):
-    return [0, 1] if dim <= 261 else [1]
+    return [0, 1, 2] if dim <= 261 else [1]
 
 
 def test_device_words_equal_the_reference_points(engine):
@@ -35,17 +36,19 @@ def test_device_words_equal_the_reference_points(engine):
             words, _ = cf.sobol_points(b, ref.shape[0], dims, cf.FP64, layout)
             assert np.array_equal(words, ref), (key, layout)
             checked += 1
-    assert checked == 22
+    assert checked == 32
 
 
 @pytest.mark.parametrize("first,count", [(0, 1), (0, 1000), (0, 65536), (5, 1000), (1065, 3835), (65535, 4097)])
-@pytest.mark.parametrize("layout", [0, 1])
-def test_device_word_ranges_equal_the_oracle(engine, first, count, layout):
+@pytest.mark.parametrize("layout", [0, 1, 2])
+@pytest.mark.parametrize("dim", [156, 157])
+def test_device_word_ranges_equal_the_oracle(engine, first, count, layout, dim):
     """Path ranges from path 0 (Sobol index 0 idle), with and without the
-    merged last block, in 156 dimensions."""
+    merged last block, in 156 dimensions and in 157 (a last group of four
+    dimensions with one dimension)."""
     cf = engine
-    words, _ = cf.sobol_points(first, count, 156, cf.FP64, layout)
-    assert np.array_equal(words, po.sobol_points(156, first, count))
+    words, _ = cf.sobol_points(first, count, dim, cf.FP64, layout)
+    assert np.array_equal(words, po.sobol_points(dim, first, count))
 
 
 def test_fp64_normals_are_the_moro_values(engine):
@@ -62,7 +65,7 @@ def test_fp64_normals_are_the_moro_values(engine):
     assert 0.1 < tail.mean() < 0.2
 
 
-@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1, 2])
 def test_fp32_normals_within_the_table_bound(engine, layout):
     """The FP32 kernels' table form: within 5e-7 of the FP64 Moro values (the
     bound the CPU restatement of the table pins at every row boundary), odd

@@ -3,7 +3,7 @@ CPU: the merge of the last block of 64 Sobol indices (against brute force
 over path ranges), the shared-memory layout (alignment, the 160 KiB LDS of a
 CU, the bank offset of the second adjoint grid, every region the kernel
 addresses inside its region and inside the launch's LDS size, the FP32
-inverse-normal rows) and the knot search (against upper_bound, interp.h:40)."""
+inverse-normal rows, the interleaved Sobol and checkpoint tables) and the knot search (against upper_bound, interp.h:40)."""
 import json
 import os
 import subprocess
@@ -24,6 +24,7 @@ def test_merge_and_layout(tmp_path):
     assert out["copy_offset_mod128"] == 64
     assert out["regions"] > 60000 and out["out_of_region"] == 0
     assert out["row_probes"] > 700 and out["bad_rows"] == 0
+    assert out["il_probes"] > 2_000_000 and out["il_bad"] == 0
 
 
 def test_knot_search_equals_upper_bound(tmp_path):
