@@ -679,7 +679,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     }
                 } else {
                     const R f = max(R(P.bar_smooth) - F, tiny_d) * R(P.inv_two_smooth);
-                    if (zone) ps.alive = a_prev * f;
+                    ps.alive = a_prev * (zone ? f : R(1));
                 }
                 // the vanilla of the last sample: after the path (the last
                 // sample is the last timeline point)
@@ -795,6 +795,14 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
         // formed there (forward and reverse)
         constexpr bool UNDER = DUP && PRD == PRD_BARRIER;
         const R x_lo = UNDER ? R(P.bar_xlo) : R(0);
+        // Dupire barrier: whether timeline points base .. base + 63 are
+        // samples, one wave-uniform 64-bit mask from one ballot (bit b: point
+        // base + b); the sweeps test a bit per step instead of reading the
+        // sample map
+        auto sample_bits = [&](int base) -> uint64_t {
+            const int p = base + lane;
+            return __builtin_amdgcn_ballot_w64(p <= n && step_sample[min(p, n)] >= 0);
+        };
         R x;
         if (ck_mode == 2) {
             // reverse only: the end state the forward-only launch kept
@@ -833,12 +841,13 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
             // at once (interleaved FP32 checkpoints), else stored here.
             R gx[4];
             uint32_t glo = 0;
-            uint32_t gw[4];   // QW: the group's four Sobol words, read at once
+            R gz[4];          // QW: the group's four normals, drawn before its steps
+            uint64_t fbits = 0;   // UNDER: bit b, point 64 (i >> 6) + 1 + b is a sample
             auto fwd_step = [&](int i0, int k, auto grp, auto qw) {
                 constexpr bool GRP = decltype(grp)::value, QW = decltype(qw)::value;
                 const int i = i0 + k;
                 R g;
-                if constexpr (QW) g = inv_normal(gw[k], invn_tab);
+                if constexpr (QW) g = gz[k];
                 else if constexpr (NBUF || LEAN) g = draw(i);
                 else g = inv_normal(hl[i] ^ low_word(i), invn_tab);
                 if constexpr (DUP) {
@@ -859,9 +868,15 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 }
                 // a wave-uniform test: the sample's code runs with all lanes (a lane
                 // under the zone changes nothing there)
-                const int s = __builtin_amdgcn_readfirstlane((step_sample + i0 + 1)[k]);
-                if (s >= 0 && (!UNDER || __builtin_amdgcn_ballot_w64(x > x_lo) != 0))
-                    observe(s, DUP ? state_spot(x, s0) : x);
+                if constexpr (UNDER) {
+                    // (a group of four steps lies in one window of 64)
+                    if ((QW ? k == 0 : true) && (i & 63) == 0) fbits = sample_bits(i + 1);
+                    if (((fbits >> (i & 63)) & 1u) && __builtin_amdgcn_ballot_w64(x > x_lo) != 0)
+                        observe(0, state_spot(x, s0));
+                } else {
+                    const int s = __builtin_amdgcn_readfirstlane((step_sample + i0 + 1)[k]);
+                    if (s >= 0) observe(s, DUP ? state_spot(x, s0) : x);
+                }
             };
             // FP32 Dupire adjoint kernels other than the book: groups of four steps
             // (the book's sample code, four times over, and FP64 measured slower)
@@ -875,7 +890,12 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 constexpr bool QW = QUADS && UF == 4;
                 for (; i + UF <= n; i += UF) {
                     glo = 0;
-                    if constexpr (QW) sobol_words4(lo_tab, hl, lane, i, gw);
+                    if constexpr (QW) {
+                        uint32_t gw[4];
+                        sobol_words4(lo_tab, hl, lane, i, gw);
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) gz[k] = inv_normal(gw[k], invn_tab);
+                    }
     #pragma unroll
                     for (int k = 0; k < UF; ++k)
                         fwd_step(i, k, std::integral_constant<bool, GRP>{}, std::integral_constant<bool, QW>{});
@@ -1188,6 +1208,8 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 static_assert(U <= kCkptGuardRows, "the prefetch reads at most kCkptGuardRows rows below 0");
                 R buf[U];
                 int lbuf[U];
+                int rwin = -1;          // UNDER: window of 64 points held in rbits
+                uint64_t rbits = 0;
                 // states and brackets of steps top, top - 1, ..., top - U + 1
                 auto load_group = [&](int top, R* xb, int* lb) {
                     if constexpr (VG) {
@@ -1215,9 +1237,18 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     // derivative of each step to its vol, the sample adjoints
                     R ws[U], sl[U], wa[U], wb[U], sf[U];
                     int js[U];
-                    // the sample index of each step (aligned groups: one 16-byte read)
+                    // the sample index of each step (aligned groups: one 16-byte
+                    // read); the Dupire barrier tests the window's sample bits
+                    // (a group lies in one window of 64 points)
                     int smps[U];
-                    if constexpr (VG) {
+                    if constexpr (UNDER) {
+                        if ((top >> 6) != rwin) {
+                            rwin = top >> 6;
+                            rbits = sample_bits(rwin << 6);
+                        }
+#pragma unroll
+                        for (int k = 0; k < U; ++k) smps[k] = int(((rbits >> ((top - k) & 63)) & 1u)) - 1;
+                    } else if constexpr (VG) {
                         const int4 v = *reinterpret_cast<const int4*>(step_sample + (top - 3));
                         smps[0] = v.w;
                         smps[1] = v.z;
@@ -1229,7 +1260,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
 #pragma unroll
                     for (int k = 0; k < U; ++k) {
                         const int i = top - k;
-                        const int smp = __builtin_amdgcn_readfirstlane(smps[k]);
+                        const int smp = UNDER ? smps[k] : __builtin_amdgcn_readfirstlane(smps[k]);
                         ws[k] = R(0);
                         sl[k] = R(0);
                         wa[k] = R(0);
