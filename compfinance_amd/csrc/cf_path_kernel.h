@@ -588,9 +588,11 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     // Book barrier state in one real per barrier:
     //   v in [0, 1]   no exactly-zero smoothing factor yet, v = product of the
     //                 factors (the survival factor)
-    //   v in [-1, 0)  one zero factor, -v = product of the others
-    //   v = -2        two or more zero factors
+    //   v in [-1, 0)  one zero factor, -v = product of the others (FP64)
+    //   v = -2        two or more zero factors (FP64)
     //   v = -4        breached (survival 0, no derivative)
+    // FP32 keeps every factor at least 1e-30 (as the single barrier keeps
+    // B + w - F above a floor), so its states are only [0, 1] or -4.
     constexpr float kTwoZeros = -2.0f, kBreached = -4.0f;
 
     for (int64_t q = q0 + int64_t(bid) * waves + wave; q <= qe; q += int64_t(nblk) * waves) {
@@ -717,10 +719,17 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                             const R v = bst[b];
                             const bool live = v > R(kBreached);
                             const bool zone = live && F > ms && !(F > bs);
-                            const R f = (bs - F) * bp.i2s;
                             R nv = v;
-                            nv = zone && f != R(0) && v >= R(-1) ? v * f : nv;          // one more factor
-                            nv = zone && f == R(0) ? (v >= R(0) ? -v : R(kTwoZeros)) : nv;   // a zero factor
+                            if constexpr (ZT) {
+                                const R f = (bs - F) * bp.i2s;
+                                nv = zone && f != R(0) && v >= R(-1) ? v * f : nv;          // one more factor
+                                nv = zone && f == R(0) ? (v >= R(0) ? -v : R(kTwoZeros)) : nv;   // a zero factor
+                            } else {
+                                // FP32: a factor at least 1e-30, never exactly 0
+                                // (the state stays in [0, 1] until a breach)
+                                const R f = max((bs - F) * bp.i2s, R(1e-30));
+                                nv = v * (zone ? f : R(1));
+                            }
                             nv = live && F > bs ? R(kBreached) : nv;                    // breach
                             bst[b] = nv;
                         }
@@ -1052,16 +1061,25 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                             if ((bm >> b) & 1u) {
                                 // d alive / dF = -(product of the other factors) / 2w
                                 const R v = bst[b];
-                                if (v > R(kBreached) && F > bp.lo) {   // smoothing zone
-                                    const R i2s = bp.i2s;
-                                    const R f = (bp.up - F) * i2s;
-                                    // product of the other factors: v / f without a zero
-                                    // factor, the non-zero ones for the only zero factor
-                                    // (v >= 0 with f == 0 only after the product underflowed
-                                    // to zero: the other factors' product is 0)
-                                    const R others = v >= R(0) ? (f != R(0) ? recip_mul(v, f) : R(0))
-                                                               : ((v >= R(-1) && f == R(0)) ? -v : R(0));
-                                    Fb -= babar[b] * others * i2s;
+                                if constexpr (ZT) {
+                                    if (v > R(kBreached) && F > bp.lo) {   // smoothing zone
+                                        const R i2s = bp.i2s;
+                                        const R f = (bp.up - F) * i2s;
+                                        // product of the other factors: v / f without a zero
+                                        // factor, the non-zero ones for the only zero factor
+                                        // (v >= 0 with f == 0 only after the product underflowed
+                                        // to zero: the other factors' product is 0)
+                                        const R others = v >= R(0) ? (f != R(0) ? recip_mul(v, f) : R(0))
+                                                                   : ((v >= R(-1) && f == R(0)) ? -v : R(0));
+                                        Fb -= babar[b] * others * i2s;
+                                    }
+                                } else {
+                                    // FP32 (factors at least 1e-30): the others' product
+                                    // is v / f, branch free
+                                    const bool zone = v > R(kBreached) && F > bp.lo;
+                                    const R f = max((bp.up - F) * bp.i2s, R(1e-30));
+                                    const R d = babar[b] * bp.i2s * recip_mul(v, f);
+                                    Fb -= zone ? d : R(0);
                                 }
                             }
                         }

@@ -15,12 +15,12 @@
 //     variant does not fit runs the recompute variant when that fits;
 //     requests that arrive while others run start at once, and the kernels
 //     of different slots overlap on the device;
-//   * steady state: when a slot sees the same launch sequence a second time
+//   * steady state: when a slot sees the same launch sequence a third time
 //     (the exact bytes of every operation's parameters: same plan, shape,
 //     arena block and staging), it captures the uploads, kernels and result
 //     copies in a HIP graph; later requests with that sequence replay the
-//     graph in one launch.  One-off requests launch directly, so a mix of
-//     distinct requests pays no instantiation;
+//     graph in one launch.  Requests seen once or twice launch directly, so a
+//     mix of distinct requests, or of pairs, pays no instantiation;
 //   * completion: a host function queued on the slot's stream after the
 //     result copies signals the dispatcher, which forms the results, frees
 //     the block and the slot and admits more work.
@@ -182,8 +182,10 @@ struct GraphEntry {
 
 // Instantiated graphs kept per slot; the least recently used goes first.
 constexpr size_t kGraphsPerSlot = 8;
-// Sequences launched directly that a slot remembers, to capture on a repeat.
+// Sequences launched directly that a slot remembers with their count, to
+// capture at the kCaptureAt-th sighting.
 constexpr size_t kSeenPerSlot = 32;
+constexpr int kCaptureAt = 3;
 
 struct Slot {
     hipStream_t stream = nullptr;
@@ -196,7 +198,7 @@ struct Slot {
     Server* srv = nullptr;
     int index = 0;
     vector<GraphEntry> graphs;   // used only by the dispatcher while the slot is idle
-    std::deque<vector<unsigned char>> seen;
+    std::deque<std::pair<vector<unsigned char>, int>> seen;   // key, direct launches
     uint64_t tick = 0;
 };
 
@@ -271,9 +273,9 @@ void key_add(vector<unsigned char>& key, const T& x)
 }
 
 // The slot's graph of the launch sequence named by `key` (the bytes of every
-// operation's parameters): found by its bytes; else, on the second sighting
-// of the key, captured from `enqueue` (the same calls as a direct launch) and
-// instantiated (*captured set); else null: launch directly.
+// operation's parameters): found by its bytes; else, on the kCaptureAt-th
+// sighting of the key, captured from `enqueue` (the same calls as a direct
+// launch) and instantiated (*captured set); else null: launch directly.
 template <typename Enqueue>
 hipGraphExec_t slot_graph(Slot& s, vector<unsigned char> key, const Enqueue& enqueue, bool* captured)
 {
@@ -284,12 +286,13 @@ hipGraphExec_t slot_graph(Slot& s, vector<unsigned char> key, const Enqueue& enq
             g.last_use = s.tick;
             return g.exec;
         }
-    auto seen = std::find(s.seen.begin(), s.seen.end(), key);
+    auto seen = std::find_if(s.seen.begin(), s.seen.end(), [&](const auto& e) { return e.first == key; });
     if (seen == s.seen.end()) {
-        s.seen.push_back(std::move(key));
+        s.seen.emplace_back(std::move(key), 1);
         if (s.seen.size() > kSeenPerSlot) s.seen.pop_front();
         return nullptr;
     }
+    if (++seen->second < kCaptureAt) return nullptr;
     s.seen.erase(seen);
     hipGraph_t graph = nullptr;
     hip_check(hipStreamBeginCapture(s.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");

@@ -288,7 +288,7 @@ typedef struct cf_server_opts {
     int64_t hbm_budget;      /* device bytes of the arena */
     int auto_recompute;      /* 1: switch AAD to the recompute variant to fit */
     int graphs;              /* 1: a slot that launches the same sequence a
-                              * second time (same kernels, parameters,
+                              * third time (same kernels, parameters,
                               * addresses and sizes) captures it in a HIP graph
                               * and replays the graph for later repeats;
                               * 0: direct launches only */

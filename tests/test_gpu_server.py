@@ -170,10 +170,10 @@ def test_open_loop_replay_small(engine):
 
 
 def test_repeated_requests_replay_a_captured_graph(engine):
-    """One slot, the same two requests four times each, one at a time: each
-    kind launches directly once, is captured at its repeat and replayed
-    after; results equal the direct launches of a server without graphs and
-    the synchronous entry points."""
+    """One slot, the same two requests five times each, one at a time: each
+    kind launches directly twice, is captured at its third sighting and
+    replayed after; results equal the direct launches of a server without
+    graphs and the synchronous entry points."""
     mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
     num = cf.NumericalParam(numPath=1 << 16, precision=cf.FP32)
     bs, eu = cf.BlackScholes(100.0, 0.2), cf.European(100.0, 1.0)
@@ -181,7 +181,7 @@ def test_repeated_requests_replay_a_captured_graph(engine):
     out = {}
     for graphs in (True, False):
         with cf.Server(slots=1, hbmBudget=1 << 30, graphs=graphs) as srv:
-            out[graphs] = [srv.wait(srv.submit(p, m, n, aad=a)) for _ in range(4) for p, m, n, a in kinds]
+            out[graphs] = [srv.wait(srv.submit(p, m, n, aad=a)) for _ in range(5) for p, m, n, a in kinds]
             st = srv.stats()
         if graphs:
             assert st["graph_captures"] == 2 and st["graph_replays"] == 4, st
@@ -206,14 +206,16 @@ def test_graphs_with_changing_seeds_and_weights(engine):
     the weights."""
     mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
     with cf.Server(slots=1, hbmBudget=1 << 30) as srv:
-        seeds = [(12345, 1234), (12345, 1234), (777, 888), (12345, 1234)]
+        seeds = [(12345, 1234), (12345, 1234), (777, 888), (12345, 1234), (12345, 1234)]
         mrg = [srv.wait(srv.submit(prd, mdl, cf.NumericalParam(numPath=1 << 14, useSobol=False, seed1=a, seed2=b)))
                for a, b in seeds]
         st0 = srv.stats()
         num = cf.NumericalParam(numPath=1 << 14, precision=cf.FP32)
-        ws = [np.array([1.0, 0.0]), np.array([0.0, 1.0]), np.array([0.5, 2.0]), np.array([1.0, 0.0])]
+        ws = [np.array([1.0, 0.0]), np.array([0.0, 1.0]), np.array([0.5, 2.0]), np.array([1.0, 0.0]),
+              np.array([2.0, -1.0])]
         risk = [srv.wait(srv.submit(prd, mdl, num, aad=True, weights=w)) for w in ws]
         st1 = srv.stats()
+    # captured at the third sighting, replayed after (with other weights too)
     assert st0["graph_captures"] == 1 and st0["graph_replays"] == 1, st0
     assert st1["graph_captures"] == 2 and st1["graph_replays"] == 3, st1
     for (a, b), g in zip(seeds, mrg):
