@@ -58,16 +58,20 @@ PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic_aad_dupire_barrier_fp3
 
 
 def _time_calls(fn, steps, warmup):
+    """Wall time of `steps` calls (nothing else in the timed region), then the
+    path kernel's HIP-event time of each of `steps` further calls."""
     for _ in range(warmup):
         fn()
     cf.synchronize()
-    kernel_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
-        kernel_ms.append(cf.last_kernel_stats()["main_kernel_ms"])
     cf.synchronize()
     wall = time.perf_counter() - t0
+    kernel_ms = []
+    for _ in range(steps):
+        fn()
+        kernel_ms.append(cf.last_kernel_stats()["main_kernel_ms"])
     return wall, kernel_ms
 
 

@@ -1073,13 +1073,11 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                                                                    : ((v >= R(-1) && f == R(0)) ? -v : R(0));
                                         Fb -= babar[b] * others * i2s;
                                     }
-                                } else {
+                                } else if (v > R(kBreached) && F > bp.lo) {
                                     // FP32 (factors at least 1e-30): the others' product
-                                    // is v / f, branch free
-                                    const bool zone = v > R(kBreached) && F > bp.lo;
+                                    // is v / f (a wave with no lane in the zone skips it)
                                     const R f = max((bp.up - F) * bp.i2s, R(1e-30));
-                                    const R d = babar[b] * bp.i2s * recip_mul(v, f);
-                                    Fb -= zone ? d : R(0);
+                                    Fb -= babar[b] * bp.i2s * recip_mul(v, f);
                                 }
                             }
                         }

@@ -107,7 +107,11 @@ enum cf_aad_mode { CF_AAD_CHECKPOINT = 0, CF_AAD_RECOMPUTE = 1 };
 typedef struct cf_num {
     int parallel;            /* accepted for compatibility; the GPU is always parallel */
     int use_sobol;           /* 1: Sobol; 0: mrg32k3a with seed1, seed2 (main.h:51-52) */
-    int num_path;
+    int num_path;            /* 1 .. 2^31 - 1 (NumericalParam::numPath is an int,
+                              * main.h:36); a run of more paths is sharded by
+                              * first_path into calls of at most that many, and
+                              * first_path + num_path stays below 2^32 - 1 (the
+                              * Sobol index range) */
     int seed1;               /* mrg32k3a seeds, converted to unsigned as the reference does */
     int seed2;
     int precision;           /* CF_FP64 or CF_FP32 */
