@@ -566,16 +566,25 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     const int* bk_as_put = reinterpret_cast<const int*>(bkb + P.bk.as_put);
     const int* bk_bar_col = reinterpret_cast<const int*>(bkb + P.bk.bar_col);
     const int* bk_bar_put = reinterpret_cast<const int*>(bkb + P.bk.bar_put);
-    // barrier parameters (strike, B + w, B - w, 1 / 2w) and fixing weights in
-    // the kernel's precision
-    const R* bk_par = reinterpret_cast<const R*>(bkb + (sizeof(R) == 4 ? P.bk.bar_par_f : P.bk.bar_par));
+    // fixing weights in the kernel's precision
     const R* bk_ginv = reinterpret_cast<const R*>(bkb + (sizeof(R) == 4 ? P.bk.grp_inv_f : P.bk.grp_inv));
-    // barrier b's (strike, B + w, B - w, 1 / 2w): one aligned 16-byte (FP32)
-    // or two (FP64) LDS reads
+    // barrier b's (strike, B + w, B - w, 1 / 2w) in the kernel's precision
     struct alignas(16) BarPar {
         R k, up, lo, i2s;
     };
-    auto bar_par = [&](int b) { return reinterpret_cast<const BarPar*>(bk_par)[b]; };
+    // (read from the global copy of the book tables through the scalar cache:
+    // wave-uniform values in scalar registers, no LDS read per barrier and step)
+    using CBarPar = const __attribute__((address_space(4))) BarPar;
+    CBarPar* bk_par_c = (CBarPar*)(static_cast<const unsigned char*>(P.book) +
+                                   (sizeof(R) == 4 ? P.bk.bar_par_f : P.bk.bar_par));
+    auto bar_par = [&](int b) -> BarPar {
+        BarPar r;
+        r.k = bk_par_c[b].k;
+        r.up = bk_par_c[b].up;
+        r.lo = bk_par_c[b].lo;
+        r.i2s = bk_par_c[b].i2s;
+        return r;
+    };
     const R bk_lo = R(P.book_lo);   // the lowest B - w (rounding keeps it the lowest)
     // Barrier smoothing factors (B + w - F) / 2w: FP64 tracks exactly-zero
     // factors (F = B + w) as the tape does.  FP32 keeps B + w - F at least
