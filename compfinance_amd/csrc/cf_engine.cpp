@@ -836,8 +836,7 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
     const int forced = forced_layout();
     auto lds_bytes = [&](int copies) {
         return cfk::smem_layout(md.n_steps, md.n_spots, int(pd.timeline.size()), pd.n_pay, waves, fp64 ? 8 : 4, dup,
-                                pd.kind == CF_PRODUCT_EUROPEANS, adjoint, P->cfg.lean, dup ? P->bt.nb : 0,
-                                book ? P->book.L.bytes : 0, copies)
+                                pd.kind == CF_PRODUCT_EUROPEANS, adjoint, P->cfg.lean, dup ? P->bt.nb : 0, copies)
             .total;
     };
     P->ivbar_copies = 1;

@@ -195,7 +195,7 @@ struct ReducePack {
 
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
-    size_t lo, hi, hi_tail, step_sample, invn, knots, bucket, iv, drift, stdv, samples, pay_first, pay_strike, book, acc,
+    size_t lo, hi, hi_tail, step_sample, invn, knots, bucket, iv, drift, stdv, samples, pay_first, pay_strike, acc,
         ivbar, total;
 };
 
@@ -359,7 +359,7 @@ CF_HD inline int ivbar_copy_stride(int n_steps, int n_spots) { return n_steps * 
 
 CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int n_pay, int waves,
                                     int real_bytes, bool dupire, bool europeans, bool adjoint, bool lean,
-                                    int n_buckets, int book_bytes, int ivbar_copies)
+                                    int n_buckets, int ivbar_copies)
 {
     SmemLayout L{};
     size_t o = 0;
@@ -381,7 +381,6 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.samples = o;     o = cf_align16(o + (dupire ? 0 : size_t(n_samples) * 3 * 8));
     L.pay_first = o;   o = cf_align16(o + (europeans ? size_t(n_samples + 1) * 4 : 0));
     L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
-    L.book = o;        o = cf_align16(o + size_t(book_bytes));
     L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
     L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * ivbar_copy_stride(n_steps, n_spots) * 8
                                                              : 0));

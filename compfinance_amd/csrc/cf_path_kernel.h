@@ -78,6 +78,17 @@ __device__ __forceinline__ void wave_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Read-only global data read with wave-uniform indices: a pointer into the
+// constant address space, which the compiler reads with scalar loads through
+// the scalar cache into scalar registers.
+template <typename T>
+using CPtr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ CPtr<T> cptr(const void* p)
+{
+    return (CPtr<T>)(p);
+}
+
 // a / b in FP64 from the hardware reciprocal refined by two Newton steps and a
 // residual correction (within an ulp of the IEEE quotient)
 __device__ __forceinline__ double fast_div(double a, double b)
@@ -417,7 +428,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     const int stride = out_stride(npay);
     constexpr bool BOOK = PRD == PRD_BOOK;
     const SmemLayout L = smem_layout(n, m, ns, npay, waves, int(sizeof(R)), DUP, EUS, AD, LEAN, P.n_buckets,
-                                     BOOK ? P.bk.bytes : 0, P.ivbar_copies);
+                                     P.ivbar_copies);
 
     uint32_t* lo_tab = reinterpret_cast<uint32_t*>(smem + L.lo);
     uint32_t* hi_all = reinterpret_cast<uint32_t*>(smem + L.hi);
@@ -475,11 +486,6 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     if constexpr (EUS) {
         for (int e = tid; e <= ns; e += nthr) pay_first[e] = P.pay_first[e];
         for (int e = tid; e < npay; e += nthr) pay_strike[e] = P.pay_strike[e];
-    }
-    if constexpr (BOOK) {
-        const uint32_t* src = static_cast<const uint32_t*>(P.book);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(smem + L.book);
-        for (int e = tid; e < P.bk.bytes / 4; e += nthr) dst[e] = src[e];
     }
     for (int e = tid; e < waves * stride; e += nthr) acc[e] = 0.0;
     if constexpr (AD && DUP)
@@ -548,35 +554,41 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     // per-lane parameter adjoints, summed over the lane's paths
     double padj[kMaxParamAdj] = {0.0, 0.0, 0.0, 0.0};
 
-    // book tables in LDS
-    const unsigned char* bkb = smem + L.book;
-    const double* bk_van_strike = reinterpret_cast<const double*>(bkb + P.bk.van_strike);
-    const int* bk_van_col = reinterpret_cast<const int*>(bkb + P.bk.van_col);
-    // per sample, one aligned 16-byte LDS read: barriers observing (bits
-    // 0-15) and maturing (16-31), schedules fixing (bits 0-3) and ending
-    // (4-7), and the sample's vanillas [van_lo, van_hi)
+    // Book tables and aggregation weights: wave-uniform values, read from
+    // global memory through the scalar cache (constant address space) into
+    // scalar registers, not staged in LDS
+    const unsigned char* bkb = static_cast<const unsigned char*>(P.book);
+    CPtr<double> bk_van_strike = cptr<double>(bkb + P.bk.van_strike);
+    CPtr<int> bk_van_col = cptr<int>(bkb + P.bk.van_col);
+    // per sample: barriers observing (bits 0-15) and maturing (16-31),
+    // schedules fixing (bits 0-3) and ending (4-7), and the sample's vanillas
+    // [van_lo, van_hi)
     struct alignas(16) SampleRow {
         uint32_t bars, grps;
         int van_lo, van_hi;
     };
-    const SampleRow* bk_rows = reinterpret_cast<const SampleRow*>(bkb + P.bk.masks);
-    const int* bk_grp_first = reinterpret_cast<const int*>(bkb + P.bk.grp_first);
-    const double* bk_as_strike = reinterpret_cast<const double*>(bkb + P.bk.as_strike);
-    const int* bk_as_col = reinterpret_cast<const int*>(bkb + P.bk.as_col);
-    const int* bk_as_put = reinterpret_cast<const int*>(bkb + P.bk.as_put);
-    const int* bk_bar_col = reinterpret_cast<const int*>(bkb + P.bk.bar_col);
-    const int* bk_bar_put = reinterpret_cast<const int*>(bkb + P.bk.bar_put);
+    CPtr<SampleRow> bk_rows_c = cptr<SampleRow>(bkb + P.bk.masks);
+    auto bk_rows = [&](int s) -> SampleRow {
+        SampleRow r;
+        r.bars = bk_rows_c[s].bars;
+        r.grps = bk_rows_c[s].grps;
+        r.van_lo = bk_rows_c[s].van_lo;
+        r.van_hi = bk_rows_c[s].van_hi;
+        return r;
+    };
+    CPtr<int> bk_grp_first = cptr<int>(bkb + P.bk.grp_first);
+    CPtr<double> bk_as_strike = cptr<double>(bkb + P.bk.as_strike);
+    CPtr<int> bk_as_col = cptr<int>(bkb + P.bk.as_col);
+    CPtr<int> bk_as_put = cptr<int>(bkb + P.bk.as_put);
+    CPtr<int> bk_bar_col = cptr<int>(bkb + P.bk.bar_col);
+    CPtr<int> bk_bar_put = cptr<int>(bkb + P.bk.bar_put);
     // fixing weights in the kernel's precision
-    const R* bk_ginv = reinterpret_cast<const R*>(bkb + (sizeof(R) == 4 ? P.bk.grp_inv_f : P.bk.grp_inv));
+    CPtr<R> bk_ginv = cptr<R>(bkb + (sizeof(R) == 4 ? P.bk.grp_inv_f : P.bk.grp_inv));
     // barrier b's (strike, B + w, B - w, 1 / 2w) in the kernel's precision
     struct alignas(16) BarPar {
         R k, up, lo, i2s;
     };
-    // (read from the global copy of the book tables through the scalar cache:
-    // wave-uniform values in scalar registers, no LDS read per barrier and step)
-    using CBarPar = const __attribute__((address_space(4))) BarPar;
-    CBarPar* bk_par_c = (CBarPar*)(static_cast<const unsigned char*>(P.book) +
-                                   (sizeof(R) == 4 ? P.bk.bar_par_f : P.bk.bar_par));
+    CPtr<BarPar> bk_par_c = cptr<BarPar>(bkb + (sizeof(R) == 4 ? P.bk.bar_par_f : P.bk.bar_par));
     auto bar_par = [&](int b) -> BarPar {
         BarPar r;
         r.k = bk_par_c[b].k;
@@ -585,6 +597,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
         r.i2s = bk_par_c[b].i2s;
         return r;
     };
+    CPtr<double> wts = cptr<double>(P.weights);
     const R bk_lo = R(P.book_lo);   // the lowest B - w (rounding keeps it the lowest)
     // Barrier smoothing factors (B + w - F) / 2w: FP64 tracks exactly-zero
     // factors (F = B + w) as the tape does.  FP32 keeps B + w - F at least
@@ -653,7 +666,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
             const double vs = wave_sum(v), vq = wave_sum(v * v);
             if (lane == kSumLane) { accw[col] += vs; accw[npay + col] += vq; }
             if (P.per_path && active) P.per_path[pidx * npay + col] = v;
-            if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[col]) * pay;
+            if constexpr (AD) ps.aggp = ps.aggp + R(wts[col]) * pay;
         };
 
         // observation of product sample s with spot S (and for Black-Scholes
@@ -702,9 +715,9 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 const double vs = wave_sum(v), vq = wave_sum(v * v);
                 if (lane == kSumLane) { accw[0] += vs; accw[npay] += vq; }
                 if (P.per_path && active) P.per_path[pidx * npay] = v;
-                if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[0]) * pay;
+                if constexpr (AD) ps.aggp = ps.aggp + R(wts[0]) * pay;
             } else if constexpr (PRD == PRD_BOOK) {
-                const SampleRow sm = bk_rows[s];
+                const SampleRow sm = bk_rows(s);
                 // vanilla calls at this sample (European / Europeans parts)
                 for (int k = sm.van_lo; k < sm.van_hi; ++k) {
                     const R kk = R(bk_van_strike[k]);
@@ -754,7 +767,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                             // weight x vanilla: the adjoint of the survival
                             // factor (the ITEM kernels keep the vanilla and
                             // weigh it before the reverse sweep)
-                            if constexpr (AD) babar[b] = ITEM ? euro : R(P.weights[col]) * euro;
+                            if constexpr (AD) babar[b] = ITEM ? euro : R(wts[col]) * euro;
                         }
                     }
                 }
@@ -777,7 +790,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                                 const int col = bk_as_col[a];
                                 emit(col, intrinsic / numv);
                                 if constexpr (AD && !ITEM) {
-                                    const R w = R(P.weights[col]);
+                                    const R w = R(wts[col]);
                                     const R d = put ? (k > avg ? R(-1) : R(0)) : (avg > k ? R(1) : R(0));
                                     dsum = dsum + w * d;
                                     qsum = qsum + w * intrinsic;
@@ -803,7 +816,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     const double vs = wave_sum(v), vq = wave_sum(v * v);
                     if (lane == kSumLane) { accw[k] += vs; accw[npay + k] += vq; }
                     if (P.per_path && active) P.per_path[pidx * npay + k] = v;
-                    if constexpr (AD) ps.aggp = ps.aggp + R(P.weights[k]) * pay;
+                    if constexpr (AD) ps.aggp = ps.aggp + R(wts[k]) * pay;
                 }
             }
         };
@@ -943,7 +956,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 const double a0 = wave_sum(v0), a1 = wave_sum(v1), b0 = wave_sum(v0 * v0), b1 = wave_sum(v1 * v1);
                 if (lane == kSumLane) { accw[0] += a0; accw[1] += a1; accw[npay] += b0; accw[npay + 1] += b1; }
                 if (P.per_path && active) { P.per_path[pidx * 2] = v0; P.per_path[pidx * 2 + 1] = v1; }
-                if constexpr (AD) ps.aggp = R(P.weights[0]) * p0 + R(P.weights[1]) * p1;
+                if constexpr (AD) ps.aggp = R(wts[0]) * p0 + R(wts[1]) * p1;
             }
         }
         if constexpr (AD) {
@@ -984,7 +997,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 // intrinsic values (the numeraire adjoint)
 #pragma unroll
                 for (int b = 0; b < NBAR; ++b)
-                    if (b < P.bk.n_bar) babar[b] = R(P.weights[bk_bar_col[b]]) * babar[b];
+                    if (b < P.bk.n_bar) babar[b] = R(wts[bk_bar_col[b]]) * babar[b];
 #pragma unroll
                 for (int g = 0; g < NGRP; ++g) {
                     if (g >= P.bk.n_grp) continue;
@@ -994,7 +1007,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         const R k = R(bk_as_strike[a]);
                         const bool put = bk_as_put[a] != 0;
                         const R intrinsic = put ? (k - avg > R(0) ? k - avg : R(0)) : (avg - k > R(0) ? avg - k : R(0));
-                        const R w = R(P.weights[bk_as_col[a]]);
+                        const R w = R(wts[bk_as_col[a]]);
                         const R d = put ? (k > avg ? R(-1) : R(0)) : (avg > k ? R(1) : R(0));
                         dsum = dsum + w * d;
                         qsum = qsum + w * intrinsic;
@@ -1005,8 +1018,8 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
             }
 
             // adjoints of the forward, numeraire and discount at sample s
-            const R w0 = R(P.weights[0]);
-            const R w1 = npay > 1 ? R(P.weights[1]) : R(0);
+            const R w0 = R(wts[0]);
+            const R w1 = npay > 1 ? R(wts[1]) : R(0);
             const R alive_eff = ps.dead ? R(0) : ps.alive;
             const R euro_bar = w0 * alive_eff + w1;       // barrier: adjoint of the vanilla
             const R alive_bar = w0 * ps.euro;             // barrier: adjoint of alive
@@ -1038,10 +1051,10 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     db = w0 * intrinsic / numv;
                     nb = -w0 * intrinsic * discv / (numv * numv);
                 } else if constexpr (PRD == PRD_BOOK) {
-                    const SampleRow sm = bk_rows[s];
+                    const SampleRow sm = bk_rows(s);
                     for (int k = sm.van_lo; k < sm.van_hi; ++k) {
                         const R kk = R(bk_van_strike[k]);
-                        const R wk = R(P.weights[bk_van_col[k]]);
+                        const R wk = R(wts[bk_van_col[k]]);
                         if (F > kk) {
                             Fb += wk / numv;
                             nb += -wk * (F - kk) / (numv * numv);
@@ -1063,7 +1076,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                                                         : (F - k > R(0) ? F - k : R(0));
                                 const R dmax = put ? (k > F ? R(-1) : R(0)) : (F > k ? R(1) : R(0));
                                 const R alive = bst[b] > R(0) ? bst[b] : R(0);
-                                const R euro_bar = R(P.weights[col]) * alive + R(P.weights[col + 1]);
+                                const R euro_bar = R(wts[col]) * alive + R(wts[col + 1]);
                                 Fb += euro_bar * dmax / numv;
                                 nb += -euro_bar * (intrinsic / numv) / numv;
                             }
@@ -1103,7 +1116,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 } else {
                     for (int k = pay_first[s]; k < pay_first[s + 1]; ++k) {
                         const R kk = R(pay_strike[k]);
-                        const R wk = R(P.weights[k]);
+                        const R wk = R(wts[k]);
                         if (F > kk) {
                             Fb += wk / numv;
                             nb += -wk * (F - kk) / (numv * numv);

@@ -27,9 +27,9 @@ static void region(size_t off, size_t used, size_t next, size_t total)
 // The kernel's largest index of each region (cf_path_kernel.h), for a block of
 // `waves` waves on n steps, m knots, ns samples, npay payoffs.
 static void kernel_extents(int n, int m, int ns, int npay, int waves, int rb, bool dup, bool eus, bool ad, bool lean,
-                           int nb, int book_bytes, int copies)
+                           int nb, int copies)
 {
-    const SmemLayout L = smem_layout(n, m, ns, npay, waves, rb, dup, eus, ad, lean, nb, book_bytes, copies);
+    const SmemLayout L = smem_layout(n, m, ns, npay, waves, rb, dup, eus, ad, lean, nb, copies);
     const size_t T = L.total;
     const int mp = lv_cols(m);
     // stage loops: lo_tab[e], e < n * 8 (LEAN) or e < 4 ceil(n / 4) * 64 (FULL,
@@ -55,8 +55,7 @@ static void kernel_extents(int n, int m, int ns, int npay, int waves, int rb, bo
     region(L.stdv, dup ? 0 : size_t(n) * rb, L.samples, T);
     region(L.samples, dup ? 0 : size_t(3 * ns) * 8, L.pay_first, T);
     region(L.pay_first, eus ? size_t(ns + 1) * 4 : 0, L.pay_strike, T);
-    region(L.pay_strike, eus ? size_t(npay) * 8 : 0, L.book, T);
-    region(L.book, size_t(book_bytes), L.acc, T);
+    region(L.pay_strike, eus ? size_t(npay) * 8 : 0, L.acc, T);
     region(L.acc, size_t(waves) * out_stride(npay) * 8, L.ivbar, T);
     // ivbar: copy c at c * ivbar_copy_stride, node (i, j + 1) with j <= m
     // (padded column m + 1), and the end columns 0 and m + 1 on write-out
@@ -92,10 +91,10 @@ int main()
     for (int real = 4; real <= 8; real += 4) {
         for (int lean = 0; lean < 2; ++lean) {
             for (int copies = 1; copies <= 2; ++copies) {
-                const SmemLayout L = smem_layout(156, 30, 157, 55, 16, real, true, true, true, lean != 0, 64, 1000,
+                const SmemLayout L = smem_layout(156, 30, 157, 55, 16, real, true, true, true, lean != 0, 64,
                                                  copies);
                 const size_t off[] = {L.lo, L.hi, L.hi_tail, L.step_sample, L.invn, L.knots, L.bucket, L.iv,
-                                      L.drift, L.stdv, L.samples, L.pay_first, L.pay_strike, L.book, L.acc,
+                                      L.drift, L.stdv, L.samples, L.pay_first, L.pay_strike, L.acc,
                                       L.ivbar, L.total};
                 for (size_t o : off) misaligned += (o % 16) ? 1 : 0;
             }
@@ -112,7 +111,7 @@ int main()
                                 for (int m : {1, 2, 30, 60, 255})
                                     for (int waves : {1, 8, 16})
                                         kernel_extents(n, m, n + 1, eus ? 31 : 2, waves, rb, dup != 0, eus != 0,
-                                                       ad != 0, lean != 0, 2 * m + 3, eus ? 0 : 1104, copies);
+                                                       ad != 0, lean != 0, 2 * m + 3, copies);
     // FP32 inverse normal: the row the kernel reads, (bits(float(xs)) >> 18 &
     // 0x3ffc) / 4 - kInvNormalRowBias, is inside the table for every xs in
     // [1, 2^31] (each binade's ends and every row boundary, from both sides)
@@ -150,9 +149,9 @@ int main()
                 if (((e >> 8) << 2) + (e & 3) != d || ((e >> 2) & 63) != r) ++il_bad;
                 if (e >= ckpt_steps(n, 4) * kWave) ++il_bad;
             }
-    fp32_c3 = smem_layout(156, 30, 157, 2, 16, 4, true, false, true, false, 64, 0, 2).total;
-    fp64_c3 = smem_layout(156, 30, 157, 2, 16, 8, true, false, true, false, 64, 0, 1).total;
-    lean_60x60 = smem_layout(261, 60, 262, 2, 16, 8, true, false, true, true, 64, 0, 1).total;
+    fp32_c3 = smem_layout(156, 30, 157, 2, 16, 4, true, false, true, false, 64, 2).total;
+    fp64_c3 = smem_layout(156, 30, 157, 2, 16, 8, true, false, true, false, 64, 1).total;
+    lean_60x60 = smem_layout(261, 60, 262, 2, 16, 8, true, false, true, true, 64, 1).total;
     copy_offset_mod128 = int((size_t(ivbar_copy_stride(156, 30)) * 8) % 128);
     std::printf("{\"cases\": %ld, \"wrong\": %ld, \"merged\": %ld, \"misaligned\": %ld, \"fp32_c3\": %zu, "
                 "\"fp64_c3\": %zu, \"lean_60x60_5y_fp64\": %zu, \"copy_offset_mod128\": %d, \"regions\": %ld, "
