@@ -26,7 +26,7 @@ __all__ = [
     "putBlackScholes", "putDupire", "putEuropean", "putBarrier", "putEuropeans", "putAsian", "putBook",
     "value", "AADriskOne", "AADriskAggregate", "AADriskMulti", "dupireAADRisk", "clearStore",
     "mcParallelSimulAADMulti",
-    "CfError", "lib", "last_kernel_stats", "LIB_PATH", "Server", "dupireCalib", "dupireSuperbucket",
+    "CfError", "lib", "last_kernel_stats", "LIB_PATH", "Server", "dupireCalib", "dupireCalibView", "dupireSuperbucket", "dupireSuperbucketBump",
     "sobol_points",
 ]
 
@@ -828,6 +828,9 @@ def _calib_lib():
         L.cf_dupire_superbucket.argtypes = [P(cf_merton_ivs), ctypes.c_double, ctypes.c_char_p,
                                             P(ctypes.c_char_p), d, i, d, i, ctypes.c_double, d, i, ctypes.c_double,
                                             d, i, d, i, P(cf_num), d, d, d]
+        L.cf_dupire_superbucket_bump.argtypes = L.cf_dupire_superbucket.argtypes
+        L.cf_dupire_calib_view.argtypes = [P(cf_merton_ivs), d, i, ctypes.c_double, d, i, ctypes.c_double, d, i, d,
+                                           i, d, d, d, d, P(i), P(i), i]
         L._calib_sigs = True
     return L
 
@@ -862,6 +865,43 @@ def dupireSuperbucket(spot, maxDt, productId, notionals, inclSpots, maxDs, inclT
                                    len(labs), _dptr(s), len(s), float(maxDs), _dptr(t), len(t), float(maxDtVol),
                                    _dptr(k), len(k), _dptr(m), len(m), ctypes.byref(n), _dptr(val), _dptr(delta),
                                    _dptr(vega)))
+    return OrderedDict(value=float(val[0]), delta=float(delta[0]), strikes=k, mats=m,
+                       vega=vega.reshape(len(k), len(m)))
+
+
+def dupireCalibView(inclSpots, maxDs, inclTimes, maxDt, spot, vol, jmpIntens, jmpAverage, jmpStd, strikes, mats,
+                    spreads):
+    """dupireCalib with a risk view (mcMdlDupire.h:337-388): local vols of the
+    Merton surface plus implied-vol spreads [strikes x mats]."""
+    L = _calib_lib()
+    ivs = cf_merton_ivs(spot, vol, jmpIntens, jmpAverage, jmpStd)
+    s, t, k, m = _f64(inclSpots), _f64(inclTimes), _f64(strikes), _f64(mats)
+    sp = _f64(spreads).reshape(len(k), len(m))
+    ns, nt = ctypes.c_int(), ctypes.c_int()
+    args = (ctypes.byref(ivs), _dptr(s), len(s), float(maxDs), _dptr(t), len(t), float(maxDt), _dptr(k), len(k),
+            _dptr(m), len(m), _dptr(sp))
+    _check(L.cf_dupire_calib_view(*args, None, None, None, ctypes.byref(ns), ctypes.byref(nt), 0))
+    S, T, V = np.zeros(ns.value), np.zeros(nt.value), np.zeros(ns.value * nt.value)
+    _check(L.cf_dupire_calib_view(*args, _dptr(S), _dptr(T), _dptr(V), ctypes.byref(ns), ctypes.byref(nt), len(V)))
+    return OrderedDict(spots=S, times=T, lVols=V.reshape(len(S), len(T)))
+
+
+def dupireSuperbucketBump(spot, maxDt, productId, notionals, inclSpots, maxDs, inclTimes, maxDtVol, strikes, mats,
+                          vol, jmpIntens, jmpAverage, jmpStd, num):
+    """dupireSuperbucketBump (main.h:575-696): the superbucket by bump and
+    reprice on the GPU (1e-8 spot bump, 1e-5 spread bumps, recalibrating the
+    local vols for each)."""
+    L = _calib_lib()
+    ivs = cf_merton_ivs(spot, vol, jmpIntens, jmpAverage, jmpStd)
+    s, t, k, m = _f64(inclSpots), _f64(inclTimes), _f64(strikes), _f64(mats)
+    labs = list(notionals.keys())
+    nots = _f64([notionals[x] for x in labs]) if labs else np.zeros(1)
+    val, delta, vega = np.zeros(1), np.zeros(1), np.zeros(len(k) * len(m))
+    n = num.descriptor()
+    _check(L.cf_dupire_superbucket_bump(ctypes.byref(ivs), float(maxDt), _id(productId), _label_array(labs),
+                                        _dptr(nots), len(labs), _dptr(s), len(s), float(maxDs), _dptr(t), len(t),
+                                        float(maxDtVol), _dptr(k), len(k), _dptr(m), len(m), ctypes.byref(n),
+                                        _dptr(val), _dptr(delta), _dptr(vega)))
     return OrderedDict(value=float(val[0]), delta=float(delta[0]), strikes=k, mats=m,
                        vega=vega.reshape(len(k), len(m)))
 

@@ -148,10 +148,40 @@ int step_weights(const vector<double>& xs, double x0, int idx[2], double w[2])
     return 2;
 }
 
-// Risk view grid (ivs.h:26-81): strikes x maturities of spreads, all 0.
+// Smooth-step interpolation with flat extrapolation (interp.h:26-63), in the
+// reference's order of operations.
+double smooth_interp(const vector<double>& xs, const double* ys, double x0)
+{
+    const auto it = std::upper_bound(xs.begin(), xs.end(), x0);
+    if (it == xs.end()) return ys[xs.size() - 1];
+    if (it == xs.begin()) return ys[0];
+    const size_t n = size_t(it - xs.begin()) - 1;
+    const double t = (x0 - xs[n]) / (xs[n + 1] - xs[n]);
+    return ys[n] + (ys[n + 1] - ys[n]) * t * t * (3.0 - 2 * t);
+}
+
+// Risk view grid (ivs.h:26-81): strikes x maturities of implied-vol spreads
+// (strike major; empty: all 0).
 struct View {
     vector<double> strikes, mats;
+    vector<double> spreads;
     bool empty() const { return strikes.empty(); }
+    // spread(strike, mat): interp2D<true> (interp.h:66-110), the smooth step
+    // over maturities within two strike rows, then across them
+    double spread(double strike, double mat) const
+    {
+        if (empty() || spreads.empty()) return 0.0;
+        const size_t n = strikes.size(), m = mats.size();
+        const auto it = std::upper_bound(strikes.begin(), strikes.end(), strike);
+        const size_t n2 = size_t(it - strikes.begin());
+        if (n2 == n) return smooth_interp(mats, &spreads[(n2 - 1) * m], mat);
+        if (n2 == 0) return smooth_interp(mats, &spreads[0], mat);
+        const size_t n1 = n2 - 1;
+        const double z1 = smooth_interp(mats, &spreads[n1 * m], mat);
+        const double z2 = smooth_interp(mats, &spreads[n2 * m], mat);
+        const double t = (strike - strikes[n1]) / (strikes[n2] - strikes[n1]);
+        return z1 + (z2 - z1) * t * t * (3.0 - 2 * t);
+    }
     // sparse gradient of spread(strike, mat) (interp2D<true>, interp.h:66-110)
     void grad(double strike, double mat, std::map<int, double>& g, double scale) const
     {
@@ -172,13 +202,17 @@ double local_vol(const Merton& ivs, double strike, double mat, const View* view,
         double k, t;
     } pts[5] = {{strike, mat}, {strike, mat - 1.0e-04}, {strike, mat + 1.0e-04}, {strike - 1.0e-04, mat},
                 {strike + 1.0e-04, mat}};
+    // the implied vol of each point plus the view's spread there (ivs.h:100-110)
     CallV c[5];
-    for (int p = 0; p < 5; ++p) c[p] = black_scholes(ivs.spot, pts[p].k, ivs.implied(pts[p].k, pts[p].t), pts[p].t);
+    for (int p = 0; p < 5; ++p)
+        c[p] = black_scholes(ivs.spot, pts[p].k,
+                             ivs.implied(pts[p].k, pts[p].t) + (view ? view->spread(pts[p].k, pts[p].t) : 0.0),
+                             pts[p].t);
     const double ct = (c[2].price - c[1].price) * 0.5e04;
     const double ckk = (c[3].price + c[4].price - 2.0 * c[0].price) * 1.0e08;
     const double r = 2.0 * ct / ckk;
     const double lv = std::sqrt(r) / strike;
-    if (view && d) {
+    if (view && d && !view->empty()) {
         // d lv = (1 / strike) (0.5 / sqrt(r)) (2 d ct / ckk - 2 ct d ckk / ckk^2)
         const double a = (0.5 / std::sqrt(r)) / strike;
         const double dct = a * 2.0 / ckk;
@@ -200,7 +234,7 @@ struct Calib {
 // dupireCalib (mcMdlDupire.h:337-388) with the per-maturity cut of
 // dupireCalibMaturity (mcMdlDupire.h:286-333)
 Calib calibrate(const Merton& ivs, const vector<double>& inclSpots, double maxDs, const vector<double>& inclTimes,
-                double maxDt, const View* view)
+                double maxDt, const View* view, bool derivatives = true)
 {
     Calib C;
     C.spots = fill_data(inclSpots, maxDs, 0.01, {});
@@ -208,7 +242,7 @@ Calib calibrate(const Merton& ivs, const vector<double>& inclSpots, double maxDs
     const int nS = int(C.spots.size()), nT = int(C.times.size());
     C.lvols.assign(size_t(nS) * nT, 0.0);
     C.node.assign(size_t(nS) * nT, 0);
-    if (view) C.dlv.resize(size_t(nS) * nT);
+    if (view && derivatives) C.dlv.resize(size_t(nS) * nT);
     for (int j = 0; j < nT; ++j) {
         const double mat = C.times[j];
         const double atm = black_scholes(ivs.spot, ivs.spot, ivs.implied(ivs.spot, mat), mat).price;
@@ -219,7 +253,7 @@ Calib calibrate(const Merton& ivs, const vector<double>& inclSpots, double maxDs
         while (ih >= 0 && C.spots[ih] > ivs.spot + 2.5 * sd) --ih;
         for (int i = il; i <= ih; ++i) {
             const size_t e = size_t(i) * nT + j;
-            C.lvols[e] = local_vol(ivs, C.spots[i], mat, view, view ? &C.dlv[e] : nullptr);
+            C.lvols[e] = local_vol(ivs, C.spots[i], mat, view, C.dlv.empty() ? nullptr : &C.dlv[e]);
             C.node[e] = i;
         }
         for (int i = 0; i < il; ++i) {
@@ -269,6 +303,30 @@ int cf_dupire_calib(const cf_merton_ivs* ivs, const double* incl_spots, int n_in
     });
 }
 
+int cf_dupire_calib_view(const cf_merton_ivs* ivs, const double* incl_spots, int n_incl_spots, double max_ds,
+                         const double* incl_times, int n_incl_times, double max_dt, const double* strikes,
+                         int n_strikes, const double* mats, int n_mats, const double* spreads, double* spots,
+                         double* times, double* lvols, int* n_spots, int* n_times, int cap)
+{
+    return guarded([&] {
+        if (!ivs || !n_spots || !n_times || !spreads) fail(CF_ERR_ARG, "null argument");
+        if (!(max_ds > 0.0) || !(max_dt > 0.0)) fail(CF_ERR_ARG, "grid spacing must be positive");
+        const Merton m{ivs->spot, ivs->vol, ivs->jmp_intens, ivs->jmp_average, ivs->jmp_std};
+        View view{vec(strikes, n_strikes), vec(mats, n_mats), {}};
+        view.spreads = vec(spreads, n_strikes * n_mats);
+        const Calib C = calibrate(m, vec(incl_spots, n_incl_spots), max_ds, vec(incl_times, n_incl_times), max_dt,
+                                  &view, false);
+        *n_spots = int(C.spots.size());
+        *n_times = int(C.times.size());
+        if (int(C.lvols.size()) <= cap) {
+            if (spots) std::copy(C.spots.begin(), C.spots.end(), spots);
+            if (times) std::copy(C.times.begin(), C.times.end(), times);
+            if (lvols) std::copy(C.lvols.begin(), C.lvols.end(), lvols);
+        }
+        return CF_OK;
+    });
+}
+
 int cf_superbucket_vega(const cf_merton_ivs* ivs, const double* incl_spots, int n_incl_spots, double max_ds,
                         const double* incl_times, int n_incl_times, double max_dt_vol, const double* strikes,
                         int n_strikes, const double* mats, int n_mats, const double* microbucket, int n_micro,
@@ -277,7 +335,7 @@ int cf_superbucket_vega(const cf_merton_ivs* ivs, const double* incl_spots, int 
     return guarded([&] {
         if (!ivs || !microbucket || !vega) fail(CF_ERR_ARG, "null argument");
         const Merton m{ivs->spot, ivs->vol, ivs->jmp_intens, ivs->jmp_average, ivs->jmp_std};
-        const View view{vec(strikes, n_strikes), vec(mats, n_mats)};
+        const View view{vec(strikes, n_strikes), vec(mats, n_mats), {}};
         const Calib C = calibrate(m, vec(incl_spots, n_incl_spots), max_ds, vec(incl_times, n_incl_times),
                                   max_dt_vol, &view);
         const int nS = int(C.spots.size()), nT = int(C.times.size());

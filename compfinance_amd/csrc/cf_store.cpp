@@ -208,4 +208,29 @@ int cf_dupire_aad_risk(const char* model_id, const char* product_id, const char*
 
 void cf_store_clear(void) { cfamd::clearStores(); }
 
+int cf_dupire_superbucket_bump(const cf_merton_ivs* ivs, double max_dt, const char* product_id,
+                               const char* const* labels, const double* notionals, int n,
+                               const double* incl_spots, int n_incl_spots, double max_ds,
+                               const double* incl_times, int n_incl_times, double max_dt_vol,
+                               const double* strikes, int n_strikes, const double* mats, int n_mats,
+                               const cf_num* num, double* value, double* delta, double* vega)
+{
+    return store_call([&] {
+        if (!ivs || !value || !delta || !vega) throw std::runtime_error("null argument");
+        if (n_incl_spots < 1 || n_incl_times < 1 || n_strikes < 1 || n_mats < 1 || !incl_spots || !incl_times ||
+            !strikes || !mats)
+            throw std::runtime_error("empty grid input");
+        auto r = cfamd::dupireSuperbucketBump(
+            ivs->spot, max_dt, id_of(product_id), notionals_of(labels, notionals, n),
+            std::vector<double>(incl_spots, incl_spots + n_incl_spots), max_ds,
+            std::vector<double>(incl_times, incl_times + n_incl_times), max_dt_vol,
+            std::vector<double>(strikes, strikes + n_strikes), std::vector<double>(mats, mats + n_mats), ivs->vol,
+            ivs->jmp_intens, ivs->jmp_average, ivs->jmp_std, to_num(num));
+        *value = r.value;
+        *delta = r.delta;
+        std::copy(r.vega.begin(), r.vega.end(), vega);
+        return CF_OK;
+    });
+}
+
 }  // extern "C"

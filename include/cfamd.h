@@ -252,6 +252,15 @@ int cf_dupire_calib(const cf_merton_ivs* ivs, const double* incl_spots, int n_in
                     const double* incl_times, int n_incl_times, double max_dt, double* spots, double* times,
                     double* lvols, int* n_spots, int* n_times, int cap);
 
+/* dupireCalib with a risk view (mcMdlDupire.h:337-388, ivs.h:26-138): the
+ * local vols of the Merton surface plus implied-vol spreads on strikes x mats
+ * (spreads[n_strikes * n_mats], strike major; interp2D<true> between them,
+ * interp.h:66-110).  Outputs as cf_dupire_calib. */
+int cf_dupire_calib_view(const cf_merton_ivs* ivs, const double* incl_spots, int n_incl_spots, double max_ds,
+                         const double* incl_times, int n_incl_times, double max_dt, const double* strikes,
+                         int n_strikes, const double* mats, int n_mats, const double* spreads, double* spots,
+                         double* times, double* lvols, int* n_spots, int* n_times, int cap);
+
 /* The chain step of dupireSuperbucket (main.h:536-566): risks to the risk
  * view of implied-vol spreads on strikes x mats (vega, strike major) from the
  * local-vol risks `microbucket` (n_spots x n_times of the calibration above,
@@ -272,6 +281,18 @@ int cf_dupire_superbucket(const cf_merton_ivs* ivs, double max_dt, const char* p
                           const double* incl_times, int n_incl_times, double max_dt_vol,
                           const double* strikes, int n_strikes, const double* mats, int n_mats,
                           const cf_num* num, double* value, double* delta, double* vega);
+
+/* dupireSuperbucketBump (main.h:575-696): the superbucket by bump and
+ * reprice on the GPU (FP64 or FP32 as num asks): value, delta by a 1e-8 spot
+ * bump, and vega[n_strikes * n_mats] (strike major), each risk-view spread
+ * bumped by 1e-5 with the local vols recalibrated.  The product is the stored
+ * product_id; labels/notionals as cf_dupire_superbucket. */
+int cf_dupire_superbucket_bump(const cf_merton_ivs* ivs, double max_dt, const char* product_id,
+                               const char* const* labels, const double* notionals, int n,
+                               const double* incl_spots, int n_incl_spots, double max_ds,
+                               const double* incl_times, int n_incl_times, double max_dt_vol,
+                               const double* strikes, int n_strikes, const double* mats, int n_mats,
+                               const cf_num* num, double* value, double* delta, double* vega);
 
 /* ------------------------------------------------------------ request server */
 

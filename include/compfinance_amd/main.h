@@ -10,6 +10,7 @@
 #pragma once
 
 #include <map>
+#include <numeric>
 
 #include "store.h"
 
@@ -166,7 +167,8 @@ inline RiskReports bumpRisk(const string& modelId, const string& productId, cons
         *parameters[i] += bump;
         auto bumpRes = value(*model, *product, num);
         *parameters[i] -= bump;
-        for (size_t j = 0; j < m; ++j) results.risks[i][j] = (bumpRes.values[j] - baseRes.values[j]) / bump;
+        // (1.0 / bump is exactly 1e8 for the default bump: the reference's scaling)
+        for (size_t j = 0; j < m; ++j) results.risks[i][j] = (bumpRes.values[j] - baseRes.values[j]) * (1.0 / bump);
     }
     return results;
 }
@@ -273,6 +275,65 @@ inline SuperbucketResults dupireSuperbucket(const double spot, const double maxD
                                 int(mats.size()), &*micro.vega.begin(), int(micro.vega.rows() * micro.vega.cols()),
                                 &*r.vega.begin()));
     return r;
+}
+
+// main.h:575-696: the superbucket by bump and reprice, the validator of the
+// AAD superbucket above: delta by a 1e-8 spot bump; each risk-view spread
+// bumped by 1e-5 in turn, the local vols recalibrated (dupireCalib with the
+// risk view, mcMdlDupire.h:337-388) and the product repriced on the GPU
+inline SuperbucketResults dupireSuperbucketBump(const double spot, const double maxDt, const string& productId,
+                                                const std::map<string, double>& notionals,
+                                                const vector<double>& inclSpots, const double maxDs,
+                                                const vector<Time>& inclTimes, const double maxDtVol,
+                                                const vector<double>& strikes, const vector<Time>& mats,
+                                                const double vol, const double jmpIntens, const double jmpAverage,
+                                                const double jmpStd, const NumericalParam& num)
+{
+    SuperbucketResults results;
+    auto params = dupireCalib(inclSpots, maxDs, inclTimes, maxDtVol, spot, vol, jmpIntens, jmpAverage, jmpStd);
+    Dupire<double> model(spot, params.spots, params.times, params.lVols, maxDt);
+    const Product<double>* product = getProduct<double>(productId);
+    if (!product) throw std::runtime_error("dupireSuperbucketBump() : product not found");
+    auto baseVals = value(model, *product, num);
+    const vector<string>& allPayoffs = baseVals.identifiers;
+    vector<double> vnots(allPayoffs.size(), 0.0);
+    for (const auto& notional : notionals) {
+        auto it = std::find(allPayoffs.begin(), allPayoffs.end(), notional.first);
+        if (it == allPayoffs.end()) throw std::runtime_error("dupireSuperbucketBump() : payoff not found");
+        vnots[std::distance(allPayoffs.begin(), it)] = notional.second;
+    }
+    auto book = [&](const ValueResults& v) {
+        return std::inner_product(vnots.begin(), vnots.end(), v.values.begin(), 0.0);
+    };
+    results.value = book(baseVals);
+    // delta
+    Dupire<double> bumpedSpot(spot + 1.0e-08, params.spots, params.times, params.lVols, maxDt);
+    results.delta = (book(value(bumpedSpot, *product, num)) - results.value) * 1.0e+08;
+    // vega: one spread of the risk view at a time
+    const cf_merton_ivs ivs{spot, vol, jmpIntens, jmpAverage, jmpStd};
+    const size_t n = strikes.size(), m = mats.size();
+    results.vega.resize(n, m);
+    vector<double> spreads(n * m, 0.0);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < m; ++j) {
+            spreads[i * m + j] += 1.0e-05;
+            int nS = 0, nT = 0;
+            cfCheck(cf_dupire_calib_view(&ivs, inclSpots.data(), int(inclSpots.size()), maxDs, inclTimes.data(),
+                                         int(inclTimes.size()), maxDtVol, strikes.data(), int(n), mats.data(), int(m),
+                                         spreads.data(), nullptr, nullptr, nullptr, &nS, &nT, 0));
+            vector<double> bs(nS);
+            vector<Time> bt(nT);
+            matrix<double> blv(nS, nT);
+            cfCheck(cf_dupire_calib_view(&ivs, inclSpots.data(), int(inclSpots.size()), maxDs, inclTimes.data(),
+                                         int(inclTimes.size()), maxDtVol, strikes.data(), int(n), mats.data(), int(m),
+                                         spreads.data(), bs.data(), bt.data(), &*blv.begin(), &nS, &nT, nS * nT));
+            Dupire<double> bumped(spot, bs, bt, blv, maxDt);
+            results.vega[i][j] = (book(value(bumped, *product, num)) - results.value) * 1.0e+05;
+            spreads[i * m + j] -= 1.0e-05;
+        }
+    results.strikes = strikes;
+    results.mats = mats;
+    return results;
 }
 
 }  // namespace cfamd

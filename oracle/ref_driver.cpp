@@ -17,6 +17,7 @@
  *        <spot> <maxDt> <nS> <spots> <maxDs> <nT> <times> <maxDtVol>
  *        <nK> <strikes> <nM> <mats> <vol> <jmpIntens> <jmpAverage> <jmpStd>
  *        then a product spec and notionals as for aad
+ *   superbucket_bump <nPath>         dupireSuperbucketBump (main.h:575-696), the same input
  * The spec is whitespace separated:
  *   model bs <spot> <vol> <qspot> <rate> <div>
  *   model dupire <spot> <maxDt> <nS> <spots> <nT> <times> <vols, spot major>
@@ -292,7 +293,7 @@ int main(int argc, char** argv)
         return 0;
     }
 
-    if (cmd == "superbucket") {
+    if (cmd == "superbucket" || cmd == "superbucket_bump") {
         const int nPath = std::atoi(argv[2]);
         double spot, maxDt, maxDs, maxDtVol, vol, ji, ja, js;
         size_t nS, nT, nK, nM;
@@ -311,6 +312,17 @@ int main(int argc, char** argv)
         num.useSobol = true;
         num.numPath = nPath;
         std::map<std::string, double> nots(spec.notionals.begin(), spec.notionals.end());
+        if (cmd == "superbucket_bump") {
+            auto res = dupireSuperbucketBump(spot, maxDt, "p", nots, spots, maxDs, times, maxDtVol, strikes, mats,
+                                             vol, ji, ja, js, num);
+            std::vector<double> vega;
+            for (size_t i = 0; i < res.vega.rows(); ++i)
+                for (size_t j = 0; j < res.vega.cols(); ++j) vega.push_back(res.vega[i][j]);
+            std::printf("{\"value\": %.17g, \"delta\": %.17g,\n", res.value, res.delta);
+            printVec("vega", vega, false);
+            std::printf("}\n");
+            return 0;
+        }
         auto res = dupireSuperbucket(spot, maxDt, "p", nots, spots, maxDs, times, maxDtVol, strikes, mats, vol, ji,
                                      ja, js, num);
         std::vector<double> vega;

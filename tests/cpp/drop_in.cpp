@@ -1,6 +1,6 @@
 // A reference-style caller compiled against the C++ mirror headers and
 // libcfamd.so: the store, the main.h entry points, the simulation templates,
-// the calibration.  `host` exercises what runs without a GPU (labels,
+// the calibration, bump and reprice.  `host` exercises what runs without a GPU (labels,
 // timelines, calibration, errors); `device` runs the simulations.  Prints one
 // JSON object for the test to compare.
 #include <cstdio>
@@ -64,6 +64,11 @@ int main(int argc, char** argv)
         printVec("uoc_risks", r.risks);
         auto a = AADriskAggregate("lv", "book", {{labels[0], 1.0}, {labels[3], -0.5}, {labels[5], 2.0}}, num);
         printVec("book_risks", a.risks);
+        // bump and reprice (main.h:316-359): the risks of the barrier payoff
+        auto br = bumpRisk("lv", "uoc", num);
+        std::vector<double> bump0;
+        for (size_t i = 0; i < br.risks.rows(); ++i) bump0.push_back(br.risks[i][0]);
+        printVec("bump_risks", bump0);
         // the template level, as mcBase.h callers use it
         const Model<Number>* mdl = getModel<Number>("lv");
         const Product<Number>* prd = getProduct<Number>("uoc");

@@ -1,6 +1,7 @@
 """Writes tests/golden/calib.json from the reference build: local-vol
 calibrations (dupireCalib, main.h:414-438) on Merton surfaces and
-superbuckets (dupireSuperbucket, main.h:453-569) of two products.
+superbuckets of two products, by AAD (dupireSuperbucket, main.h:453-569) and
+by bump and reprice (dupireSuperbucketBump, main.h:575-696).
 
     python tests/golden/make_golden_calib.py
 """
@@ -67,6 +68,9 @@ def main():
                                         " ".join("%d %r" % (i, w) for i, w in sb["notionals"].items()))
         res = json.loads(subprocess.run([drv, "superbucket", str(sb["n_path"])], input=stdin, capture_output=True,
                                         text=True, check=True).stdout)
+        bump = json.loads(subprocess.run([drv, "superbucket_bump", str(sb["n_path"])], input=stdin,
+                                         capture_output=True, text=True, check=True).stdout)
+        res["bump"] = bump
         out["superbuckets"].append(dict(sb, notionals={str(k): w for k, w in sb["notionals"].items()}, **res))
     with open(os.path.join(HERE, "calib.json"), "w") as f:
         json.dump(out, f)

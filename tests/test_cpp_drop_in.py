@@ -1,6 +1,7 @@
 """A reference-style C++ caller (tests/cpp/drop_in.cpp) compiled against the
 mirror headers and libcfamd.so: host-side results without a GPU, simulation
-results on the GPU equal to the Python binding's."""
+results on the GPU equal to the Python binding's, and the mirror's
+bump-and-reprice risks against the AAD risks."""
 import json
 import os
 import subprocess
@@ -54,3 +55,9 @@ def test_device_side(engine, drop_in):
     w[0], w[3], w[5] = 1.0, -0.5, 2.0
     rb = cf.mcParallelSimulAAD(book, mdl, n, weights=w)
     np.testing.assert_allclose(out["book_risks"], rb.risks, rtol=1e-12, atol=1e-12 * np.abs(rb.risks).max())
+    # bumpRisk (main.h:316-359): one-sided 1e-8 bumps of every parameter on the
+    # same paths; its vegas equal the AAD vegas to the finite difference's
+    # rounding (a 1e-8 bump leaves about 1e-8 of the price's last digits)
+    bump = np.array(out["bump_risks"])
+    assert bump.shape == ref.risks.shape
+    assert np.abs(bump[1:] - ref.risks[1:]).max() <= 1e-5 * np.abs(ref.risks[1:]).max()
