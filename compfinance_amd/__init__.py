@@ -525,7 +525,10 @@ def mcParallelSimulAAD(prd, mdl, nPath=None, weights=None, num=None, per_path=Fa
     if weights is None:
         w[0] = 1.0
     else:
-        w[:] = _f64(weights)
+        wa = _f64(weights).ravel()
+        if wa.size != npay:
+            raise ValueError("weights must have one entry per payoff (%d), got %d" % (npay, wa.size))
+        w[:] = wa
     pp = np.zeros((n.num_path, npay)) if per_path else None
     pa = np.zeros(n.num_path) if per_path else None
     base = buf.ctypes.data

@@ -16,12 +16,12 @@ LIB = os.path.join(HERE, "libcfamd.so")
 
 # (source, object, defines): the path kernel families compile in parallel
 SOURCES = ([("cf_kern_inst.hip", "cf_kern_r%d_a%d_b%d" % (r, a, b),
-             ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_NBUF=%d" % b])
+             ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_SRC=%d" % b])
             for r in (0, 1) for a in (0, 1, 2) for b in (0, 1)] +
-           [("cf_kern_inst.hip", "cf_kern_r%d_a1_b2" % r, ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=1", "-DCF_INST_NBUF=2"])
+           [("cf_kern_inst.hip", "cf_kern_r%d_a1_b2" % r, ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=1", "-DCF_INST_SRC=2"])
             for r in (0, 1)] +
            [("cf_kern_inst.hip", "cf_kern_r%d_a%d_b3" % (r, a),
-             ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_NBUF=3"])
+             ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_SRC=3"])
             for r in (0, 1) for a in (0, 1, 2)] +
 
            [(s, os.path.splitext(s)[0], []) for s in ("cf_kernels.hip", "cf_engine.cpp", "cf_store.cpp",

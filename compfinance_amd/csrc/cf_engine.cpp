@@ -891,9 +891,11 @@ std::shared_ptr<Prepared> prepare(Engine& E, const cf_model* mdl, const cf_produ
             P->o_knots = st.add(kx);
             P->o_invdx = st.add(ih);
             P->o_iv = st.add(v);
-            // every node at least 1e-30: an interpolated vol is never 0, so
-            // the reverse sweep never draws a normal again
-            P->iv_pos = std::all_of(v.begin(), v.end(), [](T y) { return std::isfinite(double(y)) && y >= T(1e-30); });
+            // every node at least the recovery threshold: so is every
+            // interpolated vol (a convex combination of two nodes), and the
+            // reverse sweep never draws a normal again
+            const T vmin = T(sizeof(T) == 8 ? cfk::kRecoverVolF64 : cfk::kRecoverVolF32);
+            P->iv_recover = std::all_of(v.begin(), v.end(), [vmin](T y) { return std::isfinite(double(y)) && y >= vmin; });
         };
         if (fp64) {
             tables(double(), md.log_spots);
@@ -1005,8 +1007,7 @@ static size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
 
 size_t CallBytes::total() const
 {
-    return al256(block_out) + al256(reduced) + al256(ivbar_blocks) + al256(ivbar) + al256(weights) + al256(ckpt) +
-           al256(normals);
+    return al256(block_out) + al256(reduced) + al256(ivbar_blocks) + al256(ivbar) + al256(weights) + al256(ckpt);
 }
 
 CallBytes call_bytes(const CallShape& S)
@@ -1018,7 +1019,6 @@ CallBytes call_bytes(const CallShape& S)
     b.ivbar = S.ivn * 8;
     b.weights = S.adjoint ? size_t(S.npay) * 8 : 0;
     b.ckpt = size_t(S.slots) * S.ck_rows * cfk::kWave * (S.fp64 ? 8 : 4);
-    b.normals = S.mrg ? size_t(S.n_path) * size_t(S.n_steps) * (S.fp64 ? 8 : 4) : 0;
     return b;
 }
 
@@ -1086,9 +1086,13 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     P.ckpt = B.ckpt;
     P.ivbar_blocks = B.ivbar_blocks;
     P.per_path_agg = B.per_path_agg;
-    P.normals = S.mrg ? B.normals : nullptr;
+    if (S.mrg) {
+        P.mrg_jump = reinterpret_cast<const uint64_t*>(dt + PR.o_mrg);
+        P.mrg_seed1 = uint32_t(S.seed1);
+        P.mrg_seed2 = uint32_t(S.seed2);
+    }
     P.ivbar_copies = PR.ivbar_copies;
-    P.iv_pos = PR.iv_pos ? 1 : 0;
+    P.iv_recover = PR.iv_recover ? 1 : 0;
     return P;
 }
 
@@ -1097,22 +1101,11 @@ void make_sequence(const Prepared& PR, const CallShape& S, const CallBuffers& B,
 {
     std::memset(q, 0, sizeof *q);
     q->cfg = PR.cfg;
-    q->cfg.normals = S.mrg;
+    q->cfg.mrg = S.mrg;
     std::memcpy(&q->P, &P, sizeof P);
     q->grid = S.grid;
     q->block = PR.block;
     q->smem = PR.smem;
-    if (S.mrg) {
-        q->mrg = 1;
-        q->jump = reinterpret_cast<const uint64_t*>(B.tables + PR.o_mrg);
-        q->seed1 = S.seed1;
-        q->seed2 = S.seed2;
-        q->n_steps = PR.md.n_steps;
-        q->real_bytes = S.fp64 ? 8 : 4;
-        q->first = S.first;
-        q->n_path = S.n_path;
-        q->normals = B.normals;
-    }
     q->block_out = B.block_out;
     q->stride = S.stride;
     q->reduced = B.reduced;
@@ -1125,10 +1118,10 @@ void make_sequence(const Prepared& PR, const CallShape& S, const CallBuffers& B,
 
 void ready_sequence(const Sequence& q)
 {
-    // the plan's kernel was set up at planning; the normals-buffer and
+    // the plan's kernel was set up at planning; the mrg32k3a and
     // itemized-risk variants of it are set up on first use (cached by the
     // launcher)
-    if (!q.cfg.normals && !q.cfg.item) return;
+    if (!q.cfg.mrg && !q.cfg.item) return;
     int per_cu = 0;
     const int rc = cfk::path_kernel_occupancy(q.cfg, q.block, q.smem, &per_cu);
     if (rc != 0) hip_check(hipError_t(rc), "occupancy");
@@ -1136,11 +1129,6 @@ void ready_sequence(const Sequence& q)
 
 void enqueue_sequence(const Sequence& q, hipStream_t stream, hipEvent_t ev_kernel0, hipEvent_t ev_kernel1)
 {
-    if (q.mrg) {
-        const int rc = cfk::launch_mrg_normals(q.jump, q.seed1, q.seed2, q.n_steps, q.first, q.n_path, q.real_bytes,
-                                               q.normals, stream);
-        if (rc != 0) hip_check(hipError_t(rc), "normals launch");
-    }
     if (ev_kernel0) hip_check(hipEventRecord(ev_kernel0, stream), "hipEventRecord");
     int rc = cfk::launch_path_kernel(q.cfg, q.P, q.grid, q.block, q.smem, stream);
     if (rc != 0) hip_check(hipError_t(rc), "path kernel launch");
@@ -1234,7 +1222,6 @@ int run(const cf_model* mdl, const cf_product* prd, Call& c)
         B.weights = static_cast<double*>(E.weights.ensure(nb.weights));
         B.ckpt = E.ckpt.ensure(nb.ckpt);
     }
-    if (S.mrg) B.normals = E.normals.ensure(nb.normals);
     if (c.per_path) B.per_path = static_cast<double*>(E.per_path.ensure(size_t(S.n_path) * S.npay * 8));
     if (c.per_path_agg) B.per_path_agg = static_cast<double*>(E.per_path_agg.ensure(size_t(S.n_path) * 8));
 

@@ -199,7 +199,7 @@ struct Prepared {
     int per_cu = 0;
     int block = 0;
     int ivbar_copies = 1;   // LDS local-vol adjoint grids per block
-    bool iv_pos = false;    // Dupire: every local-vol table entry positive (at least 1e-30)
+    bool iv_recover = false;   // Dupire: every local-vol table entry at least kRecoverVolF64 / F32
     BookTables book;
     vector<unsigned char> staged;   // the tables as uploaded
     DevBuf tables;                  // device copy for the synchronous calls
@@ -225,7 +225,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     uint32_t* d_dirs = nullptr;
-    DevBuf block_out, ivbar_blocks, reduced, per_path, per_path_agg, ckpt, weights, normals;
+    DevBuf block_out, ivbar_blocks, reduced, per_path, per_path_agg, ckpt, weights;
     PinnedBuf pin_in, pin_out;
     // the aggregation weights last uploaded to `weights` (skipped when a call repeats them)
     vector<double> weights_last;
@@ -286,7 +286,7 @@ std::shared_ptr<Prepared> lookup(Engine& E, const cf_model* mdl, const cf_produc
 // blocks of the device, enough waves for the 64-path Sobol blocks.
 struct CallShape {
     int64_t first = 0, n_path = 0;
-    bool mrg = false;               // mrg32k3a normals pre-generated into a buffer
+    bool mrg = false;               // mrg32k3a normals (generated in the path kernel)
     uint64_t seed1 = 0, seed2 = 0;  // reduced modulo m1, m2
     int grid = 0, waves = 0, slots = 0, stride = 0, npay = 0, ck_rows = 0, n_steps = 0;
     size_t ivn = 0;                 // Dupire adjoint: steps x spots
@@ -306,21 +306,20 @@ struct CallBuffers {
     void* ckpt = nullptr;           // [slots][ck_rows][64] reals
     double* per_path = nullptr;     // [n_path][npay] or null
     double* per_path_agg = nullptr; // [n_path] or null
-    void* normals = nullptr;        // mrg32k3a: [n_steps][n_path] reals
 };
 
 // Bytes of each device buffer (0 when unused) and of all of them, each part
 // aligned to 256 bytes, tables excluded.
 struct CallBytes {
-    size_t block_out, reduced, ivbar_blocks, ivbar, weights, ckpt, normals;
+    size_t block_out, reduced, ivbar_blocks, ivbar, weights, ckpt;
     size_t total() const;
 };
 CallBytes call_bytes(const CallShape& S);
 
 cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallShape& S, const CallBuffers& B);
 
-// The device work of one call as plain values: the normals kernel
-// (mrg32k3a), the path kernel and the cross-block reductions.  Enqueueing
+// The device work of one call as plain values: the path kernel and the
+// cross-block reductions.  Enqueueing
 // reads nothing else, so two calls whose sequences are equal byte for byte
 // enqueue the same work (the server replays a captured graph on that
 // equality).  Built zero-filled, padding included.
@@ -329,12 +328,6 @@ struct Sequence {
     cfk::LaunchParams P;
     int grid, block;
     size_t smem;
-    int mrg;                       // 1: generate the normals first
-    const uint64_t* jump;          // mrg32k3a jump matrices (device)
-    uint64_t seed1, seed2;
-    int n_steps, real_bytes;
-    int64_t first, n_path;
-    void* normals;
     double* block_out;             // [grid][stride]
     int stride;
     double* reduced;               // [stride]

@@ -27,6 +27,13 @@ constexpr int kSobolDims = 1101;     // direction-number table stride
 // registers); Black-Scholes keeps nothing (normals are regenerated).
 constexpr int kSegF32 = 8, kSegF64 = 4;
 
+// The checkpoint adjoint recovers d x_{i+1} / d vol = g - vol from the two
+// stored states as (x_{i+1} - x_i) / vol - vol / 2.  The difference is exact,
+// but it carries the rounding of x_{i+1} (half an ulp of the state), divided
+// by vol; below these vols the reverse sweep draws the normal again instead
+// (FP64: the error stays below 1e-12 for states under 8 in magnitude).
+constexpr double kRecoverVolF64 = 0x1p-10, kRecoverVolF32 = 0x1p-8;
+
 // Book of products on shared paths (also an Asian on its own): vanilla calls
 // attached to samples, up-and-out barriers and arithmetic-average groups.  Each
 // barrier and each distinct fixing schedule keeps per-lane state in registers,
@@ -136,19 +143,21 @@ struct LaunchParams {
 
     // product
     int n_pay;
+    int put;
     double strike;
     double bar_smooth, minus_smooth, two_smooth, inv_two_smooth;
     double bar_xlo;              // Dupire barrier: at or below this state the spot is under the
                                  // smoothing zone (with a margin over exp's rounding)
-    int put;
     const int* pay_first;        // Europeans [n_samples + 1]: payoffs of sample s
     const double* pay_strike;    // Europeans [n_pay]
     const void* book;            // book tables, layout bk
     BookLayout bk;
     double book_lo;              // the lowest B - w of the book's barriers (+inf without one)
 
-    // pre-generated normals [n_steps][n_path] (mrg32k3a), or null: Sobol
-    const void* normals;
+    // mrg32k3a (null: Sobol): jump matrices [32][2][9], the two component
+    // matrices raised to n_steps * 2^j (mod m1, m2), and the seeds as residues
+    const uint64_t* mrg_jump;
+    uint32_t mrg_seed1, mrg_seed2;
 
     // outputs
     double* block_out;           // [gridDim][out_stride(n_pay)]
@@ -159,7 +168,8 @@ struct LaunchParams {
     void* ckpt;                  // [slots][ckpt_rows][64] path states or normals,
                                  // then (Dupire) [n_steps][64] bracket bytes
     int ivbar_copies;            // LDS local-vol adjoint grids per block: even and odd lanes (2) or one
-    int iv_pos;                  // 1: every local-vol table entry is at least 1e-30 (no zero vol)
+    int iv_recover;              // 1: every local-vol table entry is at least kRecoverVol (the
+                                 // reverse sweep needs no normal)
     double* ivbar_blocks;        // [gridDim][n_steps][n_spots] local-vol adjoints
     double* per_path_agg;        // [n_path] or null
     // checkpoint adjoint shared across payoffs (itemized risk): 0 one call
@@ -396,13 +406,13 @@ struct KernelConfig {
     bool adjoint;
     bool lean;       // LEAN shared-memory layout
     bool recompute;  // adjoint by re-simulation instead of per-step checkpoints
-    bool normals;    // normals read from a buffer (mrg32k3a) instead of Sobol
+    bool mrg;        // normals from mrg32k3a generators in the kernel instead of Sobol
     bool item;       // checkpoint kernel with the itemized-risk modes (LaunchParams::ck_mode)
     bool packed;     // path_kernel_packed (several calls, PackArgs)
 };
 
 // Path kernel families (precision r: 0 FP64, 1 FP32; adjoint a: AdjointKind;
-// b: 0 Sobol, 1 normals buffer, 2 itemized-risk checkpoint kernels, 3 Sobol
+// b: 0 Sobol, 1 mrg32k3a, 2 itemized-risk checkpoint kernels, 3 Sobol
 // kernels of packed launches), one per build of cf_kern_inst.hip: kernel for
 // a model, product and layout.
 #define CF_PICK_CAT(r, a, b) pick_r##r##_a##a##_b##b
@@ -451,12 +461,6 @@ size_t sobol_points_smem(int real_bytes, bool lean, int dim, int block);
 // layout: 0 FULL rows, 1 LEAN, 2 FULL quads (SobolLoLayout)
 int launch_sobol_points(int real_bytes, int layout, const uint32_t* dirs, int dim, int64_t first, int64_t npath,
                         uint32_t* words, void* normals, void* stream);
-
-// mrg32k3a normals of paths [first_path, first_path + n_path), dim steps,
-// written [step][path] in FP64 (real_bytes 8) or FP32.  jump: [32][2][9] the
-// component matrices raised to dim * 2^j (mod m1, m2); seeds already reduced.
-int launch_mrg_normals(const uint64_t* jump, uint64_t seed1, uint64_t seed2, int dim, int64_t first_path,
-                       int64_t n_path, int real_bytes, void* out, void* stream);
 
 // Sets the thread-local message returned by cf_last_error() (cf_engine.cpp).
 void set_last_error(const char* msg);

@@ -1,15 +1,15 @@
 // One (precision, adjoint, normals source) family of path kernels,
 // instantiated for every model, product and LDS layout.  Built once per
 // family with -DCF_INST_REAL=0|1 (FP64, FP32), -DCF_INST_ADJ=0|1|2
-// (cfk::AdjointKind) and -DCF_INST_NBUF=0|1|2|3 (Sobol, normals buffer, Sobol
+// (cfk::AdjointKind) and -DCF_INST_SRC=0|1|2|3 (Sobol, mrg32k3a, Sobol
 // with the itemized-risk modes: checkpoint family only, Sobol kernels of
 // packed launches), so the families compile in parallel.
 #include <type_traits>
 
-#if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ) || !defined(CF_INST_NBUF)
-#error "CF_INST_REAL, CF_INST_ADJ and CF_INST_NBUF must be defined"
+#if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ) || !defined(CF_INST_SRC)
+#error "CF_INST_REAL, CF_INST_ADJ and CF_INST_SRC must be defined"
 #endif
-#if CF_INST_NBUF == 2 && CF_INST_ADJ != 1
+#if CF_INST_SRC == 2 && CF_INST_ADJ != 1
 #error "the itemized-risk kernels are checkpoint kernels"
 #endif
 
@@ -27,14 +27,14 @@ namespace {
 
 using Real = std::conditional_t<CF_INST_REAL == 0, double, float>;
 constexpr int kAdj = CF_INST_ADJ;
-// CF_INST_NBUF: 0 Sobol, 1 normals buffer, 2 Sobol with the itemized-risk
+// CF_INST_SRC: 0 Sobol, 1 mrg32k3a, 2 Sobol with the itemized-risk
 // modes (forward only keeping end states, reverse only from them), 3 Sobol
 // kernels of packed launches (path_kernel_packed: the same body; with
 // multiply-adds fused only within an expression it rounds as the single
 // kernel does)
-constexpr bool kNbuf = CF_INST_NBUF == 1;
-constexpr bool kItem = CF_INST_NBUF == 2;
-constexpr bool kPacked = CF_INST_NBUF == 3;
+constexpr bool kMrg = CF_INST_SRC == 1;
+constexpr bool kItem = CF_INST_SRC == 2;
+constexpr bool kPacked = CF_INST_SRC == 3;
 
 template <int MODEL, int PRD>
 const void* ptr(bool lean)
@@ -43,8 +43,8 @@ const void* ptr(bool lean)
         return lean ? reinterpret_cast<const void*>(&path_kernel_packed<Real, MODEL, PRD, kAdj, true>)
                     : reinterpret_cast<const void*>(&path_kernel_packed<Real, MODEL, PRD, kAdj, false>);
     else
-        return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true, kNbuf, kItem>)
-                    : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false, kNbuf, kItem>);
+        return lean ? reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, true, kMrg, kItem>)
+                    : reinterpret_cast<const void*>(&path_kernel<Real, MODEL, PRD, kAdj, false, kMrg, kItem>);
 }
 
 template <int MODEL>
@@ -61,7 +61,7 @@ const void* by_product(int prd, bool lean)
 
 }  // namespace
 
-const void* CF_PICK_NAME(CF_INST_REAL, CF_INST_ADJ, CF_INST_NBUF)(int model, int prd, bool lean)
+const void* CF_PICK_NAME(CF_INST_REAL, CF_INST_ADJ, CF_INST_SRC)(int model, int prd, bool lean)
 {
     return model == MODEL_DUPIRE ? by_product<MODEL_DUPIRE>(prd, lean) : by_product<MODEL_BS>(prd, lean);
 }

@@ -12,87 +12,6 @@
 
 namespace cfk {
 
-// ------------------------------------------------------------------ mrg32k3a
-
-// L'Ecuyer's combined multiple recursive generator as the reference runs it
-// (mrg32k3a.h:41-86): moduli, multipliers (negated ones as m - a), and the
-// uniform (x - y) / (m1 + 1) folded into (0, 1).
-constexpr uint64_t kM1 = 4294967087ull, kM2 = 4294944443ull;
-constexpr uint64_t kA12 = 1403580ull, kA13n = kM1 - 810728ull;
-constexpr uint64_t kA21 = 527612ull, kA23n = kM2 - 1370589ull;
-
-// v = A v mod M for a 3 x 3 matrix with entries below M (< 2^32): every
-// product is below 2^64 and reduced before it is added.
-template <uint64_t M>
-__device__ __forceinline__ void mat_vec_mod(const uint64_t* __restrict__ A, uint64_t v[3])
-{
-    uint64_t r[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        uint64_t acc = 0;
-#pragma unroll
-        for (int l = 0; l < 3; ++l) acc = (acc + (A[3 * j + l] * v[l]) % M) % M;
-        r[j] = acc;
-    }
-    v[0] = r[0];
-    v[1] = r[1];
-    v[2] = r[2];
-}
-
-// One thread per block of `dim` numbers: block k feeds path 2k and, negated,
-// path 2k + 1 (the reference's antithetic pairs, mrg32k3a.h:147-180; its
-// parallel batches start at even paths, mcBase.h:369-381).  The state of
-// block k is the seed state advanced by k * dim numbers through the jump
-// matrices of the bits of k (the skip-ahead of mrg32k3a.h:187-381).
-template <typename R>
-__global__ void __launch_bounds__(256) mrg_normals_kernel(const uint64_t* __restrict__ jump, uint64_t s1, uint64_t s2,
-                                                          int dim, int64_t first, int64_t n, R* __restrict__ out)
-{
-    const int64_t k = (first >> 1) + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t c0 = 2 * k - first, c1 = c0 + 1;   // columns of paths 2k, 2k + 1
-    if (c0 >= n) return;
-    uint64_t x[3] = {s1, s1, s1}, y[3] = {s2, s2, s2};
-    for (int j = 0; j < 40; ++j) {
-        if (!(k >> j)) break;
-        if ((k >> j) & 1) {
-            mat_vec_mod<kM1>(jump + size_t(2 * j) * 9, x);
-            mat_vec_mod<kM2>(jump + size_t(2 * j + 1) * 9, y);
-        }
-    }
-    const bool w0 = c0 >= 0, w1 = c1 < n;
-    for (int d = 0; d < dim; ++d) {
-        const uint64_t xn = ((kA12 * x[1]) % kM1 + (kA13n * x[2]) % kM1) % kM1;
-        x[2] = x[1];
-        x[1] = x[0];
-        x[0] = xn;
-        const uint64_t yn = ((kA21 * y[0]) % kM2 + (kA23n * y[2]) % kM2) % kM2;
-        y[2] = y[1];
-        y[1] = y[0];
-        y[0] = yn;
-        const double diff = double(xn) - double(yn);
-        const double u = xn > yn ? diff / 4294967088.0 : (diff + 4294967087.0) / 4294967088.0;
-        const double z = inv_normal_p(u);
-        if (w0) out[size_t(d) * size_t(n) + size_t(c0)] = R(z);
-        if (w1) out[size_t(d) * size_t(n) + size_t(c1)] = R(-z);
-    }
-}
-
-int launch_mrg_normals(const uint64_t* jump, uint64_t seed1, uint64_t seed2, int dim, int64_t first_path,
-                       int64_t n_path, int real_bytes, void* out, void* stream)
-{
-    const int64_t blocks_of_numbers = ((first_path + n_path - 1) >> 1) - (first_path >> 1) + 1;
-    const int threads = 256;
-    const int64_t grid = (blocks_of_numbers + threads - 1) / threads;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    if (real_bytes == 8)
-        hipLaunchKernelGGL(mrg_normals_kernel<double>, dim3(unsigned(grid)), dim3(threads), 0, st, jump, seed1, seed2,
-                           dim, first_path, n_path, static_cast<double*>(out));
-    else
-        hipLaunchKernelGGL(mrg_normals_kernel<float>, dim3(unsigned(grid)), dim3(threads), 0, st, jump, seed1, seed2,
-                           dim, first_path, n_path, static_cast<float*>(out));
-    return int(hipGetLastError());
-}
-
 // Column sums of two row-major matrices with the same row count,
 // [rows][cols0] and [rows][cols1] (cols1 may be 0), in one launch: blocks
 // [0, ceil(cols0 / 64)) take the first, the rest the second.  A block takes 64
@@ -259,16 +178,16 @@ static const void* pick(const KernelConfig& c)
         {{pick_r1_a0_b0, pick_r1_a0_b1}, {pick_r1_a1_b0, pick_r1_a1_b1}, {pick_r1_a2_b0, pick_r1_a2_b1}}};
     const int adj = !c.adjoint ? ADJ_NONE : c.recompute ? ADJ_RECOMPUTE : ADJ_CHECKPOINT;
     if (c.item) {
-        if (adj != ADJ_CHECKPOINT || c.normals || c.packed) return nullptr;
+        if (adj != ADJ_CHECKPOINT || c.mrg || c.packed) return nullptr;
         return (c.precision == 0 ? pick_r0_a1_b2 : pick_r1_a1_b2)(c.model, c.product, c.lean);
     }
     if (c.packed) {
-        if (c.normals) return nullptr;
+        if (c.mrg) return nullptr;
         static const Pick packed[2][3] = {{pick_r0_a0_b3, pick_r0_a1_b3, pick_r0_a2_b3},
                                           {pick_r1_a0_b3, pick_r1_a1_b3, pick_r1_a2_b3}};
         return packed[c.precision == 0 ? 0 : 1][adj](c.model, c.product, c.lean);
     }
-    return table[c.precision == 0 ? 0 : 1][adj][c.normals ? 1 : 0](c.model, c.product, c.lean);
+    return table[c.precision == 0 ? 0 : 1][adj][c.mrg ? 1 : 0](c.model, c.product, c.lean);
 }
 
 int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid, int block, size_t smem, void* stream)
@@ -283,7 +202,7 @@ int launch_path_kernel(const KernelConfig& cfg, const LaunchParams& p, int grid,
 
 int launch_path_kernel_packed(const KernelConfig& cfg, const PackArgs& A, int block, size_t smem, void* stream)
 {
-    if (!cfg.packed || cfg.item || cfg.normals || A.n < 1 || A.n > kMaxPack) return int(hipErrorInvalidValue);
+    if (!cfg.packed || cfg.item || cfg.mrg || A.n < 1 || A.n > kMaxPack) return int(hipErrorInvalidValue);
     const void* f = pick(cfg);
     if (!f) return int(hipErrorInvalidDeviceFunction);
     void* args[] = {const_cast<PackArgs*>(&A)};

@@ -108,7 +108,6 @@ int run_noalloc(const cf_model* mdl, const cf_product* prd, cfh::Call& c)
         B.ivbar = B.reduced + S.stride;
     }
     B.ckpt = take(nb.ckpt);
-    if (S.mrg) B.normals = take(nb.normals);
     if (c.per_path) B.per_path = reinterpret_cast<double*>(take(size_t(S.n_path) * S.npay * 8));
     if (c.per_path_agg) B.per_path_agg = reinterpret_cast<double*>(take(size_t(S.n_path) * 8));
     if (off > kNoAllocScratchBytes)

@@ -28,7 +28,7 @@
 //     (step x spot) grid.
 // This header holds the device helpers and the path kernel template; each
 // cf_kern_inst.hip build instantiates one (precision, adjoint, source of
-// normals) family.
+// normals: Sobol or mrg32k3a) family.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -196,6 +196,132 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, const double*) { retur
 static __constant__ float kInvNormalF32[kInvNormalRows * 4] = {
 #include "inv_normal_f32.inc"
 };
+
+// ------------------------------------------------------------------ mrg32k3a
+
+// L'Ecuyer's combined multiple recursive generator as the reference runs it
+// (mrg32k3a.h:41-86), one generator per lane in registers:
+//   x_n = a12 x_{n-2} - a13 x_{n-3}  mod m1,   y_n = a21 y_{n-1} - a23 y_{n-3}  mod m2,
+// uniform (x_n - y_n) / (m1 + 1) folded into (0, 1).  Both moduli are
+// 2^32 - c with small c, so a product r < 2^64 reduces by folds
+// r -> (r >> 32) c + (r mod 2^32) (one 32 x 32 + 64 multiply-add each) and a
+// last conditional subtraction.  Both recursions are invertible (a13, a23
+// are units modulo the primes m1, m2), so the generator also steps back.
+namespace mrg {
+constexpr uint32_t kM1 = 4294967087u, kM2 = 4294944443u;
+constexpr uint32_t kC1 = 209u, kC2 = 22853u;            // 2^32 - m
+constexpr uint32_t kA12 = 1403580u, kA13 = 810728u;
+constexpr uint32_t kA21 = 527612u, kA23 = 1370589u;
+constexpr uint64_t pow_mod(uint64_t a, uint64_t e, uint64_t m)
+{
+    uint64_t r = 1;
+    for (a %= m; e; e >>= 1, a = a * a % m)
+        if (e & 1) r = r * a % m;
+    return r;
+}
+constexpr uint32_t kInv13 = uint32_t(pow_mod(kA13, kM1 - 2, kM1));   // a13^-1 mod m1 (Fermat)
+constexpr uint32_t kInv23 = uint32_t(pow_mod(kA23, kM2 - 2, kM2));   // a23^-1 mod m2
+static_assert(uint64_t(kInv13) * kA13 % kM1 == 1 && uint64_t(kInv23) * kA23 % kM2 == 1, "inverses");
+
+// r mod (2^32 - C) for any r < 2^64 (C < 2^15): the three folds leave
+// r < 2^32 + C < 2 (2^32 - C)
+template <uint32_t C>
+__device__ __forceinline__ uint32_t reduce(uint64_t r)
+{
+    constexpr uint32_t M = uint32_t(0u - C);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r = uint64_t(uint32_t(r >> 32)) * C + uint32_t(r);
+    return uint32_t(r >= M ? r - M : r);
+}
+
+// v = A v mod M, A a 3 x 3 matrix of residues (row-major), v residues
+template <uint32_t C>
+__device__ __forceinline__ void mat_vec(const uint64_t* __restrict__ A, uint32_t v[3])
+{
+    uint32_t r[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) acc = reduce<C>(uint64_t(uint32_t(A[3 * j + l])) * v[l] + acc);
+        r[j] = acc;
+    }
+    v[0] = r[0];
+    v[1] = r[1];
+    v[2] = r[2];
+}
+
+// One lane's generator.  x[0], y[0] are the pos-th numbers of the lane's
+// block (x[1], x[2] the two before); number pos feeds draw pos - 1.
+struct Lane {
+    uint32_t x[3], y[3];
+    uint32_t x0[3], y0[3];   // the block's start state (pos 0)
+    int pos;
+    bool neg;                // odd path: the antithetic (negated) normals of its pair
+
+    // Path p of a call (absolute index) draws from block p >> 1 of `dim`
+    // numbers, negated for odd p (mrg32k3a.h:147-180); its start state is the
+    // seed state advanced by (p >> 1) dim numbers through the jump matrices
+    // A^(dim 2^j) of the set bits j (mrg32k3a.h:187-381; jump: 2 x 9 per j,
+    // x then y).  Seeds are residues.
+    __device__ __forceinline__ void start(const uint64_t* __restrict__ jump, uint32_t s1, uint32_t s2, int64_t p)
+    {
+        uint32_t xs[3] = {s1, s1, s1}, ys[3] = {s2, s2, s2};
+        const uint64_t k = uint64_t(p) >> 1;
+        for (int j = 0; j < 32 && (k >> j); ++j) {   // p < 2^32 (checked on the host)
+            if ((k >> j) & 1) {
+                mat_vec<kC1>(jump + size_t(18 * j), xs);
+                mat_vec<kC2>(jump + size_t(18 * j + 9), ys);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            x[l] = x0[l] = xs[l];
+            y[l] = y0[l] = ys[l];
+        }
+        pos = 0;
+        neg = (p & 1) != 0;
+    }
+    __device__ __forceinline__ void forward()
+    {
+        // a12 x1 + a13 (m1 - x2) = x_n mod m1: both terms below 2^53
+        const uint32_t xn = reduce<kC1>(uint64_t(kA12) * x[1] + uint64_t(kA13) * (kM1 - x[2]));
+        const uint32_t yn = reduce<kC2>(uint64_t(kA21) * y[0] + uint64_t(kA23) * (kM2 - y[2]));
+        x[2] = x[1]; x[1] = x[0]; x[0] = xn;
+        y[2] = y[1]; y[1] = y[0]; y[0] = yn;
+        ++pos;
+    }
+    __device__ __forceinline__ void back()
+    {
+        // x_{n-3} = (a12 x_{n-2} - x_n) / a13,  y_{n-3} = (a21 y_{n-1} - y_n) / a23
+        const uint32_t dx = reduce<kC1>(uint64_t(kA12) * x[2] + (kM1 - x[0]));
+        const uint32_t dy = reduce<kC2>(uint64_t(kA21) * y[1] + (kM2 - y[0]));
+        const uint32_t xp = reduce<kC1>(uint64_t(dx) * kInv13);
+        const uint32_t yp = reduce<kC2>(uint64_t(dy) * kInv23);
+        x[0] = x[1]; x[1] = x[2]; x[2] = xp;
+        y[0] = y[1]; y[1] = y[2]; y[2] = yp;
+        --pos;
+    }
+    // Normal of draw i (number i + 1 of the block), reached from the current
+    // position by stepping forward or back, or forward from the start when
+    // that is shorter.  FP64 inverse normal of the reference (gaussians.h).
+    __device__ __forceinline__ double normal(int i)
+    {
+        const int t = i + 1;
+        if (t < pos - t) {
+#pragma unroll
+            for (int l = 0; l < 3; ++l) { x[l] = x0[l]; y[l] = y0[l]; }
+            pos = 0;
+        }
+        while (pos < t) forward();
+        while (pos > t) back();
+        const double diff = double(x[0]) - double(y[0]);
+        const double u = x[0] > y[0] ? diff / 4294967088.0 : (diff + 4294967087.0) / 4294967088.0;
+        const double z = inv_normal_p(u);
+        return neg ? -z : z;
+    }
+};
+}   // namespace mrg
 
 // Spot from the path state: FP64 state is log(S) (reference), FP32 state is
 // log(S / S0), which keeps more significant bits in single precision.
@@ -413,7 +539,7 @@ struct ProductState {
 
 // The kernel of one call, run by block bid of the call's nblk blocks (the
 // whole grid, or the call's share of a packed launch).
-template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF, bool ITEM>
+template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool MRG, bool ITEM>
 __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, const int nblk)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -461,7 +587,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
 
     // Sobol LO layout: the FP32 Dupire adjoint kernels other than the book
     // read four dimensions at once (quads); the others one at a time (rows)
-    constexpr bool QUADS = sizeof(R) == 4 && AD && DUP && !BOOK && !LEAN && !NBUF;
+    constexpr bool QUADS = sizeof(R) == 4 && AD && DUP && !BOOK && !LEAN && !MRG;
     constexpr int LOL = LEAN ? kLoLean : (QUADS ? kLoQuads : kLoRows);
     // ---- stage the block's tables in LDS
     sobol_stage_lo<LOL>(lo_tab, P.dirs, n, tid, nthr);
@@ -547,7 +673,6 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     const double s0 = P.spot;
     const int last = ns - 1;
 
-    const R* nrm = static_cast<const R*>(P.normals);
     // this lane's local-vol adjoint grid: odd lanes use the second copy, which
     // halves the lanes contending for one address in an atomic
     double* ivbl = ivbar + ((P.ivbar_copies > 1 && (lane & 1)) ? ivbar_copy_stride(n, m) : 0);
@@ -620,7 +745,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     for (int64_t q = q0 + int64_t(bid) * waves + wave; q <= qe; q += int64_t(nblk) * waves) {
         // ---- high Sobol words of this block of 64 indices (and of the
         // merged last block); this lane's row, index and path
-        const SobolLane sl = sobol_block(hi, hi_tail, P.dirs, n, q, q0, q1, nlo, nhi, merge, lane, !NBUF);
+        const SobolLane sl = sobol_block(hi, hi_tail, P.dirs, n, q, q0, q1, nlo, nhi, merge, lane, !MRG);
         const uint32_t* hl = sl.hl;
         const bool active = sl.active;
         const int64_t pidx = sl.pidx;
@@ -632,9 +757,10 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
             if constexpr (CKB) ckl = reinterpret_cast<uint8_t*>(ck + size_t(nck) * kWave);
             stq = static_cast<R*>(P.state) + sb * kStateRows * kWave + lane;
         }
-        // normal of step i: from the Sobol point, or from the buffer of
-        // pre-generated normals ([step][path] of the call's paths)
-        const size_t ncol = size_t(active ? pidx : 0);
+        // normal of step i: from the Sobol point, or from this lane's
+        // mrg32k3a generator (idle lanes run the call's first path)
+        mrg::Lane gen;
+        if constexpr (MRG) gen.start(P.mrg_jump, P.mrg_seed1, P.mrg_seed2, P.first_path + (active ? pidx : 0));
         // the inverse-normal table (LDS, or for LEAN the constant array
         // itself) less kInvNormalRowBias rows
         const R* invn_tab;
@@ -642,7 +768,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
         else invn_tab = invn;
         invn_tab -= 4 * kInvNormalRowBias;
         auto draw = [&](int i) -> R {
-            if constexpr (NBUF) return nrm[size_t(i) * size_t(P.n_path) + ncol];
+            if constexpr (MRG) return R(gen.normal(i));
             else return inv_normal(hl[i] ^ low_word(i), invn_tab);
         };
 
@@ -879,7 +1005,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                 const int i = i0 + k;
                 R g;
                 if constexpr (QW) g = gz[k];
-                else if constexpr (NBUF || LEAN) g = draw(i);
+                else if constexpr (MRG || LEAN) g = draw(i);
                 else g = inv_normal(hl[i] ^ low_word(i), invn_tab);
                 if constexpr (DUP) {
                     if constexpr (GRP) gx[k] = x;
@@ -1174,6 +1300,13 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         for (int q4 = 0; q4 < kSeg; q4 += 4)
                             if (i0 + q4 < n) sobol_words4(lo_tab, hl, lane, i0 + q4, sw + q4);
                     }
+                    // mrg32k3a: the segment's normals drawn top down, so the
+                    // generator only steps back over the reverse sweep
+                    R mz[kSeg];
+                    if constexpr (MRG) {
+#pragma unroll
+                        for (int k = kSeg - 1; k >= 0; --k) mz[k] = i0 + k < n ? draw(i0 + k) : R(0);
+                    }
 #pragma unroll
                     for (int k = 0; k < kSeg; ++k) {
                         const int i = i0 + k;
@@ -1185,6 +1318,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         if (i >= n) continue;
                         R g;
                         if constexpr (QUADS) g = inv_normal(sw[k], invn_tab);
+                        else if constexpr (MRG) g = mz[k];
                         else g = draw(i);
                         const Bracket<R> b = local_vol<LEAN>(ks, kxi, iv + i * mp, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol
@@ -1230,8 +1364,10 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     }
                 }
                 R xnext = x;     // state at point i + 1
-                // every local-vol node positive: no vol is 0, no normal drawn again
-                const bool iv_pos = P.iv_pos != 0;
+                // every local-vol node at least the recovery threshold: no
+                // normal is drawn again
+                const bool iv_recover = P.iv_recover != 0;
+                const R vmin = R(sizeof(R) == 8 ? kRecoverVolF64 : kRecoverVolF32);
                 // groups of U steps, top down to top - U + 1; the checkpoint
                 // loads of the next group are issued before the current group
                 // is worked (FP64 and the book hold more registers: U = 1).
@@ -1310,12 +1446,13 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         const int lo = ck_lo ? lbuf[k] : count_le<LEAN>(ks, xi);
                         const Bracket<R> b = local_vol_at(kxi, iv + i * mp, xi, lo);
                         // d x_{i+1} / d vol = g - vol, recovered from the two states:
-                        // x_{i+1} - x_i = vol (g - vol / 2), exact difference in FP64
-                        // where a local vol is 0 (rare; a wave-uniform test) the
-                        // normal is drawn again
+                        // x_{i+1} - x_i = vol (g - vol / 2) up to the rounding of
+                        // x_{i+1}; where a local vol is below the recovery
+                        // threshold (rare; a wave-uniform test) the normal is
+                        // drawn again
                         ws[k] = recip_mul(xnext - xi, b.vol) - R(0.5) * b.vol;
-                        if (!iv_pos && __builtin_amdgcn_ballot_w64(b.vol == R(0)) != 0)
-                            if (b.vol == R(0)) ws[k] = draw(i);
+                        if (!iv_recover && __builtin_amdgcn_ballot_w64(b.vol < vmin) != 0)
+                            if (b.vol < vmin) ws[k] = draw(i) - b.vol;
                         sl[k] = b.slope;
                         wa[k] = b.wa;
                         wb[k] = b.wb;
@@ -1457,11 +1594,11 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
 #define CF_PATH_KERNEL_ATTR
 #endif
 
-template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool NBUF, bool ITEM>
+template <typename R, int MODEL, int PRD, int ADJ, bool LEAN, bool MRG, bool ITEM>
 __global__ void __launch_bounds__(book_block_limit(PRD, int(sizeof(R)), ADJ)) CF_PATH_KERNEL_ATTR
 path_kernel(const LaunchParams P)
 {
-    path_body<R, MODEL, PRD, ADJ, LEAN, NBUF, ITEM>(P, int(blockIdx.x), int(gridDim.x));
+    path_body<R, MODEL, PRD, ADJ, LEAN, MRG, ITEM>(P, int(blockIdx.x), int(gridDim.x));
 }
 
 // Several calls of one kernel in one launch: blocks [first[r], first[r + 1])

@@ -347,7 +347,6 @@ void launch(Server& S, Slot& s, Request& r)
     B.ivbar = SH.ivn ? hout_dev + SH.stride : nullptr;
     B.weights = reinterpret_cast<double*>(take(nb.weights));
     B.ckpt = take(nb.ckpt);
-    B.normals = take(nb.normals);
     // host staging: tables then weights, one upload each
     const size_t tb = PR.staged.size();
     unsigned char* hin = static_cast<unsigned char*>(s.pin_in.ensure(tb + nb.weights + 8));
@@ -458,7 +457,6 @@ void launch_pack(Server& S, Slot& s)
         B.ivbar_blocks = reinterpret_cast<double*>(take(nb.ivbar_blocks));
         (void)take(nb.ivbar);
         B.ckpt = take(nb.ckpt);
-        B.normals = nullptr;
         r.out_off = o_off;
         B.reduced = hout_dev + o_off;
         B.ivbar = SH.ivn ? B.reduced + SH.stride : nullptr;

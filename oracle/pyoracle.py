@@ -214,6 +214,8 @@ def aad(mdl, prd, n, risk_index=0, weights=None, mrg=None):
     agg = np.zeros(n)
     risks = np.zeros(npar)
     w = _arr(weights) if weights is not None else None
+    if w is not None and w.size != npay:
+        raise ValueError("weights must have one entry per payoff (%d), got %d" % (npay, w.size))
     s1, s2 = mrg if mrg else (0, 0)
     _check(lib().or_aad_rng(ctypes.byref(mdl.d), ctypes.byref(prd.d), n, risk_index,
                             w.ctypes.data_as(_dp) if w is not None else None, int(mrg is not None), s1, s2,

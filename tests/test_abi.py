@@ -186,3 +186,16 @@ def test_numerical_parameters_are_validated_before_the_device(kw, code, msg):
     assert e.value.code == code
     with pytest.raises(cf.CfError, match=msg):
         cf.mcParallelSimulAAD(prd, mdl, num=cf.NumericalParam(**kw))
+
+
+def test_aggregation_weights_need_one_entry_per_payoff():
+    """The barrier has two payoffs (the option and its European): a weight
+    vector of another length is an error in the binding and in the oracle,
+    checked before anything reaches the device."""
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    assert len(prd.payoffLabels()) == 2
+    for w in ([1.0], [1.0, 0.0, 0.0]):
+        with pytest.raises(ValueError, match="one entry per payoff"):
+            cf.mcParallelSimulAAD(prd, mdl, 64, weights=w)
+        with pytest.raises(ValueError, match="one entry per payoff"):
+            po.aad(po.Model(**W.dupire_args(30)), po.Product(**W.barrier_args()), 8, weights=w)
