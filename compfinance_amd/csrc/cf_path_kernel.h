@@ -302,10 +302,10 @@ struct Lane {
         y[0] = y[1]; y[1] = y[2]; y[2] = yp;
         --pos;
     }
-    // Normal of draw i (number i + 1 of the block), reached from the current
-    // position by stepping forward or back, or forward from the start when
-    // that is shorter.  FP64 inverse normal of the reference (gaussians.h).
-    __device__ __forceinline__ double normal(int i)
+    // Draw i is number i + 1 of the block, reached from the current position
+    // by stepping forward or back, or forward from the start when that is
+    // shorter.
+    __device__ __forceinline__ void seek(int i)
     {
         const int t = i + 1;
         if (t < pos - t) {
@@ -315,9 +315,25 @@ struct Lane {
         }
         while (pos < t) forward();
         while (pos > t) back();
+    }
+    // FP64: the reference's uniform and inverse normal (gaussians.h:47-87)
+    __device__ __forceinline__ double normal(int i)
+    {
+        seek(i);
         const double diff = double(x[0]) - double(y[0]);
         const double u = x[0] > y[0] ? diff / 4294967088.0 : (diff + 4294967087.0) / 4294967088.0;
         const double z = inv_normal_p(u);
+        return neg ? -z : z;
+    }
+    // FP32: the uniform d / (m1 + 1), d in [1, m1], as a word of 32 bits,
+    // w = d + floor(208 d / 2^32) in [1, 2^32 - 2] (within one unit of
+    // 2^32 u), for the FP32 inverse normal of Sobol words
+    __device__ __forceinline__ float normal_f32(int i, const float* tabm)
+    {
+        seek(i);
+        const uint32_t d = x[0] > y[0] ? x[0] - y[0] : x[0] - y[0] + kM1;
+        const uint32_t w = d + uint32_t((uint64_t(d) * 208u) >> 32);
+        const float z = inv_normal(w, tabm);
         return neg ? -z : z;
     }
 };
@@ -768,7 +784,10 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
         else invn_tab = invn;
         invn_tab -= 4 * kInvNormalRowBias;
         auto draw = [&](int i) -> R {
-            if constexpr (MRG) return R(gen.normal(i));
+            if constexpr (MRG) {
+                if constexpr (sizeof(R) == 4) return gen.normal_f32(i, invn_tab);
+                else return gen.normal(i);
+            }
             else return inv_normal(hl[i] ^ low_word(i), invn_tab);
         };
 

@@ -73,7 +73,7 @@ def test_gpu_mrg_matches_oracle(engine, case):
         r = cf.mcParallelSimulAAD(prd, mdl, weights=w, num=n2)
         _, _, risks = po.aad(om, op, 800, weights=w, mrg=seeds)
         assert np.abs(r.risks - risks).max() <= 1e-10 * np.abs(risks).max()
-    # ranges starting at even paths reproduce the full run
+    # path ranges reproduce the full run (the last one starts at an odd path)
     full = cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=640, useSobol=False, seed1=seeds[0],
                                                                seed2=seeds[1])).payoffs
     parts = [cf.mcParallelSimul(prd, mdl, num=cf.NumericalParam(numPath=c, firstPath=f, useSobol=False,
@@ -183,3 +183,18 @@ def test_gpu_mrg_fp32_risks_follow_fp64(engine, mode):
     r32 = cf.mcParallelSimulAAD(prd, mdl, weights=[1.0, 0.0], num=cf.NumericalParam(precision=cf.FP32, **kw))
     assert abs(r32.agg_mean - r64.agg_mean) <= 1e-4 * abs(r64.agg_mean)
     assert np.abs(r32.risks - r64.risks).max() <= 2e-3 * np.abs(r64.risks).max()
+
+
+def test_fp32_word_of_the_mrg_uniform():
+    """The FP32 kernels map the uniform d / (m1 + 1), d in [1, m1], to the
+    word w = d + floor(208 d / 2^32) of the FP32 inverse normal: within one
+    unit of 2^32 u (exact rationals), never 0 (outside the table) and never
+    past 2^32 - 1, at both ends and on random d."""
+    from fractions import Fraction
+    m1 = 4294967087
+    rng = np.random.default_rng(5)
+    ds = [1, 2, 208, 20650, 2 ** 31, m1 - 1, m1] + [int(v) for v in rng.integers(1, m1 + 1, 2000)]
+    for d in ds:
+        w = d + ((d * 208) >> 32)
+        assert 1 <= w <= 2 ** 32 - 2
+        assert abs(w - Fraction(d * 2 ** 32, m1 + 1)) <= 1
