@@ -1093,6 +1093,7 @@ cfk::LaunchParams launch_params(const Engine& E, const Prepared& PR, const CallS
     }
     P.ivbar_copies = PR.ivbar_copies;
     P.iv_recover = PR.iv_recover ? 1 : 0;
+    P.rev_point = md.n_steps;
     return P;
 }
 
@@ -1439,6 +1440,40 @@ int cf_mc_simul_aad(const cf_model* mdl, const cf_product* prd, const cf_num* nu
 
 namespace cfh {
 
+// Time of the last observation of each payoff column, in column order: a
+// European of a Europeans product at its maturity, any other payoff at its
+// product's last date (its value depends on no later state).
+static void column_last_times(const ProductPlan& d, vector<double>& out)
+{
+    if (d.kind == CF_PRODUCT_EUROPEANS) {
+        for (size_t s = 0; s + 1 < d.pay_first.size(); ++s)
+            for (int c = d.pay_first[s]; c < d.pay_first[s + 1]; ++c) out.push_back(d.timeline[s]);
+    } else if (d.kind == CF_PRODUCT_BOOK) {
+        for (const ProductPlan& q : d.parts) column_last_times(q, out);
+    } else {
+        for (int c = 0; c < d.n_pay; ++c) out.push_back(d.timeline.back());
+    }
+}
+
+// The simulation timeline point of each payoff column's last observation
+// (n_steps where it is not found, which sweeps every step).
+static vector<int> payoff_last_points(const Prepared& PR)
+{
+    vector<double> t;
+    column_last_times(PR.pd, t);
+    const vector<double>& tl = PR.pd.timeline;
+    const vector<int>& ss = PR.md.step_sample;
+    vector<int> pts(t.size(), PR.md.n_steps);
+    for (size_t c = 0; c < t.size(); ++c) {
+        const auto it = std::lower_bound(tl.begin(), tl.end(), t[c]);
+        if (it == tl.end() || *it != t[c]) continue;
+        const int s = int(it - tl.begin());
+        for (int ip = 0; ip < int(ss.size()); ++ip)
+            if (ss[size_t(ip)] == s) pts[c] = std::min(ip, PR.md.n_steps);
+    }
+    return pts;
+}
+
 // Itemized risk over shared paths (the reference's single sweep with vector
 // adjoints, mcBase.h:858-989): one forward-only launch keeps every block of 64
 // paths' checkpoints and end states, then one reverse-only launch per payoff
@@ -1487,11 +1522,14 @@ bool run_multi(const cf_model* mdl, const cf_product* prd, const cf_num* num, do
     const int npar = PR.md.kind == CF_MODEL_DUPIRE ? 1 + PR.md.n_spots * PR.md.n_times : 4;
     vector<double> w(static_cast<size_t>(npay), 0.0), red(static_cast<size_t>(S.stride) + S.ivn),
         col(static_cast<size_t>(npar)), agg(1);
+    // each payoff's reverse starts at its last observation
+    const vector<int> last_pt = payoff_last_points(PR);
     auto launch = [&](int mode, int k) {
         std::fill(w.begin(), w.end(), 0.0);
         w[size_t(k)] = 1.0;
         hip_check(hipMemcpyAsync(B.weights, w.data(), size_t(npay) * 8, hipMemcpyHostToDevice, E.stream), "upload");
         P.ck_mode = mode;
+        P.rev_point = mode == 2 ? last_pt[size_t(k)] : PR.md.n_steps;
         Sequence q;
         make_sequence(PR, S, B, P, &q);
         q.cfg.item = true;

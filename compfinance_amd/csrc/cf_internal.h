@@ -177,6 +177,8 @@ struct LaunchParams {
     // checkpoint slot per block of 64 Sobol indices kept and each path's end
     // state written to `state`; 2 reverse only, from those slots and states
     int ck_mode;
+    int rev_point;               // the reverse starts at this timeline point's step (n_steps: all;
+                                 // itemized risk: the one payoff's last observation)
     void* state;                 // [blocks of 64 paths][state_rows(product)][64] reals
 };
 

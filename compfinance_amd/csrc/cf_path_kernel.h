@@ -1513,7 +1513,12 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     for (int k = 0; k < U; ++k) { buf[k] = nbuf[k]; lbuf[k] = nlbuf[k]; }
                 };
                 if (n > 0) {
-                    int top = VG ? (((n - 1) >> 2) << 2) + 3 : n - 1;
+                    // the sweep starts at the last point whose sample carries
+                    // weight (one payoff's reverse of itemized risk: later
+                    // steps have no adjoint), the next state from its checkpoint
+                    const int last = min(n - 1, P.rev_point);
+                    int top = VG ? ((last >> 2) << 2) + 3 : last;
+                    if (top + 1 < n) xnext = ck[ck_el(top + 1)];
                     load_group(top, buf, lbuf);
                     if (VG && top >= n) {
                         group(top, std::true_type{});
@@ -1561,7 +1566,8 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     const int s = step_sample[0];
                     if (s >= 0) accumulate(s, spot);
                 }
-                for (int i = 0; i < n; ++i) {
+                const int nrev = min(n, P.rev_point);   // samples after it carry no weight
+                for (int i = 0; i < nrev; ++i) {
                     R g;
                     if constexpr (RECOMP) g = draw(i);   // regenerated
                     else g = ck[i * kWave + lane];

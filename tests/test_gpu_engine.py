@@ -136,6 +136,7 @@ def test_repeat_calls_are_deterministic(engine):
 
 
 @pytest.mark.parametrize("mname,pk", [("lv30", W.barrier_args()),
+                                      ("lv30", dict(maturities=[0.5, 1.0, 1.0, 2.5, 0.27], strikes=[100.0, 95.0, 110.0, 100.0, 102.0])),
                                       ("bs", dict(maturities=[1.0, 2.0, 2.0], strikes=[95.0, 100.0, 110.0]))])
 def test_itemized_risks_match_oracle_per_payoff(engine, mname, pk):
     """mcParallelSimulAADMulti / AADriskMulti: column k is the risk of payoff k,
@@ -164,6 +165,26 @@ def test_itemized_risks_match_oracle_per_payoff(engine, mname, pk):
     np.testing.assert_allclose(rep["risks"], risks, rtol=1e-12, atol=1e-12 * np.abs(risks).max())
     assert rep["payoffs"] == prd.payoffLabels()
     cf.clearStore()
+
+
+@pytest.mark.parametrize("prec", [cf.FP32, cf.FP64])
+def test_itemized_reverse_starts_at_each_maturity(engine, prec):
+    """Each payoff's reverse sweep of the itemized risk starts at its own last
+    observation: Europeans maturing at timeline points of every residue mod 4
+    (the FP32 reverse works aligned groups of four steps) equal the one-hot
+    runs, which sweep every step."""
+    mdl = cf.Dupire(**W.dupire_args(30))
+    mats = [0.25, 0.27, 0.29, 0.31, 1.0, 1.02, 2.0, 2.98, 3.0]
+    prd = cf.Europeans(mats, [100.0 + 2.0 * i for i in range(len(mats))])
+    num = cf.NumericalParam(numPath=2000, precision=prec)
+    means, risks = cf.mcParallelSimulAADMulti(prd, mdl, num=num)
+    npay = len(prd.payoffLabels())
+    for k in range(npay):
+        w = np.zeros(npay)
+        w[k] = 1.0
+        one = cf.mcParallelSimulAAD(prd, mdl, weights=w, num=num)
+        tol = (1e-12 if prec == cf.FP64 else 1e-6) * max(np.abs(one.risks).max(), 1e-300)
+        assert np.abs(risks[:, k] - one.risks).max() <= tol, (k, np.abs(risks[:, k] - one.risks).max())
 
 
 def test_repeated_and_changed_inputs_between_calls(engine):
