@@ -241,23 +241,27 @@ def test_barrier_edges_with_a_frozen_spot(engine, prec, rtol):
     assert pay(150.1, 1e-8)[0] == 0.0
 
 
+@pytest.mark.parametrize("rng", ["sobol", "mrg"])
 @pytest.mark.parametrize("n", [512, 1000])
 @pytest.mark.parametrize("mode", [cf.AAD_CHECKPOINT, cf.AAD_RECOMPUTE])
 @pytest.mark.parametrize("prec", [cf.FP64, cf.FP32])
-def test_largest_serving_grid_fits(engine, prec, mode, n):
+def test_largest_serving_grid_fits(engine, prec, mode, n, rng):
     """The largest grid and timeline of the serving mix, 60 spots x 60 months
     with 5 years of weekly monitoring (260 steps, the LEAN layout), runs in
-    both precisions and adjoint variants; FP64 matches the oracle tape.
-    1000 paths from path 0 do not merge the last block of 64 Sobol indices,
-    so lane 0 of the first block (Sobol index 0, no path) stays idle and
-    draws through the global inverse-normal table."""
+    both precisions and adjoint variants, with Sobol and with mrg32k3a (whose
+    FP32 draws read the global inverse-normal table in this layout); FP64
+    matches the oracle tape.  1000 paths from path 0 do not merge the last
+    block of 64 Sobol indices, so lane 0 of the first block (Sobol index 0,
+    no path) stays idle and draws through the global inverse-normal table."""
     mk = W.dupire_args(60, 60)
     pk = W.barrier_args(maturity=5.0)
     mdl, prd = cf.Dupire(**mk), cf.UOC(**pk)
+    seeds = (31, 4159) if rng == "mrg" else None
+    kw = dict(useSobol=False, seed1=seeds[0], seed2=seeds[1]) if seeds else {}
     res = cf.mcParallelSimulAAD(prd, mdl, n, weights=[1.0, 0.0], per_path=True,
-                                num=cf.NumericalParam(numPath=n, precision=prec, aadMode=mode))
+                                num=cf.NumericalParam(numPath=n, precision=prec, aadMode=mode, **kw))
     assert cf.last_kernel_stats()["lean_layout"] == 1
-    pay, agg, risks = po.aad(po.Model(**mk), po.Product(**pk), n, 0)
+    pay, agg, risks = po.aad(po.Model(**mk), po.Product(**pk), n, 0, mrg=seeds)
     scale = max(np.abs(risks).max(), 1e-300)
     if prec == cf.FP64:
         np.testing.assert_allclose(res.payoffs, pay, rtol=1e-11, atol=1e-11)
