@@ -18,8 +18,9 @@ LIB = os.path.join(HERE, "libcfamd.so")
 SOURCES = ([("cf_kern_inst.hip", "cf_kern_r%d_a%d_b%d" % (r, a, b),
              ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_SRC=%d" % b])
             for r in (0, 1) for a in (0, 1, 2) for b in (0, 1)] +
-           [("cf_kern_inst.hip", "cf_kern_r%d_a1_b2" % r, ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=1", "-DCF_INST_SRC=2"])
-            for r in (0, 1)] +
+           [("cf_kern_inst.hip", "cf_kern_r%d_a1_b%d" % (r, b),
+             ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=1", "-DCF_INST_SRC=%d" % b])
+            for r in (0, 1) for b in (2, 4)] +
            [("cf_kern_inst.hip", "cf_kern_r%d_a%d_b3" % (r, a),
              ["-DCF_INST_REAL=%d" % r, "-DCF_INST_ADJ=%d" % a, "-DCF_INST_SRC=3"])
             for r in (0, 1) for a in (0, 1, 2)] +

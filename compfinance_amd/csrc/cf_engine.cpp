@@ -1571,7 +1571,7 @@ int cf_mc_simul_aad_multi(const cf_model* mdl, const cf_product* prd, const cf_n
         if (!risks) fail(CF_ERR_ARG, "null risks");
         validate_num(num);
         validate_model(mdl);
-        if (num->aad_mode == CF_AAD_CHECKPOINT && num->use_sobol && run_multi(mdl, prd, num, payoff_means, risks))
+        if (num->aad_mode == CF_AAD_CHECKPOINT && run_multi(mdl, prd, num, payoff_means, risks))
             return mdl->kind == CF_MODEL_BLACK_SCHOLES ? 4 : 1 + mdl->n_spots * mdl->n_times;
         const int npay = plan_product(prd).n_pay;
         validate_model(mdl);

@@ -414,7 +414,7 @@ struct KernelConfig {
 };
 
 // Path kernel families (precision r: 0 FP64, 1 FP32; adjoint a: AdjointKind;
-// b: 0 Sobol, 1 mrg32k3a, 2 itemized-risk checkpoint kernels, 3 Sobol
+// b: 0 Sobol, 1 mrg32k3a, 2 itemized-risk checkpoint kernels (4 with mrg32k3a), 3 Sobol
 // kernels of packed launches), one per build of cf_kern_inst.hip: kernel for
 // a model, product and layout.
 #define CF_PICK_CAT(r, a, b) pick_r##r##_a##a##_b##b
@@ -431,6 +431,8 @@ CF_PICK_DECL(1, 2)
 #undef CF_PICK_DECL
 const void* pick_r0_a1_b2(int model, int prd, bool lean);
 const void* pick_r1_a1_b2(int model, int prd, bool lean);
+const void* pick_r0_a1_b4(int model, int prd, bool lean);
+const void* pick_r1_a1_b4(int model, int prd, bool lean);
 #define CF_PICK_DECL_PACKED(r)                                                                                       \
     const void* pick_r##r##_a0_b3(int model, int prd, bool lean);                                                  \
     const void* pick_r##r##_a1_b3(int model, int prd, bool lean);                                                  \

@@ -1,15 +1,16 @@
 // One (precision, adjoint, normals source) family of path kernels,
 // instantiated for every model, product and LDS layout.  Built once per
 // family with -DCF_INST_REAL=0|1 (FP64, FP32), -DCF_INST_ADJ=0|1|2
-// (cfk::AdjointKind) and -DCF_INST_SRC=0|1|2|3 (Sobol, mrg32k3a, Sobol
+// (cfk::AdjointKind) and -DCF_INST_SRC=0|1|2|3|4 (Sobol, mrg32k3a, Sobol
 // with the itemized-risk modes: checkpoint family only, Sobol kernels of
-// packed launches), so the families compile in parallel.
+// packed launches, mrg32k3a with the itemized-risk modes), so the families
+// compile in parallel.
 #include <type_traits>
 
 #if !defined(CF_INST_REAL) || !defined(CF_INST_ADJ) || !defined(CF_INST_SRC)
 #error "CF_INST_REAL, CF_INST_ADJ and CF_INST_SRC must be defined"
 #endif
-#if CF_INST_SRC == 2 && CF_INST_ADJ != 1
+#if (CF_INST_SRC == 2 || CF_INST_SRC == 4) && CF_INST_ADJ != 1
 #error "the itemized-risk kernels are checkpoint kernels"
 #endif
 
@@ -31,9 +32,9 @@ constexpr int kAdj = CF_INST_ADJ;
 // modes (forward only keeping end states, reverse only from them), 3 Sobol
 // kernels of packed launches (path_kernel_packed: the same body; with
 // multiply-adds fused only within an expression it rounds as the single
-// kernel does)
-constexpr bool kMrg = CF_INST_SRC == 1;
-constexpr bool kItem = CF_INST_SRC == 2;
+// kernel does), 4 mrg32k3a with the itemized-risk modes
+constexpr bool kMrg = CF_INST_SRC == 1 || CF_INST_SRC == 4;
+constexpr bool kItem = CF_INST_SRC == 2 || CF_INST_SRC == 4;
 constexpr bool kPacked = CF_INST_SRC == 3;
 
 template <int MODEL, int PRD>

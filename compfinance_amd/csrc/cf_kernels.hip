@@ -178,7 +178,8 @@ static const void* pick(const KernelConfig& c)
         {{pick_r1_a0_b0, pick_r1_a0_b1}, {pick_r1_a1_b0, pick_r1_a1_b1}, {pick_r1_a2_b0, pick_r1_a2_b1}}};
     const int adj = !c.adjoint ? ADJ_NONE : c.recompute ? ADJ_RECOMPUTE : ADJ_CHECKPOINT;
     if (c.item) {
-        if (adj != ADJ_CHECKPOINT || c.mrg || c.packed) return nullptr;
+        if (adj != ADJ_CHECKPOINT || c.packed) return nullptr;
+        if (c.mrg) return (c.precision == 0 ? pick_r0_a1_b4 : pick_r1_a1_b4)(c.model, c.product, c.lean);
         return (c.precision == 0 ? pick_r0_a1_b2 : pick_r1_a1_b2)(c.model, c.product, c.lean);
     }
     if (c.packed) {

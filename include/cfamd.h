@@ -173,7 +173,9 @@ int cf_mc_simul_aad_noalloc(const cf_model* mdl, const cf_product* prd, const cf
 
 /* mcParallelSimulAADMulti (mcBase.h:858-989): the risk of every payoff
  * separately, risks[n_param][n_pay] = d E[payoff_k] / d parameter_i, on the
- * same paths; one adjoint sweep per payoff.  Returns n_param. */
+ * same paths; one forward sweep (checkpoint adjoint, when its kept states fit
+ * the free device memory) and one reverse sweep per payoff, else one adjoint
+ * call per payoff.  Returns n_param. */
 int cf_mc_simul_aad_multi(const cf_model* mdl, const cf_product* prd, const cf_num* num,
                           double* payoff_means, double* risks);
 

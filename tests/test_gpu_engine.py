@@ -138,20 +138,26 @@ def test_repeat_calls_are_deterministic(engine):
 @pytest.mark.parametrize("mname,pk", [("lv30", W.barrier_args()),
                                       ("lv30", dict(maturities=[0.5, 1.0, 1.0, 2.5, 0.27], strikes=[100.0, 95.0, 110.0, 100.0, 102.0])),
                                       ("bs", dict(maturities=[1.0, 2.0, 2.0], strikes=[95.0, 100.0, 110.0]))])
-def test_itemized_risks_match_oracle_per_payoff(engine, mname, pk):
+@pytest.mark.parametrize("rng", ["sobol", "mrg"])
+def test_itemized_risks_match_oracle_per_payoff(engine, mname, pk, rng):
     """mcParallelSimulAADMulti / AADriskMulti: column k is the risk of payoff k,
-    equal to the oracle tape's risk of that payoff alone."""
+    equal to the oracle tape's risk of that payoff alone (Sobol, and mrg32k3a
+    whose itemized kernels regenerate the draws in the reverse-only launches)."""
     mk = W.dupire_args(30) if mname == "lv30" else dict(spot=100.0, vol=0.25, rate=0.03, div=0.01)
     mdl = cf.Dupire(**mk) if "spots" in mk else cf.BlackScholes(**mk)
     prd = cf.UOC(**pk) if "barrier" in pk else cf.Europeans(**pk)
     om, op = po.Model(**mk), po.Product(**pk)
     n = 1000
-    means, risks = cf.mcParallelSimulAADMulti(prd, mdl, n)
+    seeds = (777, 999) if rng == "mrg" else None
+    num = cf.NumericalParam(numPath=n, useSobol=seeds is None, **(dict(seed1=seeds[0], seed2=seeds[1]) if seeds else {}))
+    means, risks = cf.mcParallelSimulAADMulti(prd, mdl, num=num)
     assert risks.shape == (mdl.numParams(), len(prd.payoffLabels()))
     for k in range(risks.shape[1]):
-        _, _, ref = po.aad(om, op, n, k)
+        _, _, ref = po.aad(om, op, n, k, mrg=seeds)
         assert np.abs(risks[:, k] - ref).max() <= 1e-10 * max(np.abs(ref).max(), 1e-300)
-    np.testing.assert_allclose(means, po.value_paths(om, op, n).mean(0), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(means, po.value_paths(om, op, n, mrg=seeds).mean(0), rtol=1e-12, atol=1e-14)
+    if seeds:
+        return
     cf.clearStore()
     if "spots" in mk:
         cf.putDupire(**mk, store="m")

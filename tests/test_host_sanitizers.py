@@ -15,9 +15,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "compfinance_amd", "csrc")
 BUILD = os.path.join(ROOT, "compfinance_amd", "_build")
 HOST_SOURCES = ["cf_engine.cpp", "cf_store.cpp", "cf_calib.cpp", "cf_server.cpp"]
-KERNEL_OBJECTS = (["cf_kern_r%d_a%d_b%d.o" % (r, a, b) for r in (0, 1) for a in (0, 1, 2) for b in (0, 1)] +
-                  ["cf_kern_r%d_a1_b2.o" % r for r in (0, 1)] +
-                  ["cf_kern_r%d_a%d_b3.o" % (r, a) for r in (0, 1) for a in (0, 1, 2)] + ["cf_kernels.o", "cf_noalloc.o"])
+# the device-code objects of the regular build (every .hip unit it compiles)
+from compfinance_amd import build as _build  # noqa: E402
+KERNEL_OBJECTS = [o + ".o" for src, o, _ in _build.SOURCES if src.endswith(".hip")]
 SAN = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
        "-Xarch_host", "-fno-sanitize-recover=all"]
 
