@@ -275,3 +275,41 @@ def test_largest_serving_grid_fits(engine, prec, mode, n, rng):
     else:
         assert abs(res.agg_mean - agg.mean()) <= 1e-4 * max(abs(agg.mean()), 1e-12)
         assert np.abs(res.risks - risks).max() <= 2e-3 * scale
+
+
+@pytest.mark.parametrize("prec", [cf.FP64, cf.FP32])
+def test_flat_grid_vegas_sum_to_black_scholes_vega_at_full_size(engine, prec):
+    """A flat 30 x 36 local-vol grid is Black-Scholes, step by step (the
+    log-Euler step is exact for a constant vol): on C3's 2^20 paths and
+    weekly timeline the barrier's delta equals the Black-Scholes model's on
+    the same paths, and the 1,080 local-vol risks sum to its vega (a parallel
+    shift of every node); the European's against the closed form within the
+    Monte Carlo error."""
+    import math
+    from scipy.stats import norm
+    n = W.C3_PATHS
+    spots = 50.0 + 150.0 * np.arange(30) / 29
+    times = (np.arange(36) + 1) / 12.0
+    flat = cf.Dupire(spot=100.0, spots=spots, times=times, vols=np.full((30, 36), 0.2), maxDt=1.0 / 52)
+    bs = cf.BlackScholes(spot=100.0, vol=0.2)
+    prd = cf.UOC(**W.barrier_args(strike=110.0, barrier=140.0, maturity=3.0))
+    num = cf.NumericalParam(numPath=n, precision=prec)
+    rd = cf.mcParallelSimulAAD(prd, flat, weights=[1.0, 0.0], num=num)
+    rb = cf.mcParallelSimulAAD(prd, bs, weights=[1.0, 0.0], num=num)
+    if prec == cf.FP64:
+        assert abs(rd.risks[0] - rb.risks[0]) <= 1e-9 * abs(rb.risks[0])
+        assert abs(rd.risks[1:].sum() - rb.risks[1]) <= 1e-9 * abs(rb.risks[1])
+    else:
+        # FP32: the two models round differently; against the largest risk
+        scale = np.abs(rb.risks).max()
+        assert abs(rd.risks[0] - rb.risks[0]) <= 2e-4 * scale
+        assert abs(rd.risks[1:].sum() - rb.risks[1]) <= 2e-4 * scale
+    # the vanilla against the closed form: price, delta, vega (risk of payoff 1)
+    eu = cf.mcParallelSimulAAD(prd, flat, weights=[0.0, 1.0], num=num)
+    T, K, v = 3.0, 110.0, 0.2
+    d1 = (math.log(100.0 / K) + 0.5 * v * v * T) / (v * math.sqrt(T))
+    price = 100.0 * norm.cdf(d1) - K * norm.cdf(d1 - v * math.sqrt(T))
+    assert abs(eu.payoff_means[1] - price) <= 2e-3 * price
+    assert abs(eu.risks[0] - norm.cdf(d1)) <= 2e-3 * norm.cdf(d1)
+    vega = 100.0 * norm.pdf(d1) * math.sqrt(T)
+    assert abs(eu.risks[1:].sum() - vega) <= 3e-3 * vega
