@@ -186,6 +186,11 @@ struct LaunchParams {
 // [first[r], first[r + 1]).  Passed by value (kernel arguments stay under
 // 4 KiB).
 constexpr int kMaxPack = 8;
+
+// Cross-block column sums: a block of the reduction takes kReduceCols
+// columns of one matrix (launch_reduce_rows, the packed reduction).
+constexpr int kReduceCols = 16;
+CF_HD inline int reduce_blocks(int cols) { return (cols + kReduceCols - 1) / kReduceCols; }
 struct PackArgs {
     LaunchParams p[kMaxPack];
     int first[kMaxPack + 1];

@@ -14,29 +14,34 @@ namespace cfk {
 
 // Column sums of two row-major matrices with the same row count,
 // [rows][cols0] and [rows][cols1] (cols1 may be 0), in one launch: blocks
-// [0, ceil(cols0 / 64)) take the first, the rest the second.  A block takes 64
-// columns; wave w sums rows w, w + 16, ... in order and the 16 partial sums
-// are added in wave order, so the result does not depend on scheduling.
+// [0, reduce_blocks(cols0)) take the first, the rest the second.  A block of
+// 1024 threads takes kReduceCols (16) columns, so a grid of local-vol
+// adjoints spreads over every compute unit; lane l of wave w works column
+// l % 16 and sums rows p, p + 64, ... in order (p = 4w + l / 16, the row
+// phase), and the 64 phase sums of a column are added in phase order, so the
+// result does not depend on scheduling.
 __device__ __forceinline__ void reduce_rows_block(const double* in0, int cols0, double* out0, const double* in1,
                                                   int cols1, double* out1, int rows, int bid)
 {
-    __shared__ double part[16][kWave];
+    static_assert(kReduceCols == 16, "four row phases of 16 columns per wave");
+    __shared__ double part[64][kReduceCols];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int b0 = (cols0 + kWave - 1) / kWave;
+    const int b0 = reduce_blocks(cols0);
     const bool first = bid < b0;
     const double* in = first ? in0 : in1;
     double* out = first ? out0 : out1;
     const int cols = first ? cols0 : cols1;
-    const int c = (first ? bid : bid - b0) * kWave + lane;
+    const int col = lane & (kReduceCols - 1), phase = 4 * w + (lane >> 4);
+    const int c = (first ? bid : bid - b0) * kReduceCols + col;
     double s = 0.0;
     if (c < cols)
-        for (int r = w; r < rows; r += 16) s += in[size_t(r) * cols + c];
-    part[w][lane] = s;
+        for (int r = phase; r < rows; r += 64) s += in[size_t(r) * cols + c];
+    part[phase][col] = s;
     __syncthreads();
-    if (w == 0 && c < cols) {
+    if (w == 0 && lane < kReduceCols && c < cols) {
         double t = 0.0;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) t += part[k][lane];
+#pragma unroll 16
+        for (int k = 0; k < 64; ++k) t += part[k][col];
         out[c] = t;
     }
 }
@@ -222,7 +227,7 @@ int launch_reduce_rows(const double* in0, int cols0, double* out0, const double*
                        int rows, void* stream)
 {
     const int threads = 1024;
-    const int blocks = (cols0 + kWave - 1) / kWave + (cols1 + kWave - 1) / kWave;
+    const int blocks = reduce_blocks(cols0) + reduce_blocks(cols1);
     if (blocks == 0) return 0;
     hipLaunchKernelGGL(reduce_rows_kernel, dim3(blocks), dim3(threads), 0, static_cast<hipStream_t>(stream), in0,
                        cols0, out0, in1, cols1, out1, rows);

@@ -471,8 +471,7 @@ void launch_pack(Server& S, Slot& s)
         RP.cols1[i] = int(SH.ivn);
         RP.out1[i] = B.ivbar;
         RP.rows[i] = SH.grid;
-        RP.first[i + 1] = RP.first[i] + (SH.stride + cfk::kWave - 1) / cfk::kWave +
-                          (int(SH.ivn) + cfk::kWave - 1) / cfk::kWave;
+        RP.first[i + 1] = RP.first[i] + cfk::reduce_blocks(SH.stride) + cfk::reduce_blocks(int(SH.ivn));
         r.packed = n;
     }
     if (b_off > S.arena.size_of(s.off)) fail(CF_ERR_CAPACITY, "packed block exceeds its arena allocation");
