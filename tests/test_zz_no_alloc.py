@@ -57,6 +57,21 @@ def test_noalloc_equals_the_allocating_entry_point_on_two_calls(engine):
     _check_equal(_noalloc(prd, mdl2, num2, w2, per_path=True), ref2, per_path=True)
 
 
+def test_noalloc_with_mrg32k3a(engine):
+    """mrg32k3a needs no scratch beyond Sobol's (the generators run in the
+    kernel): the benchmark risk run with mrg32k3a, then a smaller book call."""
+    mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
+    num = cf.NumericalParam(numPath=1 << 20, precision=cf.FP32, useSobol=False, seed1=17, seed2=23)
+    w = np.array([1.0, 0.0])
+    _check_equal(_noalloc(prd, mdl, num, w), cf.mcParallelSimulAAD(prd, mdl, weights=w, num=num))
+    cls = {"europeans": cf.Europeans, "barrier": cf.UOC, "asian": cf.Asian}
+    book = cf.Book([cls[k](**kw) for k, kw in W.c4_parts()])
+    wb = W.c4_notionals(len(book.payoffLabels()))
+    num2 = cf.NumericalParam(numPath=3000, precision=cf.FP64, useSobol=False, seed1=17, seed2=23)
+    ref2 = cf.mcParallelSimulAAD(book, mdl, weights=wb, num=num2, per_path=True)
+    _check_equal(_noalloc(book, mdl, num2, wb, per_path=True), ref2, per_path=True)
+
+
 def test_noalloc_refuses_a_call_beyond_its_scratch(engine):
     # FP64 checkpoints of 2^20 paths need more than the array holds
     mdl, prd = cf.Dupire(**W.dupire_args(30)), cf.UOC(**W.barrier_args())
