@@ -213,7 +213,7 @@ struct ReducePack {
 // Shared-memory carve of one block.  Offsets in bytes, 16-byte aligned.
 struct SmemLayout {
     size_t lo, hi, hi_tail, step_sample, invn, knots, bucket, iv, drift, stdv, samples, pay_first, pay_strike, acc,
-        ivbar, total;
+        tails, ivbar, total;
 };
 
 CF_HD inline size_t cf_align16(size_t x) { return (x + 15) & ~size_t(15); }
@@ -399,6 +399,8 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.pay_first = o;   o = cf_align16(o + (europeans ? size_t(n_samples + 1) * 4 : 0));
     L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
     L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
+    // FP64 FULL: one row of 64 doubles per wave for the inverse normal's tail draws
+    L.tails = o;       o = cf_align16(o + (real_bytes == 8 && !lean ? size_t(waves) * kWave * 8 : 0));
     L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * ivbar_copy_stride(n_steps, n_spots) * 8
                                                              : 0));
     L.total = o;

@@ -132,18 +132,18 @@ __device__ __forceinline__ double log_pos(double x)
 }
 
 // Beasley-Springer-Moro inverse normal (gaussians.h:47-87), FP64, same
-// branches and constants as the reference.
-__device__ __forceinline__ double inv_normal_p(double p)
+// branches and constants as the reference: the central rational form of
+// x = min(p, 1 - p) - 1/2 for |x| < 0.42, else the tail polynomial in
+// log(-log(min(p, 1 - p))); both halves are also evaluated apart
+// (inv_normal4_f64) and give the same values.
+__device__ __forceinline__ double moro_central(double x)
 {
-    const bool sup = p > 0.5;
-    const double up = sup ? 1.0 - p : p;
-    const double x = up - 0.5;
-    if (fabs(x) < 0.42) {
-        const double r = x * x;
-        const double z = x * (((-25.44106049637 * r + 41.39119773534) * r + -18.61500062529) * r + 2.50662823884) /
-                         ((((3.13082909833 * r + -21.06224101826) * r + 23.08336743743) * r + -8.47351093090) * r + 1.0);
-        return sup ? -z : z;
-    }
+    const double r = x * x;
+    return x * (((-25.44106049637 * r + 41.39119773534) * r + -18.61500062529) * r + 2.50662823884) /
+           ((((3.13082909833 * r + -21.06224101826) * r + 23.08336743743) * r + -8.47351093090) * r + 1.0);
+}
+__device__ __forceinline__ double moro_tail(double up)
+{
     double r = log_pos(-log_pos(up));
     r = 0.3374754822726147 +
         r * (0.9761690190917186 +
@@ -153,6 +153,18 @@ __device__ __forceinline__ double inv_normal_p(double p)
         r * (0.0003951896511919 +
         r * (0.0000321767881768 +
         r * (0.0000002888167364 + r * 0.0000003960315187)))))));
+    return r;
+}
+__device__ __forceinline__ double inv_normal_p(double p)
+{
+    const bool sup = p > 0.5;
+    const double up = sup ? 1.0 - p : p;
+    const double x = up - 0.5;
+    if (fabs(x) < 0.42) {
+        const double z = moro_central(x);
+        return sup ? -z : z;
+    }
+    const double r = moro_tail(up);
     return sup ? r : -r;
 }
 
@@ -191,6 +203,57 @@ __device__ __forceinline__ float inv_normal(uint32_t xi, const float* tabm)
 
 // FP64: the reference's function; the table pointer is unused
 __device__ __forceinline__ double inv_normal(uint32_t xi, const double*) { return inv_normal(xi, 0.0); }
+
+// Four FP64 normals of this lane's Sobol words, with Moro's tail evaluated
+// once per wave.  About 16% of draws need the tail (two logarithms and a
+// polynomial), so a wave of 64 lanes almost always has one that does and,
+// draw by draw, every lane would run it.  Here every lane runs the central
+// form of its four draws; the tail draws of the wave, counted by ballots, go
+// to `row` (this wave's 64 doubles of LDS) by their rank, each lane evaluates
+// at most one, and the owners read their values back (in rounds of 64 when
+// more draws need the tail).  The values equal inv_normal's, draw by draw.
+__device__ __forceinline__ void inv_normal4_f64(const uint32_t w[4], double z[4], double* row, int lane)
+{
+    double up[4];
+    bool sup[4], tail[4];
+    uint64_t m[4];
+    int base[4];
+    int total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double p = 2.3283064365387E-10 * double(w[k]);
+        sup[k] = p > 0.5;
+        up[k] = sup[k] ? 1.0 - p : p;
+        const double x = up[k] - 0.5;
+        tail[k] = !(fabs(x) < 0.42);
+        const double c = moro_central(x);
+        z[k] = sup[k] ? -c : c;
+        m[k] = __builtin_amdgcn_ballot_w64(tail[k]);
+        base[k] = total;
+        total += __builtin_popcountll(m[k]);
+    }
+    for (int r0 = 0; r0 < total; r0 += kWave) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int pos = base[k] +
+                            int(__builtin_amdgcn_mbcnt_hi(uint32_t(m[k] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m[k]), 0u))) - r0;
+            if (tail[k] && pos >= 0 && pos < kWave) row[pos] = up[k];
+        }
+        wave_sync_lds();
+        if (lane < total - r0) row[lane] = moro_tail(row[lane]);
+        wave_sync_lds();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int pos = base[k] +
+                            int(__builtin_amdgcn_mbcnt_hi(uint32_t(m[k] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m[k]), 0u))) - r0;
+            if (tail[k] && pos >= 0 && pos < kWave) {
+                const double t = row[pos];
+                z[k] = sup[k] ? t : -t;
+            }
+        }
+        wave_sync_lds();
+    }
+}
 
 // FP32 inverse-normal table: [rows][4 monomial coefficients]
 static __constant__ float kInvNormalF32[kInvNormalRows * 4] = {
@@ -586,6 +649,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     int* pay_first = reinterpret_cast<int*>(smem + L.pay_first);
     double* pay_strike = reinterpret_cast<double*>(smem + L.pay_strike);
     double* acc = reinterpret_cast<double*>(smem + L.acc);
+    double* tails = reinterpret_cast<double*>(smem + L.tails);   // FP64 FULL: [waves][64]
     double* ivbar = reinterpret_cast<double*>(smem + L.ivbar);
     R* invn = reinterpret_cast<R*>(smem + L.invn);   // FP32 inverse-normal table
 
@@ -1081,6 +1145,20 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     }
                 }
             }
+            // FP64 Sobol, FULL layout: four draws at a time with the inverse
+            // normal's tail once per wave (inv_normal4_f64), the steps one by
+            // one (not the book: its four steps' code measured slower there)
+            if constexpr (sizeof(R) == 8 && !MRG && !LEAN && !BOOK) {
+                double* trow = tails + wave * kWave;
+                for (; i + 4 <= n; i += 4) {
+                    uint32_t gw[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) gw[k] = hl[i + k] ^ low_word(i + k);
+                    inv_normal4_f64(gw, gz, trow, lane);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) fwd_step(i, k, std::false_type{}, std::true_type{});
+                }
+            }
             for (; i < n; ++i) fwd_step(i, 0, std::false_type{}, std::false_type{});
 
             // ---- payoffs of the barrier (known at the end of the path)
@@ -1320,11 +1398,19 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                             if (i0 + q4 < n) sobol_words4(lo_tab, hl, lane, i0 + q4, sw + q4);
                     }
                     // mrg32k3a: the segment's normals drawn top down, so the
-                    // generator only steps back over the reverse sweep
+                    // generator only steps back over the reverse sweep; FP64
+                    // Sobol: the segment's four draws with the inverse
+                    // normal's tail once per wave (steps from n on draw 1/2)
                     R mz[kSeg];
                     if constexpr (MRG) {
 #pragma unroll
                         for (int k = kSeg - 1; k >= 0; --k) mz[k] = i0 + k < n ? draw(i0 + k) : R(0);
+                    } else if constexpr (sizeof(R) == 8 && !LEAN) {
+                        static_assert(kSeg == 4, "FP64 segments of four draws");
+                        uint32_t gw[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) gw[k] = i0 + k < n ? (hl[i0 + k] ^ low_word(i0 + k)) : 0x80000000u;
+                        inv_normal4_f64(gw, mz, tails + wave * kWave, lane);
                     }
 #pragma unroll
                     for (int k = 0; k < kSeg; ++k) {
@@ -1337,7 +1423,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         if (i >= n) continue;
                         R g;
                         if constexpr (QUADS) g = inv_normal(sw[k], invn_tab);
-                        else if constexpr (MRG) g = mz[k];
+                        else if constexpr (MRG || (sizeof(R) == 8 && !LEAN)) g = mz[k];
                         else g = draw(i);
                         const Bracket<R> b = local_vol<LEAN>(ks, kxi, iv + i * mp, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol

@@ -56,7 +56,9 @@ static void kernel_extents(int n, int m, int ns, int npay, int waves, int rb, bo
     region(L.samples, dup ? 0 : size_t(3 * ns) * 8, L.pay_first, T);
     region(L.pay_first, eus ? size_t(ns + 1) * 4 : 0, L.pay_strike, T);
     region(L.pay_strike, eus ? size_t(npay) * 8 : 0, L.acc, T);
-    region(L.acc, size_t(waves) * out_stride(npay) * 8, L.ivbar, T);
+    region(L.acc, size_t(waves) * out_stride(npay) * 8, L.tails, T);
+    // FP64 FULL: tails[wave * 64 + l], l < 64 (the inverse normal's tail draws)
+    region(L.tails, rb == 8 && !lean ? size_t(waves) * kWave * 8 : 0, L.ivbar, T);
     // ivbar: copy c at c * ivbar_copy_stride, node (i, j + 1) with j <= m
     // (padded column m + 1), and the end columns 0 and m + 1 on write-out
     const size_t last = size_t(copies - 1) * ivbar_copy_stride(n, m) + size_t(n - 1) * mp + m + 1;
@@ -95,7 +97,7 @@ int main()
                                                  copies);
                 const size_t off[] = {L.lo, L.hi, L.hi_tail, L.step_sample, L.invn, L.knots, L.bucket, L.iv,
                                       L.drift, L.stdv, L.samples, L.pay_first, L.pay_strike, L.acc,
-                                      L.ivbar, L.total};
+                                      L.tails, L.ivbar, L.total};
                 for (size_t o : off) misaligned += (o % 16) ? 1 : 0;
             }
         }
