@@ -211,8 +211,8 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, const double*) { retur
 // form of its four draws; the tail draws of the wave, counted by ballots, go
 // to `row` (this wave's 64 doubles of LDS) by their rank, each lane evaluates
 // at most one, and the owners read their values back (in rounds of 64 when
-// more draws need the tail).  The values equal inv_normal's, draw by draw.
-__device__ __forceinline__ void inv_normal4_f64(const uint32_t w[4], double z[4], double* row, int lane)
+// more draws need the tail).  The values equal inv_normal_p's, draw by draw.
+__device__ __forceinline__ void inv_normal4_p(const double p[4], double z[4], double* row, int lane)
 {
     double up[4];
     bool sup[4], tail[4];
@@ -221,9 +221,8 @@ __device__ __forceinline__ void inv_normal4_f64(const uint32_t w[4], double z[4]
     int total = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const double p = 2.3283064365387E-10 * double(w[k]);
-        sup[k] = p > 0.5;
-        up[k] = sup[k] ? 1.0 - p : p;
+        sup[k] = p[k] > 0.5;
+        up[k] = sup[k] ? 1.0 - p[k] : p[k];
         const double x = up[k] - 0.5;
         tail[k] = !(fabs(x) < 0.42);
         const double c = moro_central(x);
@@ -253,6 +252,15 @@ __device__ __forceinline__ void inv_normal4_f64(const uint32_t w[4], double z[4]
         }
         wave_sync_lds();
     }
+}
+
+// Of Sobol words (sobol.h:26, :110-116)
+__device__ __forceinline__ void inv_normal4_f64(const uint32_t w[4], double z[4], double* row, int lane)
+{
+    double p[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) p[k] = 2.3283064365387E-10 * double(w[k]);
+    inv_normal4_p(p, z, row, lane);
 }
 
 // FP32 inverse-normal table: [rows][4 monomial coefficients]
@@ -379,13 +387,17 @@ struct Lane {
         while (pos < t) forward();
         while (pos > t) back();
     }
-    // FP64: the reference's uniform and inverse normal (gaussians.h:47-87)
-    __device__ __forceinline__ double normal(int i)
+    // FP64: the reference's uniform (mrg32k3a.h:84-86) and inverse normal
+    // (gaussians.h:47-87)
+    __device__ __forceinline__ double uniform(int i)
     {
         seek(i);
         const double diff = double(x[0]) - double(y[0]);
-        const double u = x[0] > y[0] ? diff / 4294967088.0 : (diff + 4294967087.0) / 4294967088.0;
-        const double z = inv_normal_p(u);
+        return x[0] > y[0] ? diff / 4294967088.0 : (diff + 4294967087.0) / 4294967088.0;
+    }
+    __device__ __forceinline__ double normal(int i)
+    {
+        const double z = inv_normal_p(uniform(i));
         return neg ? -z : z;
     }
     // FP32: the uniform d / (m1 + 1), d in [1, m1], as a word of 32 bits,
@@ -1145,16 +1157,27 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     }
                 }
             }
-            // FP64 Sobol, FULL layout: four draws at a time with the inverse
-            // normal's tail once per wave (inv_normal4_f64), the steps one by
+            // FP64, FULL layout: four draws at a time with the inverse
+            // normal's tail once per wave (inv_normal4_p), the steps one by
             // one (not the book: its four steps' code measured slower there)
-            if constexpr (sizeof(R) == 8 && !MRG && !LEAN && !BOOK) {
+            if constexpr (sizeof(R) == 8 && !LEAN && !BOOK) {
                 double* trow = tails + wave * kWave;
                 for (; i + 4 <= n; i += 4) {
-                    uint32_t gw[4];
+                    if constexpr (MRG) {
+                        double pu[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) gw[k] = hl[i + k] ^ low_word(i + k);
-                    inv_normal4_f64(gw, gz, trow, lane);
+                        for (int k = 0; k < 4; ++k) pu[k] = gen.uniform(i + k);
+                        inv_normal4_p(pu, gz, trow, lane);
+                        if (gen.neg) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) gz[k] = -gz[k];
+                        }
+                    } else {
+                        uint32_t gw[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) gw[k] = hl[i + k] ^ low_word(i + k);
+                        inv_normal4_f64(gw, gz, trow, lane);
+                    }
 #pragma unroll
                     for (int k = 0; k < 4; ++k) fwd_step(i, k, std::false_type{}, std::true_type{});
                 }
