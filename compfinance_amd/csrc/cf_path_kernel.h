@@ -1182,6 +1182,36 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     for (int k = 0; k < 4; ++k) fwd_step(i, k, std::false_type{}, std::true_type{});
                 }
             }
+            if constexpr (sizeof(R) == 8 && !LEAN && BOOK) {
+                // the book, FP64 FULL: the same four-draw batches, the steps
+                // in a loop that is not unrolled (each takes its draw by a
+                // select, so the book's step is compiled once)
+                double* trow = tails + wave * kWave;
+                double gb[4] = {0.0, 0.0, 0.0, 0.0};
+                const int n4 = n & ~3;
+                for (; i < n4; ++i) {
+                    const int k = i & 3;
+                    if (k == 0) {
+                        if constexpr (MRG) {
+                            double pu[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) pu[j] = gen.uniform(i + j);
+                            inv_normal4_p(pu, gb, trow, lane);
+                            if (gen.neg) {
+#pragma unroll
+                                for (int j = 0; j < 4; ++j) gb[j] = -gb[j];
+                            }
+                        } else {
+                            uint32_t gw[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) gw[j] = hl[i + j] ^ low_word(i + j);
+                            inv_normal4_f64(gw, gb, trow, lane);
+                        }
+                    }
+                    gz[0] = k == 0 ? gb[0] : k == 1 ? gb[1] : k == 2 ? gb[2] : gb[3];
+                    fwd_step(i, 0, std::false_type{}, std::true_type{});
+                }
+            }
             for (; i < n; ++i) fwd_step(i, 0, std::false_type{}, std::false_type{});
 
             // ---- payoffs of the barrier (known at the end of the path)
