@@ -1455,7 +1455,17 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     // Sobol: the segment's four draws with the inverse
                     // normal's tail once per wave (steps from n on draw 1/2)
                     R mz[kSeg];
-                    if constexpr (MRG) {
+                    if constexpr (MRG && sizeof(R) == 8 && !LEAN) {
+                        static_assert(kSeg == 4, "FP64 segments of four draws");
+                        double pu[4];
+#pragma unroll
+                        for (int k = 3; k >= 0; --k) pu[k] = i0 + k < n ? gen.uniform(i0 + k) : 0.5;
+                        inv_normal4_p(pu, mz, tails + wave * kWave, lane);
+                        if (gen.neg) {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) mz[k] = -mz[k];
+                        }
+                    } else if constexpr (MRG) {
 #pragma unroll
                         for (int k = kSeg - 1; k >= 0; --k) mz[k] = i0 + k < n ? draw(i0 + k) : R(0);
                     } else if constexpr (sizeof(R) == 8 && !LEAN) {
