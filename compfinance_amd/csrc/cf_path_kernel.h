@@ -18,6 +18,10 @@
 //     table and timeline sample map in LDS (the LEAN layout of long timelines
 //     and large grids reads the local-vol, knot search and inverse-normal
 //     tables from global memory through L1 instead);
+//   * FP64 draws four steps at a time, the inverse normal's tail form
+//     evaluated once per wave for the lanes that need it (a per-wave LDS row);
+//   * mrg32k3a instead of Sobol: one generator per lane in registers, jumped
+//     to its path's block of numbers and stepped back in the reverse sweeps;
 //   * payoffs reduced per wave with DPP cross-lane moves into per-wave LDS
 //     accumulators, then per block, then across blocks by a separate kernel
 //     in fixed order;
