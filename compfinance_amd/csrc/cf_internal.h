@@ -20,6 +20,9 @@ enum ProductKind { PRD_EUROPEAN = 0, PRD_BARRIER = 1, PRD_EUROPEANS = 2, PRD_BOO
 enum AdjointKind { ADJ_NONE = 0, ADJ_CHECKPOINT = 1, ADJ_RECOMPUTE = 2 };
 
 constexpr int kWave = 64;
+
+// Doubles per wave in the FP64 kernels' LDS row of inverse-normal tail draws.
+CF_HD constexpr int tail_row(bool lean) { return lean ? 32 : 64; }
 constexpr int kSobolDims = 1101;     // direction-number table stride
 
 // Recompute variant: the Dupire state is kept at every kSeg-th timeline
@@ -399,8 +402,9 @@ CF_HD inline SmemLayout smem_layout(int n_steps, int n_spots, int n_samples, int
     L.pay_first = o;   o = cf_align16(o + (europeans ? size_t(n_samples + 1) * 4 : 0));
     L.pay_strike = o;  o = cf_align16(o + (europeans ? size_t(n_pay) * 8 : 0));
     L.acc = o;         o = cf_align16(o + size_t(waves) * out_stride(n_pay) * 8);
-    // FP64 FULL: one row of 64 doubles per wave for the inverse normal's tail draws
-    L.tails = o;       o = cf_align16(o + (real_bytes == 8 && !lean ? size_t(waves) * kWave * 8 : 0));
+    // FP64: one row per wave for the inverse normal's tail draws, 64 doubles
+    // (FULL) or 32 (LEAN, beside the largest grids' tables)
+    L.tails = o;       o = cf_align16(o + (real_bytes == 8 ? size_t(waves) * tail_row(lean) * 8 : 0));
     L.ivbar = o;       o = cf_align16(o + (adjoint && dupire ? size_t(ivbar_copies) * ivbar_copy_stride(n_steps, n_spots) * 8
                                                              : 0));
     L.total = o;

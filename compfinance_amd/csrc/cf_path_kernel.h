@@ -19,7 +19,8 @@
 //     and large grids reads the local-vol, knot search and inverse-normal
 //     tables from global memory through L1 instead);
 //   * FP64 draws four steps at a time, the inverse normal's tail form
-//     evaluated once per wave for the lanes that need it (a per-wave LDS row);
+//     evaluated once per wave for the lanes that need it (a per-wave LDS row
+//     of 64 doubles, 32 in the LEAN layout);
 //   * mrg32k3a instead of Sobol: one generator per lane in registers, jumped
 //     to its path's block of numbers and stepped back in the reverse sweeps;
 //   * payoffs reduced per wave with DPP cross-lane moves into per-wave LDS
@@ -213,9 +214,10 @@ __device__ __forceinline__ double inv_normal(uint32_t xi, const double*) { retur
 // polynomial), so a wave of 64 lanes almost always has one that does and,
 // draw by draw, every lane would run it.  Here every lane runs the central
 // form of its four draws; the tail draws of the wave, counted by ballots, go
-// to `row` (this wave's 64 doubles of LDS) by their rank, each lane evaluates
-// at most one, and the owners read their values back (in rounds of 64 when
+// to `row` (this wave's ROW doubles of LDS) by their rank, each lane evaluates
+// at most one, and the owners read their values back (in rounds of ROW when
 // more draws need the tail).  The values equal inv_normal_p's, draw by draw.
+template <int ROW>
 __device__ __forceinline__ void inv_normal4_p(const double p[4], double z[4], double* row, int lane)
 {
     double up[4];
@@ -235,21 +237,22 @@ __device__ __forceinline__ void inv_normal4_p(const double p[4], double z[4], do
         base[k] = total;
         total += __builtin_popcountll(m[k]);
     }
-    for (int r0 = 0; r0 < total; r0 += kWave) {
+    static_assert(ROW >= 1 && ROW <= kWave, "a row of at most one double per lane");
+    for (int r0 = 0; r0 < total; r0 += ROW) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int pos = base[k] +
                             int(__builtin_amdgcn_mbcnt_hi(uint32_t(m[k] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m[k]), 0u))) - r0;
-            if (tail[k] && pos >= 0 && pos < kWave) row[pos] = up[k];
+            if (tail[k] && pos >= 0 && pos < ROW) row[pos] = up[k];
         }
         wave_sync_lds();
-        if (lane < total - r0) row[lane] = moro_tail(row[lane]);
+        if (lane < ROW && lane < total - r0) row[lane] = moro_tail(row[lane]);
         wave_sync_lds();
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int pos = base[k] +
                             int(__builtin_amdgcn_mbcnt_hi(uint32_t(m[k] >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m[k]), 0u))) - r0;
-            if (tail[k] && pos >= 0 && pos < kWave) {
+            if (tail[k] && pos >= 0 && pos < ROW) {
                 const double t = row[pos];
                 z[k] = sup[k] ? t : -t;
             }
@@ -259,12 +262,13 @@ __device__ __forceinline__ void inv_normal4_p(const double p[4], double z[4], do
 }
 
 // Of Sobol words (sobol.h:26, :110-116)
+template <int ROW>
 __device__ __forceinline__ void inv_normal4_f64(const uint32_t w[4], double z[4], double* row, int lane)
 {
     double p[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) p[k] = 2.3283064365387E-10 * double(w[k]);
-    inv_normal4_p(p, z, row, lane);
+    inv_normal4_p<ROW>(p, z, row, lane);
 }
 
 // FP32 inverse-normal table: [rows][4 monomial coefficients]
@@ -665,7 +669,8 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
     int* pay_first = reinterpret_cast<int*>(smem + L.pay_first);
     double* pay_strike = reinterpret_cast<double*>(smem + L.pay_strike);
     double* acc = reinterpret_cast<double*>(smem + L.acc);
-    double* tails = reinterpret_cast<double*>(smem + L.tails);   // FP64 FULL: [waves][64]
+    constexpr int TR = tail_row(LEAN);
+    double* tails = reinterpret_cast<double*>(smem + L.tails);   // FP64: [waves][TR]
     double* ivbar = reinterpret_cast<double*>(smem + L.ivbar);
     R* invn = reinterpret_cast<R*>(smem + L.invn);   // FP32 inverse-normal table
 
@@ -1161,17 +1166,17 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     }
                 }
             }
-            // FP64, FULL layout: four draws at a time with the inverse
-            // normal's tail once per wave (inv_normal4_p), the steps one by
-            // one (not the book: its four steps' code measured slower there)
-            if constexpr (sizeof(R) == 8 && !LEAN && !BOOK) {
-                double* trow = tails + wave * kWave;
+            // FP64: four draws at a time with the inverse normal's tail once
+            // per wave (inv_normal4_p), the steps one by one (not the book:
+            // its four steps' code measured slower there)
+            if constexpr (sizeof(R) == 8 && !BOOK) {
+                double* trow = tails + wave * TR;
                 for (; i + 4 <= n; i += 4) {
                     if constexpr (MRG) {
                         double pu[4];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) pu[k] = gen.uniform(i + k);
-                        inv_normal4_p(pu, gz, trow, lane);
+                        inv_normal4_p<TR>(pu, gz, trow, lane);
                         if (gen.neg) {
 #pragma unroll
                             for (int k = 0; k < 4; ++k) gz[k] = -gz[k];
@@ -1180,17 +1185,17 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         uint32_t gw[4];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) gw[k] = hl[i + k] ^ low_word(i + k);
-                        inv_normal4_f64(gw, gz, trow, lane);
+                        inv_normal4_f64<TR>(gw, gz, trow, lane);
                     }
 #pragma unroll
                     for (int k = 0; k < 4; ++k) fwd_step(i, k, std::false_type{}, std::true_type{});
                 }
             }
-            if constexpr (sizeof(R) == 8 && !LEAN && BOOK) {
-                // the book, FP64 FULL: the same four-draw batches, the steps
+            if constexpr (sizeof(R) == 8 && BOOK) {
+                // the book, FP64: the same four-draw batches, the steps
                 // in a loop that is not unrolled (each takes its draw by a
                 // select, so the book's step is compiled once)
-                double* trow = tails + wave * kWave;
+                double* trow = tails + wave * TR;
                 double gb[4] = {0.0, 0.0, 0.0, 0.0};
                 const int n4 = n & ~3;
                 for (; i < n4; ++i) {
@@ -1200,7 +1205,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                             double pu[4];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) pu[j] = gen.uniform(i + j);
-                            inv_normal4_p(pu, gb, trow, lane);
+                            inv_normal4_p<TR>(pu, gb, trow, lane);
                             if (gen.neg) {
 #pragma unroll
                                 for (int j = 0; j < 4; ++j) gb[j] = -gb[j];
@@ -1209,7 +1214,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                             uint32_t gw[4];
 #pragma unroll
                             for (int j = 0; j < 4; ++j) gw[j] = hl[i + j] ^ low_word(i + j);
-                            inv_normal4_f64(gw, gb, trow, lane);
+                            inv_normal4_f64<TR>(gw, gb, trow, lane);
                         }
                     }
                     gz[0] = k == 0 ? gb[0] : k == 1 ? gb[1] : k == 2 ? gb[2] : gb[3];
@@ -1459,12 +1464,12 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     // Sobol: the segment's four draws with the inverse
                     // normal's tail once per wave (steps from n on draw 1/2)
                     R mz[kSeg];
-                    if constexpr (MRG && sizeof(R) == 8 && !LEAN) {
+                    if constexpr (MRG && sizeof(R) == 8) {
                         static_assert(kSeg == 4, "FP64 segments of four draws");
                         double pu[4];
 #pragma unroll
                         for (int k = 3; k >= 0; --k) pu[k] = i0 + k < n ? gen.uniform(i0 + k) : 0.5;
-                        inv_normal4_p(pu, mz, tails + wave * kWave, lane);
+                        inv_normal4_p<TR>(pu, mz, tails + wave * TR, lane);
                         if (gen.neg) {
 #pragma unroll
                             for (int k = 0; k < 4; ++k) mz[k] = -mz[k];
@@ -1472,12 +1477,12 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                     } else if constexpr (MRG) {
 #pragma unroll
                         for (int k = kSeg - 1; k >= 0; --k) mz[k] = i0 + k < n ? draw(i0 + k) : R(0);
-                    } else if constexpr (sizeof(R) == 8 && !LEAN) {
+                    } else if constexpr (sizeof(R) == 8) {
                         static_assert(kSeg == 4, "FP64 segments of four draws");
                         uint32_t gw[4];
 #pragma unroll
                         for (int k = 0; k < 4; ++k) gw[k] = i0 + k < n ? (hl[i0 + k] ^ low_word(i0 + k)) : 0x80000000u;
-                        inv_normal4_f64(gw, mz, tails + wave * kWave, lane);
+                        inv_normal4_f64<TR>(gw, mz, tails + wave * TR, lane);
                     }
 #pragma unroll
                     for (int k = 0; k < kSeg; ++k) {
@@ -1490,7 +1495,7 @@ __device__ __forceinline__ void path_body(const LaunchParams& P, const int bid, 
                         if (i >= n) continue;
                         R g;
                         if constexpr (QUADS) g = inv_normal(sw[k], invn_tab);
-                        else if constexpr (MRG || (sizeof(R) == 8 && !LEAN)) g = mz[k];
+                        else if constexpr (MRG || sizeof(R) == 8) g = mz[k];
                         else g = draw(i);
                         const Bracket<R> b = local_vol<LEAN>(ks, kxi, iv + i * mp, xi);
                         const R ws = g - b.vol;    // d x_{i+1} / d vol

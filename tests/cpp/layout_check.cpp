@@ -57,8 +57,8 @@ static void kernel_extents(int n, int m, int ns, int npay, int waves, int rb, bo
     region(L.pay_first, eus ? size_t(ns + 1) * 4 : 0, L.pay_strike, T);
     region(L.pay_strike, eus ? size_t(npay) * 8 : 0, L.acc, T);
     region(L.acc, size_t(waves) * out_stride(npay) * 8, L.tails, T);
-    // FP64 FULL: tails[wave * 64 + l], l < 64 (the inverse normal's tail draws)
-    region(L.tails, rb == 8 && !lean ? size_t(waves) * kWave * 8 : 0, L.ivbar, T);
+    // FP64: tails[wave * row + l], l < row (the inverse normal's tail draws)
+    region(L.tails, rb == 8 ? size_t(waves) * tail_row(lean) * 8 : 0, L.ivbar, T);
     // ivbar: copy c at c * ivbar_copy_stride, node (i, j + 1) with j <= m
     // (padded column m + 1), and the end columns 0 and m + 1 on write-out
     const size_t last = size_t(copies - 1) * ivbar_copy_stride(n, m) + size_t(n - 1) * mp + m + 1;
